@@ -1,0 +1,150 @@
+/*
+ * pw.h -- C-ABI of libpocketwind, the MI355X (gfx950) implementation of pocketwind's
+ * D3Q19 wind-tunnel hot path. Plain pointers and sizes only; no HIP or torch types.
+ *
+ * Every entry point names the reference interface it replaces (pocketwind repository,
+ * file:line). Calls on one handle are not thread-safe; separate handles are independent.
+ * All device work of a handle is ordered on that handle's own HIP stream.
+ */
+#ifndef PW_H
+#define PW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PW_ABI_VERSION 1
+
+/* Errors (src/lbm/gpu-resources.ts:3-27, gpu-force.ts:192-205). EINVAL is the reference's
+ * RangeError, ENOMEM its AllocationFailure, EBUSY its 'busy' readback. */
+typedef enum {
+  PW_OK = 0,
+  PW_EINVAL = 1,
+  PW_ENOMEM = 2,
+  PW_EBUSY = 3,
+  PW_EDEVICE = 4,
+  PW_ESTATE = 5 /* handle destroyed or used out of order */
+} pw_status;
+
+/* Cell kinds in bits 0-2 of a geometry flag; bit 3+i marks that incoming direction i pulls
+ * from a wall (src/geometry/voxel-grid.ts:3-5, docs/numerics.md "Macro buffer ... walls"). */
+enum { PW_FLUID = 0, PW_SOLID = 1, PW_SUPPORT = 2, PW_FLOOR = 3, PW_EQ = 4 };
+#define PW_CELL_TYPE_MASK 7u
+#define PW_NO_Q 0xffffffffu
+
+typedef enum { PW_OP_TRT = 0, PW_OP_REG = 1 } pw_operator;           /* app/settings Operator */
+typedef enum { PW_WALLS_BB = 0, PW_WALLS_BOUZIDI = 1 } pw_walls;     /* app/settings Walls */
+/* Population storage: f32, f16 scaled by 2^15, or bf16; both 16-bit forms store f - w with
+ * lowbias32 stochastic rounding (src/lbm/shaders/prelude-f16.wgsl:7-22). */
+typedef enum { PW_STORAGE_F32 = 0, PW_STORAGE_F16 = 1, PW_STORAGE_BF16 = 2 } pw_storage;
+/* Near-wall closure under LES: the reference's Werner-Wengle wall law
+ * (src/lbm/shaders/stream_collide.wgsl:8-80), or the learned MLP closure loaded with
+ * pw_solver_set_wall_closure, which falls back to the wall law outside its training range. */
+typedef enum { PW_WALL_LAW = 0, PW_WALL_LEARNED = 1 } pw_wall_model;
+
+typedef struct {
+  uint32_t nx, ny, nz;
+} pw_layout;
+
+/* SolverParams (src/lbm/gpu-solver.ts:19-30). */
+typedef struct {
+  double tau0;         /* baseline relaxation time, > 1/2 after f32 rounding */
+  double u_in;         /* inlet speed along +x, cells per step */
+  uint32_t les;        /* Smagorinsky sub-grid model (Cs = 0.1) and wall model */
+  uint32_t op;         /* pw_operator */
+  uint32_t walls;      /* pw_walls */
+  uint32_t storage;    /* pw_storage */
+  uint32_t ramp_steps; /* linear inlet ramp length; 0 = full speed from the start */
+  uint32_t wall_model; /* pw_wall_model */
+} pw_params;
+
+/* ForceSample (src/lbm/gpu-force.ts:11); components accumulated and returned in FP64. */
+typedef struct {
+  uint64_t step;
+  double fx, fy, fz;
+} pw_force_sample;
+
+typedef struct pw_solver pw_solver;
+
+/* Last error text of the calling thread (the message the reference puts in RangeError). */
+const char *pw_last_error_message(void);
+int pw_abi_version(void);
+/* Number of visible HIP devices (0 without a GPU; no kernel is launched). */
+int pw_device_count(void);
+
+/* GpuSolver.create (src/lbm/gpu-solver.ts:158-335): validates the packed geometry
+ * [flags N][q offsets N][f32 q per link] (src/lbm/geometry-contract.ts:39-66) and the
+ * boundary list (src/lbm/gpu-force.ts:76-81), copies both to the device, allocates two
+ * population sets, geometry and macros, and initialises both sets to equilibrium.
+ * On allocation failure returns PW_ENOMEM with nothing left allocated. */
+pw_status pw_solver_create(const pw_layout *layout, const uint32_t *geometry,
+                           size_t geometry_words, const uint32_t *boundary_cells,
+                           size_t n_boundary, const pw_params *params, pw_solver **out);
+
+/* GpuSolver.step(encoder, count, writeMacroOnLast) (src/lbm/gpu-solver.ts:378-395).
+ * Stream-ordered and asynchronous; count 0 launches nothing. */
+pw_status pw_solver_step(pw_solver *s, uint32_t count, int write_macro_on_last);
+
+/* GpuSolver.encodeForce (src/lbm/gpu-solver.ts:448-451, gpu-force.ts:175-189): samples the
+ * mean momentum exchange of the two latest states; skipped while a sample is unread. */
+pw_status pw_solver_encode_force(pw_solver *s);
+
+/* GpuSolver.readForce (src/lbm/gpu-solver.ts:454-456, gpu-force.ts:192-205): waits for the
+ * encoded sample; PW_EBUSY when none is encoded. */
+pw_status pw_solver_read_force(pw_solver *s, pw_force_sample *out);
+
+/* GpuSolver.reset (src/lbm/gpu-solver.ts:347-355): both sets back to equilibrium, state 0,
+ * step count 0, an unread force sample discarded. */
+pw_status pw_solver_reset(pw_solver *s);
+
+/* GpuSolver.stepCount (src/lbm/gpu-solver.ts:342-344). */
+uint64_t pw_solver_step_count(const pw_solver *s);
+
+/* GpuSolver.destroy (src/lbm/gpu-solver.ts:459-464). Waits for queued work. */
+void pw_solver_destroy(pw_solver *s);
+
+/* macroBuffer readback (docs/numerics.md "Macro buffer"): 4*N floats (rho, ux, uy, uz). */
+pw_status pw_solver_read_macro(pw_solver *s, float *out, size_t out_floats);
+
+/* populationBuffers readback (src/lbm/gpu-solver.ts:338-340, docs/numerics.md "Population
+ * storage"): the latest state as 19 direction-major planes, A (0-6), B (7-12), C (13-18)
+ * back to back, in storage format (f32, or raw 16-bit words). */
+pw_status pw_solver_read_populations(pw_solver *s, void *out, size_t out_bytes);
+/* The test-side queue.writeBuffer into populationBuffers (tests/browser/force.spec.ts:145-153):
+ * overwrite the latest state with raw storage-format data of the same size. */
+pw_status pw_solver_write_populations(pw_solver *s, const void *in, size_t in_bytes);
+size_t pw_solver_population_bytes(const pw_solver *s);
+
+/* injectNaNForRunner (src/lbm/gpu-solver.ts:358-375): write quiet-NaN bits into direction 0
+ * of the first fluid cell of the latest state. */
+pw_status pw_debug_inject_nan(pw_solver *s);
+
+/* Device views for co-resident kernels (particles, interop). Pointers stay valid until
+ * destroy; `populations` is the latest state and changes after every step. */
+typedef struct {
+  const void *populations;
+  const float *macro;   /* float4 per cell */
+  const uint32_t *geometry;
+  void *stream;         /* hipStream_t of the handle */
+} pw_device_state;
+pw_status pw_solver_device_state(pw_solver *s, pw_device_state *out);
+
+/* Wait for all queued work of the handle. */
+pw_status pw_solver_synchronize(pw_solver *s);
+
+/* Measurement (bench/precision-hash.mjs:102-150 timestamps the first and last pass): run
+ * `count` steps without macro writes between two HIP events on the handle's stream and
+ * return the elapsed milliseconds. */
+pw_status pw_solver_time_steps(pw_solver *s, uint32_t count, float *elapsed_ms);
+
+/* measureCopyBandwidth (src/app/bench-page.ts:204-242): device-to-device float4 copy of
+ * `bytes`, `passes` times; returns GB/s of read+write traffic. */
+pw_status pw_copy_bandwidth(size_t bytes, uint32_t passes, double *gbps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

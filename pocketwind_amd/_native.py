@@ -1,0 +1,105 @@
+"""ctypes binding of libpocketwind.so (the C-ABI declared in include/pw.h).
+
+The library must exist: nothing here falls back to a CPU path. Build it with
+``python -m pocketwind_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import build as _build
+
+LIB_PATH = _build.LIB
+
+PW_OK, PW_EINVAL, PW_ENOMEM, PW_EBUSY, PW_EDEVICE, PW_ESTATE = range(6)
+
+
+class Layout(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("tau0", C.c_double), ("u_in", C.c_double), ("les", C.c_uint32),
+                ("op", C.c_uint32), ("walls", C.c_uint32), ("storage", C.c_uint32),
+                ("ramp_steps", C.c_uint32), ("wall_model", C.c_uint32)]
+
+
+class ForceSample(C.Structure):
+    _fields_ = [("step", C.c_uint64), ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double)]
+
+
+class DeviceState(C.Structure):
+    _fields_ = [("populations", C.c_void_p), ("macro", C.c_void_p), ("geometry", C.c_void_p),
+                ("stream", C.c_void_p)]
+
+
+_u32p = C.POINTER(C.c_uint32)
+_lib = None
+
+# Every entry point of include/pw.h with its ctypes signature (restype, argtypes).
+SIGNATURES = {
+    "pw_last_error_message": (C.c_char_p, []),
+    "pw_abi_version": (C.c_int, []),
+    "pw_device_count": (C.c_int, []),
+    "pw_solver_create": (C.c_int, [C.POINTER(Layout), _u32p, C.c_size_t, _u32p, C.c_size_t,
+                                   C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "pw_solver_step": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int]),
+    "pw_solver_encode_force": (C.c_int, [C.c_void_p]),
+    "pw_solver_read_force": (C.c_int, [C.c_void_p, C.POINTER(ForceSample)]),
+    "pw_solver_reset": (C.c_int, [C.c_void_p]),
+    "pw_solver_step_count": (C.c_uint64, [C.c_void_p]),
+    "pw_solver_destroy": (None, [C.c_void_p]),
+    "pw_solver_read_macro": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_solver_read_populations": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_solver_write_populations": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_solver_population_bytes": (C.c_size_t, [C.c_void_p]),
+    "pw_debug_inject_nan": (C.c_int, [C.c_void_p]),
+    "pw_solver_device_state": (C.c_int, [C.c_void_p, C.POINTER(DeviceState)]),
+    "pw_solver_synchronize": (C.c_int, [C.c_void_p]),
+    "pw_solver_time_steps": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
+    "pw_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_uint32, C.POINTER(C.c_double)]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libpocketwind.so or raise: the product has no fallback path."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing; build it with `python -m pocketwind_amd.build`.")
+        handle = C.CDLL(LIB_PATH)
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().pw_last_error_message()
+    return msg.decode() if msg else ""
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure inside libpocketwind."""
+
+
+def check(status: int) -> int:
+    """Map a pw_status onto the reference's error kinds (RangeError -> ValueError)."""
+    if status in (PW_OK, PW_EBUSY):
+        return status
+    message = last_error()
+    if status == PW_EINVAL:
+        raise ValueError(message)
+    if status == PW_ENOMEM:
+        raise MemoryError(message)
+    if status == PW_ESTATE:
+        raise RuntimeError(message)
+    raise DeviceError(message)
+
+
+def device_count() -> int:
+    return int(lib().pw_device_count())

@@ -1,0 +1,456 @@
+// solver.hip -- pw_solver: the C-ABI step / force / readback API on one HIP stream.
+//
+// Host semantics follow src/lbm/gpu-solver.ts:158-465 (create, reset, step with macro on the
+// last dispatch of a batch, linear inlet ramp) and src/lbm/gpu-force.ts:45-216 (one force
+// slot that is skipped while occupied and returns busy when empty).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "force_kernels.h"
+
+namespace pw {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& message) { g_error = message; }
+
+pw_status device_error(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PW_ENOMEM : PW_EDEVICE;
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline bool wall_kind(uint32_t k) { return k >= kSolid && k <= kFloor; }
+
+// geometry-contract.ts:39-66: flags must equal kind | pull wall mask over wrapped
+// neighbours, q offsets must be the running count of links, the length must match.
+bool validate_geometry(const pw_layout& L, const uint32_t* geo, size_t words) {
+  const size_t N = (size_t)L.nx * L.ny * L.nz;
+  if (words < 2 * N) {
+    set_error("Geometry must match the solver layout and contain flags and q offsets.");
+    return false;
+  }
+  size_t fractions = 0;
+  for (size_t cell = 0; cell < N; ++cell) {
+    const uint32_t flag = geo[cell];
+    const uint32_t kind = flag & kTypeMask;
+    if (kind > kEq) {
+      set_error("Unknown geometry cell kind at cell " + std::to_string(cell) + ".");
+      return false;
+    }
+    uint32_t mask = 0;
+    if (kind == kFluid) {
+      const long x = cell % L.nx, y = (cell / L.nx) % L.ny, z = cell / ((size_t)L.nx * L.ny);
+      for (int i = 1; i < kQ; ++i) {
+        const long sx = (x - cx(i) + L.nx) % L.nx, sy = (y - cy(i) + L.ny) % L.ny,
+                   sz = (z - cz(i) + L.nz) % L.nz;
+        if (wall_kind(geo[sx + L.nx * (sy + (size_t)L.ny * sz)] & kTypeMask)) mask |= 1u << (3 + i);
+      }
+    }
+    if (flag != (kind | mask)) {
+      set_error("Inconsistent wall mask at cell " + std::to_string(cell) + ".");
+      return false;
+    }
+    const uint32_t offset = geo[N + cell];
+    if (offset == kNoQ) continue;
+    if (mask == 0 || offset != fractions) {
+      set_error("Invalid geometry q offset at cell " + std::to_string(cell) + ".");
+      return false;
+    }
+    fractions += __builtin_popcount(mask);
+  }
+  if (words != 2 * N + fractions) {
+    set_error("Geometry length must match its cell planes and packed wall-link fractions.");
+    return false;
+  }
+  return true;
+}
+
+// gpu-solver.ts:80-116: the f32 uniform, including the re-derived f32 viscosity.
+bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepParams* out) {
+  const double cells = (double)L.nx * L.ny * L.nz;
+  if (cells * 7 > 4294967295.0) {
+    set_error("Population buffers exceed u32 shader indexing.");
+    return false;
+  }
+  const float tau0 = (float)prm.tau0;
+  const float uin = (float)u_in;
+  const float nu = (float)(((double)tau0 - 0.5) / 3);
+  const float shader_tau = (float)((float)(3 * (double)nu) + 0.5);
+  if (!std::isfinite(tau0) || !std::isfinite(shader_tau) || shader_tau <= 0.5f || !std::isfinite(uin)) {
+    set_error("Relaxation time must exceed 0.5 and inlet speed must be finite.");
+    return false;
+  }
+  out->nx = L.nx;
+  out->ny = L.ny;
+  out->nz = L.nz;
+  out->cells = (uint32_t)cells;
+  out->u_in = uin;
+  out->nu = nu;
+  out->tau0 = 3.0f * nu + 0.5f;
+  out->cs_term = (float)(18 * M_SQRT2 * 0.1 * 0.1);
+  out->lambda = 3.0f / 16.0f;
+  return true;
+}
+
+// gpu-solver.ts:119-123
+double ramp_factor(uint64_t step, uint32_t ramp) {
+  if (ramp == 0) return 1;
+  if (step == 0) return 0;
+  return std::fmin(1.0, (double)step / ramp);
+}
+
+inline unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <class S>
+void launch_init(pw_solver* s, void* out, const StepParams& p) {
+  hipLaunchKernelGGL(init_kernel<S>, dim3(blocks_for(s->cells)), dim3(kBlock), 0, s->stream,
+                     (typename S::T*)out, s->geometry, s->macro, p);
+}
+
+template <class S, int OP, bool LES, bool BB>
+void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const StepParams& p) {
+  constexpr bool BOUZIDI = !BB;
+  auto k = macro ? stream_collide_kernel<S, OP, LES, BOUZIDI, true> : stream_collide_kernel<S, OP, LES, BOUZIDI, false>;
+  hipLaunchKernelGGL(k, dim3(blocks_for(s->cells)), dim3(kBlock), 0, s->stream,
+                     (const typename S::T*)in, (typename S::T*)out, s->geometry, s->macro, p);
+}
+
+template <class S>
+void launch_step_s(pw_solver* s, const void* in, void* out, bool macro, const StepParams& p) {
+  const bool reg = s->params.op == PW_OP_REG, les = s->params.les != 0, bb = s->params.walls == PW_WALLS_BB;
+  if (reg) {
+    if (les) bb ? launch_step_t<S, kOpReg, true, true>(s, in, out, macro, p) : launch_step_t<S, kOpReg, true, false>(s, in, out, macro, p);
+    else bb ? launch_step_t<S, kOpReg, false, true>(s, in, out, macro, p) : launch_step_t<S, kOpReg, false, false>(s, in, out, macro, p);
+  } else {
+    if (les) bb ? launch_step_t<S, kOpTrt, true, true>(s, in, out, macro, p) : launch_step_t<S, kOpTrt, true, false>(s, in, out, macro, p);
+    else bb ? launch_step_t<S, kOpTrt, false, true>(s, in, out, macro, p) : launch_step_t<S, kOpTrt, false, false>(s, in, out, macro, p);
+  }
+}
+
+void launch_step(pw_solver* s, bool macro, const StepParams& p) {
+  const void* in = s->pops[s->current];
+  void* out = s->pops[s->current ^ 1];
+  switch (s->params.storage) {
+    case PW_STORAGE_F16: launch_step_s<StoreF16>(s, in, out, macro, p); break;
+    case PW_STORAGE_BF16: launch_step_s<StoreBF16>(s, in, out, macro, p); break;
+    default: launch_step_s<StoreF32>(s, in, out, macro, p); break;
+  }
+}
+
+void launch_init_both(pw_solver* s, const StepParams& p) {
+  for (int k = 0; k < 2; ++k) {
+    switch (s->params.storage) {
+      case PW_STORAGE_F16: launch_init<StoreF16>(s, s->pops[k], p); break;
+      case PW_STORAGE_BF16: launch_init<StoreBF16>(s, s->pops[k], p); break;
+      default: launch_init<StoreF32>(s, s->pops[k], p); break;
+    }
+  }
+}
+
+template <class S>
+void launch_force_s(pw_solver* s) {
+  const unsigned blocks = (unsigned)s->n_partials;
+  if (blocks > 0) {
+    auto k = s->params.walls == PW_WALLS_BOUZIDI ? force_partials_kernel<S, true> : force_partials_kernel<S, false>;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(kForceBlock), 0, s->stream, (const typename S::T*)s->pops[0],
+                       (const typename S::T*)s->pops[1], s->geometry, s->boundary, (uint32_t)s->n_boundary,
+                       s->partials, s->sp);
+  }
+  hipLaunchKernelGGL(force_reduce_kernel, dim3(1), dim3(kForceBlock), 0, s->stream, s->partials,
+                     (uint32_t)s->n_partials, s->force_dev);
+}
+
+void free_all(pw_solver* s) {
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (void* p : s->pops) if (p) hipFree(p);
+  if (s->geometry) hipFree(s->geometry);
+  if (s->boundary) hipFree(s->boundary);
+  if (s->macro) hipFree(s->macro);
+  if (s->partials) hipFree(s->partials);
+  if (s->force_dev) hipFree(s->force_dev);
+  if (s->force_host) hipHostFree(s->force_host);
+  if (s->force_event) hipEventDestroy(s->force_event);
+  if (s->t0) hipEventDestroy(s->t0);
+  if (s->t1) hipEventDestroy(s->t1);
+  if (s->stream) hipStreamDestroy(s->stream);
+}
+
+pw_status check_handle(pw_solver* s) {
+  if (!s) {
+    set_error("Null solver handle.");
+    return PW_EINVAL;
+  }
+  if (s->destroyed) {
+    set_error("The solver has been destroyed.");
+    return PW_ESTATE;
+  }
+  return PW_OK;
+}
+
+#define PW_TRY(expr, what)                                  \
+  do {                                                      \
+    hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) return ::pw::device_error(e_, what); \
+  } while (0)
+
+}  // namespace
+}  // namespace pw
+
+using namespace pw;
+
+extern "C" {
+
+const char* pw_last_error_message(void) { return g_error.c_str(); }
+int pw_abi_version(void) { return PW_ABI_VERSION; }
+
+int pw_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, size_t geometry_words,
+                           const uint32_t* boundary_cells, size_t n_boundary, const pw_params* params,
+                           pw_solver** out) {
+  if (!layout || !params || !out || (!geometry && geometry_words)) {
+    set_error("Null argument.");
+    return PW_EINVAL;
+  }
+  *out = nullptr;
+  const pw_layout L = *layout;
+  if (L.nx < 1 || L.ny < 1 || L.nz < 1) {
+    set_error("Grid dimensions must be positive safe integers.");
+    return PW_EINVAL;
+  }
+  if (params->storage > PW_STORAGE_BF16 || params->op > PW_OP_REG || params->walls > PW_WALLS_BOUZIDI ||
+      params->wall_model > PW_WALL_LEARNED) {
+    set_error("Unknown operator, wall, storage or wall-model option.");
+    return PW_EINVAL;
+  }
+  if (params->wall_model == PW_WALL_LEARNED) {
+    set_error("The learned wall closure is not available in this build.");
+    return PW_EINVAL;
+  }
+  // gpu-solver.ts:172-177: a ramp starts from rest.
+  StepParams sp{};
+  if (!pack_params(L, *params, params->ramp_steps > 0 ? 0.0 : params->u_in, &sp)) return PW_EINVAL;
+  StepParams full{};
+  if (!pack_params(L, *params, params->u_in, &full)) return PW_EINVAL;
+  if (!validate_geometry(L, geometry, geometry_words)) return PW_EINVAL;
+  const size_t N = (size_t)L.nx * L.ny * L.nz;
+  long long first_fluid = -1;
+  for (size_t c = 0; c < N && first_fluid < 0; ++c)
+    if ((geometry[c] & kTypeMask) == kFluid) first_fluid = (long long)c;
+  // gpu-force.ts:76-81
+  for (size_t k = 0; k < n_boundary; ++k) {
+    const uint32_t cell = boundary_cells[k];
+    if (cell >= N || (geometry[cell] & kTypeMask) != kFluid || (geometry[cell] >> 4) == 0) {
+      set_error("Boundary cells must index fluid cells with wall links.");
+      return PW_EINVAL;
+    }
+  }
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    set_error("No HIP device is available.");
+    return PW_EDEVICE;
+  }
+  pw_solver* s = new (std::nothrow) pw_solver();
+  if (!s) {
+    set_error("Host allocation failed.");
+    return PW_ENOMEM;
+  }
+  s->layout = L;
+  s->cells = N;
+  s->params = *params;
+  s->sp = full;
+  s->elem_bytes = params->storage == PW_STORAGE_F32 ? 4 : 2;
+  s->geometry_words = geometry_words;
+  s->n_boundary = n_boundary;
+  s->first_fluid = first_fluid;
+  s->n_partials = (n_boundary + kForceBlock - 1) / kForceBlock;
+  hipGetDevice(&s->device);
+  auto fail = [&](pw_status st) {
+    free_all(s);
+    delete s;
+    return st;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) return fail(device_error(e, "hipStreamCreate"));
+  const size_t set_bytes = (size_t)kQ * N * s->elem_bytes;
+  for (int k = 0; k < 2; ++k)
+    if ((e = hipMalloc(&s->pops[k], set_bytes)) != hipSuccess) return fail(device_error(e, "population allocation"));
+  if ((e = hipMalloc((void**)&s->geometry, geometry_words * 4)) != hipSuccess) return fail(device_error(e, "geometry allocation"));
+  if ((e = hipMalloc((void**)&s->macro, N * sizeof(float4))) != hipSuccess) return fail(device_error(e, "macro allocation"));
+  if ((e = hipMalloc((void**)&s->boundary, (n_boundary ? n_boundary : 1) * 4)) != hipSuccess) return fail(device_error(e, "boundary allocation"));
+  if ((e = hipMalloc((void**)&s->partials, (s->n_partials ? s->n_partials : 1) * 3 * sizeof(double))) != hipSuccess) return fail(device_error(e, "force partial allocation"));
+  if ((e = hipMalloc((void**)&s->force_dev, 4 * sizeof(double))) != hipSuccess) return fail(device_error(e, "force allocation"));
+  if ((e = hipHostMalloc((void**)&s->force_host, 4 * sizeof(double), hipHostMallocDefault)) != hipSuccess) return fail(device_error(e, "pinned force slot"));
+  if ((e = hipEventCreateWithFlags(&s->force_event, hipEventDisableTiming)) != hipSuccess) return fail(device_error(e, "event"));
+  if ((e = hipEventCreate(&s->t0)) != hipSuccess || (e = hipEventCreate(&s->t1)) != hipSuccess) return fail(device_error(e, "event"));
+  if ((e = hipMemcpyAsync(s->geometry, geometry, geometry_words * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "geometry upload"));
+  if (n_boundary && (e = hipMemcpyAsync(s->boundary, boundary_cells, n_boundary * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "boundary upload"));
+  launch_init_both(s, sp);
+  if ((e = hipGetLastError()) != hipSuccess) return fail(device_error(e, "init launch"));
+  if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return fail(device_error(e, "create"));
+  *out = s;
+  return PW_OK;
+}
+
+pw_status pw_solver_step(pw_solver* s, uint32_t count, int write_macro_on_last) {
+  if (pw_status st = check_handle(s)) return st;
+  for (uint32_t k = 0; k < count; ++k) {
+    const bool macro = write_macro_on_last && k + 1 == count;
+    StepParams p = s->sp;
+    if (s->params.ramp_steps > 0 && s->steps < s->params.ramp_steps) {
+      // gpu-solver.ts:401-414: the ramped speed of step `steps + 1`.
+      p.u_in = (float)(s->params.u_in * ramp_factor(s->steps + 1, s->params.ramp_steps));
+    }
+    launch_step(s, macro, p);
+    s->current ^= 1;
+    s->steps += 1;
+  }
+  if (count) PW_TRY(hipGetLastError(), "step launch");
+  return PW_OK;
+}
+
+pw_status pw_solver_encode_force(pw_solver* s) {
+  if (pw_status st = check_handle(s)) return st;
+  if (s->force_encoded) return PW_OK;  // gpu-force.ts:176: an occupied slot skips the sample
+  switch (s->params.storage) {
+    case PW_STORAGE_F16: launch_force_s<StoreF16>(s); break;
+    case PW_STORAGE_BF16: launch_force_s<StoreBF16>(s); break;
+    default: launch_force_s<StoreF32>(s); break;
+  }
+  PW_TRY(hipGetLastError(), "force launch");
+  PW_TRY(hipMemcpyAsync(s->force_host, s->force_dev, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream), "force readback");
+  PW_TRY(hipEventRecord(s->force_event, s->stream), "force event");
+  s->force_encoded = 1;
+  s->force_step = s->steps;
+  return PW_OK;
+}
+
+pw_status pw_solver_read_force(pw_solver* s, pw_force_sample* out) {
+  if (pw_status st = check_handle(s)) return st;
+  if (!out) {
+    set_error("Null output.");
+    return PW_EINVAL;
+  }
+  if (!s->force_encoded) return PW_EBUSY;
+  s->force_encoded = 0;
+  PW_TRY(hipEventSynchronize(s->force_event), "force wait");
+  out->step = s->force_step;
+  out->fx = s->force_host[0];
+  out->fy = s->force_host[1];
+  out->fz = s->force_host[2];
+  return PW_OK;
+}
+
+pw_status pw_solver_reset(pw_solver* s) {
+  if (pw_status st = check_handle(s)) return st;
+  StepParams p = s->sp;
+  if (s->params.ramp_steps > 0) p.u_in = 0.0f;
+  launch_init_both(s, p);
+  PW_TRY(hipGetLastError(), "reset launch");
+  s->current = 0;
+  s->steps = 0;
+  s->ramp_settled = 0;
+  s->force_encoded = 0;  // gpu-force.ts:208-210: an unread sample is discarded
+  return PW_OK;
+}
+
+uint64_t pw_solver_step_count(const pw_solver* s) { return s ? s->steps : 0; }
+
+void pw_solver_destroy(pw_solver* s) {
+  if (!s || s->destroyed) return;
+  free_all(s);
+  s->destroyed = 1;
+  delete s;
+}
+
+pw_status pw_solver_read_macro(pw_solver* s, float* out, size_t out_floats) {
+  if (pw_status st = check_handle(s)) return st;
+  if (!out || out_floats < 4 * s->cells) {
+    set_error("Macro output needs 4 floats per cell.");
+    return PW_EINVAL;
+  }
+  PW_TRY(hipMemcpyAsync(out, s->macro, 4 * s->cells * sizeof(float), hipMemcpyDeviceToHost, s->stream), "macro readback");
+  PW_TRY(hipStreamSynchronize(s->stream), "macro readback");
+  return PW_OK;
+}
+
+size_t pw_solver_population_bytes(const pw_solver* s) { return s ? (size_t)kQ * s->cells * s->elem_bytes : 0; }
+
+pw_status pw_solver_read_populations(pw_solver* s, void* out, size_t out_bytes) {
+  if (pw_status st = check_handle(s)) return st;
+  const size_t bytes = pw_solver_population_bytes(s);
+  if (!out || out_bytes < bytes) {
+    set_error("Population output is smaller than one state.");
+    return PW_EINVAL;
+  }
+  PW_TRY(hipMemcpyAsync(out, s->pops[s->current], bytes, hipMemcpyDeviceToHost, s->stream), "population readback");
+  PW_TRY(hipStreamSynchronize(s->stream), "population readback");
+  return PW_OK;
+}
+
+pw_status pw_solver_write_populations(pw_solver* s, const void* in, size_t in_bytes) {
+  if (pw_status st = check_handle(s)) return st;
+  const size_t bytes = pw_solver_population_bytes(s);
+  if (!in || in_bytes != bytes) {
+    set_error("Population input must hold exactly one state.");
+    return PW_EINVAL;
+  }
+  PW_TRY(hipMemcpyAsync(s->pops[s->current], in, bytes, hipMemcpyHostToDevice, s->stream), "population upload");
+  PW_TRY(hipStreamSynchronize(s->stream), "population upload");
+  return PW_OK;
+}
+
+pw_status pw_debug_inject_nan(pw_solver* s) {
+  if (pw_status st = check_handle(s)) return st;
+  if (s->first_fluid < 0) return PW_OK;
+  const size_t off = (size_t)s->first_fluid * s->elem_bytes;
+  if (s->elem_bytes == 4) {
+    const uint32_t nan_bits = 0x7fc00000u;
+    PW_TRY(hipMemcpy((char*)s->pops[s->current] + off, &nan_bits, 4, hipMemcpyHostToDevice), "inject");
+  } else {
+    const uint16_t nan_bits = s->params.storage == PW_STORAGE_F16 ? 0x7e00u : 0x7fc0u;
+    PW_TRY(hipMemcpy((char*)s->pops[s->current] + off, &nan_bits, 2, hipMemcpyHostToDevice), "inject");
+  }
+  return PW_OK;
+}
+
+pw_status pw_solver_device_state(pw_solver* s, pw_device_state* out) {
+  if (pw_status st = check_handle(s)) return st;
+  out->populations = s->pops[s->current];
+  out->macro = (const float*)s->macro;
+  out->geometry = s->geometry;
+  out->stream = (void*)s->stream;
+  return PW_OK;
+}
+
+pw_status pw_solver_synchronize(pw_solver* s) {
+  if (pw_status st = check_handle(s)) return st;
+  PW_TRY(hipStreamSynchronize(s->stream), "synchronize");
+  return PW_OK;
+}
+
+pw_status pw_solver_time_steps(pw_solver* s, uint32_t count, float* elapsed_ms) {
+  if (pw_status st = check_handle(s)) return st;
+  PW_TRY(hipEventRecord(s->t0, s->stream), "timing event");
+  if (pw_status st = pw_solver_step(s, count, 0)) return st;
+  PW_TRY(hipEventRecord(s->t1, s->stream), "timing event");
+  PW_TRY(hipEventSynchronize(s->t1), "timing wait");
+  PW_TRY(hipEventElapsedTime(elapsed_ms, s->t0, s->t1), "timing");
+  return PW_OK;
+}
+
+}  // extern "C"

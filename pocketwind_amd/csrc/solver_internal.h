@@ -1,0 +1,57 @@
+// solver_internal.h -- host-side state of a pw_solver handle and shared helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/pw.h"
+
+namespace pw {
+
+// Per-launch scalars of the step kernel (the reference's 48-byte uniform, struct Params in
+// src/lbm/shaders/common.wgsl:31-46), passed by value so a ramped step needs no buffer write.
+struct StepParams {
+  uint32_t nx, ny, nz;
+  uint32_t cells;
+  float u_in;     // inlet speed this step (ramped)
+  float nu;       // f32 viscosity, (tau0 - 1/2) / 3 rounded as the reference packs it
+  float tau0;     // 3 nu + 1/2 evaluated in f32, as relaxation_time does
+  float cs_term;  // 18 sqrt(2) Cs^2
+  float lambda;   // TRT magic parameter 3/16
+};
+
+void set_error(const std::string& message);
+pw_status device_error(hipError_t e, const char* what);
+
+}  // namespace pw
+
+struct pw_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  pw_layout layout{};
+  size_t cells = 0;
+  pw_params params{};
+  pw::StepParams sp{};
+  uint32_t elem_bytes = 4;
+  void* pops[2] = {nullptr, nullptr};
+  uint32_t* geometry = nullptr;
+  size_t geometry_words = 0;
+  uint32_t* boundary = nullptr;
+  size_t n_boundary = 0;
+  float4* macro = nullptr;
+  double* partials = nullptr;
+  size_t n_partials = 0;
+  double* force_dev = nullptr;    // fx, fy, fz
+  double* force_host = nullptr;   // pinned
+  hipEvent_t force_event = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  int current = 0;
+  uint64_t steps = 0;
+  int force_encoded = 0;
+  uint64_t force_step = 0;
+  int ramp_settled = 0;
+  long long first_fluid = -1;
+  int destroyed = 0;
+};

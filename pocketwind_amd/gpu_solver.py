@@ -1,0 +1,214 @@
+"""GpuSolver: the reference's solver step/readback API over libpocketwind's C-ABI.
+
+Mirrors src/lbm/gpu-solver.ts:19-465 and src/lbm/gpu-force.ts:11-216 -- the same method
+names (snake_case), argument meaning and error behaviour:
+
+* invalid input raises ``ValueError`` (the reference's RangeError);
+* an allocation failure is *returned* as :class:`AllocationFailure` with nothing left
+  allocated (gpu-solver.ts:330-334);
+* ``read_force()`` returns ``'busy'`` when no sample is encoded (gpu-force.ts:192-205).
+
+Device work is queued on the solver's own HIP stream. Readbacks wait for it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Literal, Union
+
+import numpy as np
+
+from . import _native as N
+from .lattice import CELL_TYPE_MASK, FLUID, Layout, VoxelGrid
+
+FORCE_EVERY_STEPS = 8  # src/lbm/params.ts:12
+
+_OPERATORS = {"trt": 0, "reg": 1}
+_WALLS = {"bb": 0, "bouzidi": 1}
+_STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+_WALL_MODELS = {"law": 0, "learned": 1}
+
+
+@dataclass(frozen=True)
+class SolverParams:
+    """SolverParams (gpu-solver.ts:19-30); ``precision`` adds bf16 to f32/f16."""
+
+    tau0: float
+    u_in: float
+    les: bool
+    operator: str = "trt"
+    walls: str = "bb"
+    precision: str = "f32"
+    ramp_steps: int = 0
+    wall_model: str = "law"
+
+
+@dataclass(frozen=True)
+class ForceSample:
+    step: int
+    fx: float
+    fy: float
+    fz: float
+
+
+@dataclass(frozen=True)
+class AllocationFailure:
+    """Returned, not raised, when device memory runs out (gpu-resources.ts:3-5)."""
+
+    message: str
+    kind: str = "allocation-failure"
+
+
+def _pack(params: SolverParams) -> N.Params:
+    for name, table, value in (("operator", _OPERATORS, params.operator),
+                               ("walls", _WALLS, params.walls),
+                               ("precision", _STORAGE, params.precision),
+                               ("wall_model", _WALL_MODELS, params.wall_model)):
+        if value not in table:
+            raise ValueError(f"Unknown {name} {value!r}.")
+    if not isinstance(params.ramp_steps, (int, np.integer)) or params.ramp_steps < 0:
+        raise ValueError("Ramp steps must be a non-negative safe integer.")
+    return N.Params(float(params.tau0), float(params.u_in), int(bool(params.les)),
+                    _OPERATORS[params.operator], _WALLS[params.walls], _STORAGE[params.precision],
+                    int(params.ramp_steps), _WALL_MODELS[params.wall_model])
+
+
+class GpuSolver:
+    """Pull-stream/collide driver with f32 arithmetic and f32, f16 or bf16 storage."""
+
+    def __init__(self, handle: int, layout: Layout, params: SolverParams, grid: VoxelGrid):
+        self._h = C.c_void_p(handle)
+        self.layout = layout
+        self.params = params
+        self.cells = layout.nx * layout.ny * layout.nz
+        self._fluid_cell = int(np.flatnonzero((grid.flags & CELL_TYPE_MASK) == FLUID)[0]) \
+            if np.any((grid.flags & CELL_TYPE_MASK) == FLUID) else -1
+        self._injected_nan = False
+        self._destroyed = False
+
+    # -- creation -------------------------------------------------------------------
+    @staticmethod
+    def create(layout, grid: VoxelGrid, params: SolverParams) -> Union["GpuSolver", AllocationFailure]:
+        layout = Layout(*layout)
+        if (grid.nx, grid.ny, grid.nz) != tuple(layout):
+            raise ValueError("Geometry must match the solver layout and contain flags and q offsets.")
+        geo = np.ascontiguousarray(grid.geometry, dtype=np.uint32)
+        bc = np.ascontiguousarray(grid.boundary_cells, dtype=np.uint32)
+        lay = N.Layout(*layout)
+        prm = _pack(params)
+        out = C.c_void_p()
+        status = N.lib().pw_solver_create(
+            C.byref(lay), geo.ctypes.data_as(N._u32p), geo.size, bc.ctypes.data_as(N._u32p),
+            bc.size, C.byref(prm), C.byref(out))
+        if status == N.PW_ENOMEM:
+            return AllocationFailure(N.last_error())
+        N.check(status)
+        return GpuSolver(out.value, layout, params, grid)
+
+    def _live(self):
+        if self._destroyed:
+            raise RuntimeError("The solver has been destroyed.")
+        return self._h
+
+    # -- stepping -------------------------------------------------------------------
+    def step(self, count: int, write_macro_on_last: bool) -> None:
+        """Queue `count` steps; only the last writes macros when asked (gpu-solver.ts:378-395)."""
+        if not isinstance(count, (int, np.integer)) or count < 0 or count >= 2**32:
+            raise ValueError("Step count must be a non-negative safe integer.")
+        N.check(N.lib().pw_solver_step(self._live(), int(count), int(bool(write_macro_on_last))))
+
+    @property
+    def step_count(self) -> int:
+        return int(N.lib().pw_solver_step_count(self._live()))
+
+    def reset(self) -> None:
+        N.check(N.lib().pw_solver_reset(self._live()))
+
+    def encode_force(self) -> None:
+        N.check(N.lib().pw_solver_encode_force(self._live()))
+
+    def read_force(self) -> Union[ForceSample, Literal["busy"]]:
+        out = N.ForceSample()
+        status = N.check(N.lib().pw_solver_read_force(self._live(), C.byref(out)))
+        if status == N.PW_EBUSY:
+            return "busy"
+        return ForceSample(int(out.step), out.fx, out.fy, out.fz)
+
+    # -- readback -------------------------------------------------------------------
+    @property
+    def storage_dtype(self):
+        return {"f32": np.float32, "f16": np.float16, "bf16": np.uint16}[self.params.precision]
+
+    def read_populations(self) -> np.ndarray:
+        """Latest state as a (19, N) array in storage format (gpu-solver.ts:338-340)."""
+        out = np.empty((19, self.cells), dtype=self.storage_dtype)
+        N.check(N.lib().pw_solver_read_populations(self._live(), out.ctypes.data, out.nbytes))
+        return out
+
+    @property
+    def population_buffers(self):
+        """The three direction groups A (0-6), B (7-12), C (13-18) of the latest state."""
+        pops = self.read_populations()
+        return pops[0:7], pops[7:13], pops[13:19]
+
+    def read_populations_f64(self) -> np.ndarray:
+        """Decoded deviations g = f - w of the latest state, (19, N) float64."""
+        pops = self.read_populations()
+        if self.params.precision == "f16":
+            return pops.astype(np.float64) / 32768.0
+        if self.params.precision == "bf16":
+            return (pops.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        return pops.astype(np.float64)
+
+    def write_populations(self, pops: np.ndarray) -> None:
+        arr = np.ascontiguousarray(pops, dtype=self.storage_dtype)
+        N.check(N.lib().pw_solver_write_populations(self._live(), arr.ctypes.data, arr.nbytes))
+
+    def read_macro(self) -> np.ndarray:
+        """macroBuffer as (N, 4) float32: rho, ux, uy, uz (docs/numerics.md)."""
+        out = np.empty((self.cells, 4), dtype=np.float32)
+        N.check(N.lib().pw_solver_read_macro(self._live(), out.ctypes.data, out.size))
+        return out
+
+    macro_buffer = property(read_macro)
+
+    def inject_nan_for_runner(self, step: int) -> None:
+        """gpu-solver.ts:358-375: NaN into the first fluid cell once, at the given step."""
+        if self._destroyed or self._injected_nan or self._fluid_cell < 0 or step <= 0 \
+                or self.step_count != step:
+            return
+        N.check(N.lib().pw_debug_inject_nan(self._live()))
+        self._injected_nan = True
+
+    def device_state(self) -> N.DeviceState:
+        out = N.DeviceState()
+        N.check(N.lib().pw_solver_device_state(self._live(), C.byref(out)))
+        return out
+
+    def synchronize(self) -> None:
+        N.check(N.lib().pw_solver_synchronize(self._live()))
+
+    def time_steps(self, count: int) -> float:
+        """Milliseconds for `count` steps between two events on the solver's stream."""
+        ms = C.c_float()
+        N.check(N.lib().pw_solver_time_steps(self._live(), int(count), C.byref(ms)))
+        return float(ms.value)
+
+    def destroy(self) -> None:
+        if self._destroyed:
+            return
+        N.lib().pw_solver_destroy(self._h)
+        self._destroyed = True
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def copy_bandwidth(nbytes: int = 128 << 20, passes: int = 40) -> float:
+    """Measured device copy GB/s (bench-page.ts:204-242)."""
+    out = C.c_double()
+    N.check(N.lib().pw_copy_bandwidth(int(nbytes), int(passes), C.byref(out)))
+    return out.value

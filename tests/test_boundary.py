@@ -1,0 +1,106 @@
+"""The C-ABI library loads, exports every symbol include/pw.h declares, and rejects invalid
+input on the host before any device work (gpu-solver.ts:164-181, geometry-contract.ts:39-66,
+gpu-force.ts:76-81). No kernel is launched here."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from grids import blob_grid
+from pocketwind_amd import _native as N
+from pocketwind_amd.gpu_solver import GpuSolver, SolverParams
+from pocketwind_amd.lattice import NO_Q, VoxelGrid
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "pw.h")).read()
+    return sorted(set(re.findall(r"\b(pw_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = N.lib()
+    names = declared_symbols()
+    assert len(names) >= 18
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(N.SIGNATURES), set(names) - set(N.SIGNATURES)
+    assert lib.pw_abi_version() == 1
+
+
+def _create(grid, params, boundary=None):
+    geo = np.ascontiguousarray(grid.geometry, np.uint32)
+    bc = np.ascontiguousarray(grid.boundary_cells if boundary is None else boundary, np.uint32)
+    out = C.c_void_p()
+    status = N.lib().pw_solver_create(C.byref(N.Layout(grid.nx, grid.ny, grid.nz)),
+                                      geo.ctypes.data_as(N._u32p), geo.size,
+                                      bc.ctypes.data_as(N._u32p), bc.size, C.byref(params),
+                                      C.byref(out))
+    return status, N.last_error()
+
+
+def _params(**kw):
+    base = dict(tau0=0.8, u_in=0.06, les=0, op=0, walls=0, storage=0, ramp_steps=0, wall_model=0)
+    base.update(kw)
+    return N.Params(**base)
+
+
+def test_rejects_tau_at_or_below_half_after_f32_rounding():
+    grid = blob_grid("bb")
+    for tau in (0.5, 0.5 + 1e-9, float("nan"), 0.2):
+        status, msg = _create(grid, _params(tau0=tau))
+        assert status == N.PW_EINVAL and "Relaxation time" in msg
+
+
+def test_rejects_inconsistent_wall_mask():
+    grid = blob_grid("bb")
+    geo = grid.geometry.copy()
+    fluid = int(np.flatnonzero(geo[: grid.cells] == 0)[0])
+    geo[fluid] |= 1 << 4
+    bad = VoxelGrid(grid.nx, grid.ny, grid.nz, geo, grid.boundary_cells)
+    status, msg = _create(bad, _params())
+    assert status == N.PW_EINVAL and "Inconsistent wall mask" in msg
+
+
+def test_rejects_bad_q_offsets_and_length():
+    grid = blob_grid("bouzidi")
+    geo = grid.geometry.copy()
+    cells = grid.cells
+    with_q = int(np.flatnonzero(geo[cells:2 * cells] != NO_Q)[1])
+    geo[cells + with_q] += 1
+    status, msg = _create(VoxelGrid(32, 32, 32, geo, grid.boundary_cells), _params(walls=1))
+    assert status == N.PW_EINVAL and "q offset" in msg
+    short = VoxelGrid(32, 32, 32, grid.geometry[:-1].copy(), grid.boundary_cells)
+    status, msg = _create(short, _params(walls=1))
+    assert status == N.PW_EINVAL and "length" in msg
+
+
+def test_rejects_boundary_cells_without_wall_links():
+    grid = blob_grid("bb")
+    status, msg = _create(grid, _params(), boundary=np.array([0], np.uint32))
+    assert status == N.PW_EINVAL and "Boundary cells" in msg
+    status, msg = _create(grid, _params(), boundary=np.array([grid.cells], np.uint32))
+    assert status == N.PW_EINVAL
+
+
+def test_rejects_unknown_options_and_kinds():
+    grid = blob_grid("bb")
+    assert _create(grid, _params(storage=7))[0] == N.PW_EINVAL
+    geo = grid.geometry.copy()
+    geo[0] = 5
+    assert _create(VoxelGrid(32, 32, 32, geo, grid.boundary_cells), _params())[0] == N.PW_EINVAL
+
+
+def test_host_mirror_raises_value_error_like_range_error():
+    grid = blob_grid("bb")
+    with pytest.raises(ValueError, match="Relaxation time"):
+        GpuSolver.create(grid.layout, grid, SolverParams(0.5, 0.06, False))
+    with pytest.raises(ValueError):
+        GpuSolver.create(grid.layout, grid, SolverParams(0.8, 0.06, False, operator="bgk"))
+    with pytest.raises(ValueError):
+        GpuSolver.create((16, 32, 32), grid, SolverParams(0.8, 0.06, False))

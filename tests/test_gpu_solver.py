@@ -1,0 +1,187 @@
+"""GPU parity of the fused stream+collide step against the FP64 oracle.
+
+Restates tests/browser/gpu-vs-cpu.spec.ts: the seeded 32^3 blob (EQ planes, one SUPPORT and
+one FLOOR cell), tau0 0.8, u_in 0.06, checkpoints 0/1/10/50/100, normalised peak error
+< 1e-5 for f32 storage (TRT/bb, TRT/Bouzidi, REG+LES/bb), < 2e-3 for f16; macros unchanged
+across 98 steps without macro writes; walls stay at rest; periodic f16 mass drift < 2e-6.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from grids import blob_grid
+from pocketwind_amd.gpu_solver import GpuSolver, SolverParams
+from pocketwind_amd.lattice import CELL_TYPE_MASK, DIRECTIONS, FLOOR, NO_Q, SOLID, VoxelGrid, WEIGHTS
+
+pytestmark = pytest.mark.gpu
+
+F32_EPS = 2.0 ** -23
+TAU0, U_IN = 0.8, 0.06
+CHECKPOINTS = [0, 1, 10, 50, 100]
+CONFIGS = [("trt", False, "bb"), ("trt", False, "bouzidi"), ("reg", True, "bb")]
+
+
+def normalized_peak(actual, expected) -> float:
+    actual = np.asarray(actual, np.float64)
+    expected = np.asarray(expected, np.float64)
+    diff = np.abs(actual - expected)
+    if not np.all(np.isfinite(diff)):
+        return np.inf
+    return float(diff.max() / max(np.abs(expected).max(), F32_EPS))
+
+
+def oracle_states(grid: VoxelGrid, operator: str, les: bool, walls: str, checkpoints):
+    cells = grid.cells
+    cpu = oracle.Solver(grid.layout, TAU0, {"trt": oracle.TRT, "reg": oracle.REG}[operator], les,
+                        {"bb": oracle.BB, "bouzidi": oracle.BOUZIDI}[walls],
+                        flags=grid.geometry if walls == "bouzidi" else grid.geometry[:cells],
+                        equilibrium_state=(1.0, (U_IN, 0.0, 0.0)), threads=4)
+    done, out = 0, {}
+    for step in checkpoints:
+        cpu.step(step - done)
+        done = step
+        macro = cpu.macro_all()
+        macro[:, 0] -= 1
+        out[step] = (macro, cpu.populations.copy().reshape(19, cells), cpu.momentum_exchange())
+    return out
+
+
+def compare(snapshot, reference, tol, step, label):
+    gpu_macro, gpu_pops = snapshot
+    ref_macro, ref_pops, _ = reference
+    shifted = gpu_macro.astype(np.float64)
+    shifted[:, 0] -= 1
+    errors = {
+        "g": normalized_peak(gpu_pops, ref_pops),
+        "rho-1": normalized_peak(shifted[:, 0], ref_macro[:, 0]),
+        "u": normalized_peak(shifted[:, 1:], ref_macro[:, 1:]),
+    }
+    for name, err in errors.items():
+        assert err < tol, f"{label} step {step}: {name} normalized error {err:.3e} >= {tol}"
+    return errors
+
+
+def assert_resting_walls(grid, macro, pops):
+    kinds = grid.flags & CELL_TYPE_MASK
+    walls = (kinds >= SOLID) & (kinds <= FLOOR)
+    assert np.all(macro[walls] == np.array([1, 0, 0, 0], np.float32))
+    assert np.abs(pops[:, walls].astype(np.float64)).max() == 0
+
+
+@pytest.mark.parametrize("operator,les,walls", CONFIGS)
+def test_f32_matches_oracle_on_seeded_blob(operator, les, walls):
+    grid = blob_grid(walls)
+    ref = oracle_states(grid, operator, les, walls, CHECKPOINTS)
+    params = SolverParams(TAU0, U_IN, les, operator=operator, walls=walls)
+    solver = GpuSolver.create(grid.layout, grid, params)
+    diag = GpuSolver.create(grid.layout, grid, params)
+    label = f"{operator}{'+les' if les else ''}/{walls}"
+    try:
+        snaps = {}
+        solver.step(0, True)
+        snaps[0] = (solver.read_macro(), solver.read_populations())
+        solver.step(1, True)
+        snaps[1] = (solver.read_macro(), solver.read_populations())
+        solver.step(98, False)
+        preserved = solver.read_macro()
+        solver.step(1, True)
+        snaps[100] = (solver.read_macro(), solver.read_populations())
+        for step in (10, 50):
+            diag.step(step - diag.step_count, True)
+            snaps[step] = (diag.read_macro(), diag.read_populations())
+        assert solver.step_count == 100
+        assert np.array_equal(preserved, snaps[1][0]), "macros changed without write_macro"
+        growth = {}
+        for step in CHECKPOINTS:
+            growth[step] = compare(snaps[step], ref[step], 1e-5, step, label)
+            assert_resting_walls(grid, snaps[step][0], snaps[step][1])
+        print(label, json.dumps({k: {n: f"{e:.2e}" for n, e in v.items()} for k, v in growth.items()}))
+    finally:
+        solver.destroy()
+        diag.destroy()
+    assert grid.boundary_cells.size > 0
+
+
+@pytest.mark.parametrize("precision,tol", [("f32", 2e-3), ("f16", 2e-3), ("bf16", 2e-2)])
+def test_low_precision_storage_stays_close(precision, tol):
+    grid = blob_grid("bb")
+    ref = oracle_states(grid, "trt", False, "bb", CHECKPOINTS)
+    solver = GpuSolver.create(grid.layout, grid, SolverParams(TAU0, U_IN, False, precision=precision))
+    try:
+        record = {}
+        for step in CHECKPOINTS:
+            solver.step(step - solver.step_count, True)
+            macro = solver.read_macro()
+            pops = solver.read_populations_f64()
+            assert_resting_walls(grid, macro, solver.read_populations())
+            shifted = macro.astype(np.float64)
+            shifted[:, 0] -= 1
+            ref_macro, ref_pops, _ = ref[step]
+            record[step] = {
+                "g": normalized_peak(pops, ref_pops),
+                "rho-1": normalized_peak(shifted[:, 0], ref_macro[:, 0]),
+                "u": normalized_peak(shifted[:, 1:], ref_macro[:, 1:]),
+                "mass": float((pops - ref_pops).sum() / grid.cells),
+            }
+        print(precision, json.dumps(record))
+        for name in ("g", "rho-1", "u"):
+            assert record[100][name] < tol, (precision, name, record)
+    finally:
+        solver.destroy()
+
+
+def periodic_grid(n=32) -> VoxelGrid:
+    cells = n ** 3
+    geometry = np.zeros(2 * cells, np.uint32)
+    geometry[cells:] = NO_Q
+    return VoxelGrid(n, n, n, geometry)
+
+
+@pytest.mark.parametrize("u_in", [0.004, U_IN])
+@pytest.mark.parametrize("precision,rel_tol,drift_tol", [("f16", 1e-3, 2e-6), ("bf16", 1.6e-2, 2e-5)])
+def test_low_precision_mass_drift_in_periodic_flow(u_in, precision, rel_tol, drift_tol):
+    grid = periodic_grid()
+    solver = GpuSolver.create(grid.layout, grid, SolverParams(TAU0, u_in, False, precision=precision))
+    try:
+        exact = np.array([w * (3 * c[0] * u_in + 4.5 * (c[0] * u_in) ** 2 - 1.5 * u_in ** 2)
+                          for w, c in zip(WEIGHTS, DIRECTIONS)])
+        first = solver.read_populations_f64()
+        rel = np.abs((first - exact[:, None]) / exact[:, None]).max()
+        assert rel < rel_tol, f"{precision} per-population storage error {rel}"
+        solver.step(100, True)
+        last = solver.read_populations_f64()
+        for step, pops in ((0, first), (100, last)):
+            drift = pops.sum() / grid.cells
+            assert abs(drift) < drift_tol, f"{precision} mass drift {drift} at step {step}"
+    finally:
+        solver.destroy()
+
+
+def test_zero_step_batch_changes_nothing():
+    grid = blob_grid("bb")
+    solver = GpuSolver.create(grid.layout, grid, SolverParams(TAU0, U_IN, False))
+    try:
+        solver.step(3, True)
+        before = solver.read_populations().copy(), solver.read_macro().copy()
+        solver.step(0, True)
+        assert solver.step_count == 3
+        assert np.array_equal(before[0], solver.read_populations())
+        assert np.array_equal(before[1], solver.read_macro())
+    finally:
+        solver.destroy()
+
+
+def test_ramp_starts_from_rest_and_reaches_full_speed():
+    grid = periodic_grid(8)
+    ramp = 4
+    solver = GpuSolver.create(grid.layout, grid, SolverParams(TAU0, U_IN, False, ramp_steps=ramp))
+    try:
+        assert np.abs(solver.read_populations_f64()).max() == 0
+        assert np.all(solver.read_macro()[:, 1] == 0)
+    finally:
+        solver.destroy()
