@@ -132,6 +132,14 @@ typedef struct {
 } pw_device_state;
 pw_status pw_solver_device_state(pw_solver *s, pw_device_state *out);
 
+/* Step-kernel variants: one cell per lane, or two x-adjacent cells per lane moving each
+ * direction with one wide access. AUTO times both on the handle's buffers (at create, or
+ * when set later, restoring the state afterwards) and keeps the faster. No reference
+ * counterpart: the reference has one WGSL kernel (src/lbm/shaders/stream_collide.wgsl). */
+typedef enum { PW_KERNEL_AUTO = 0, PW_KERNEL_CELL = 1, PW_KERNEL_PAIR = 2 } pw_kernel;
+pw_status pw_solver_set_kernel(pw_solver *s, uint32_t kernel);
+uint32_t pw_solver_kernel(const pw_solver *s);
+
 /* Wait for all queued work of the handle. */
 pw_status pw_solver_synchronize(pw_solver *s);
 

@@ -56,6 +56,8 @@ SIGNATURES = {
     "pw_solver_population_bytes": (C.c_size_t, [C.c_void_p]),
     "pw_debug_inject_nan": (C.c_int, [C.c_void_p]),
     "pw_solver_device_state": (C.c_int, [C.c_void_p, C.POINTER(DeviceState)]),
+    "pw_solver_set_kernel": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "pw_solver_kernel": (C.c_uint32, [C.c_void_p]),
     "pw_solver_synchronize": (C.c_int, [C.c_void_p]),
     "pw_solver_time_steps": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
     "pw_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_uint32, C.POINTER(C.c_double)]),

@@ -30,14 +30,17 @@ __host__ __device__ constexpr float wf(int i) {
 }
 __host__ __device__ constexpr int opp(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
 
+__host__ __device__ __forceinline__ float as_float(uint32_t u) { return __builtin_bit_cast(float, u); }
+__host__ __device__ __forceinline__ uint32_t as_uint(float f) { return __builtin_bit_cast(uint32_t, f); }
+
 enum : uint32_t { kFluid = 0, kSolid = 1, kSupport = 2, kFloor = 3, kEq = 4, kTypeMask = 7 };
 constexpr uint32_t kNoQ = 0xffffffffu;
 
-__device__ __forceinline__ bool is_wall(uint32_t kind) { return kind - 1u < 3u; }
-__device__ __forceinline__ bool has_wall(uint32_t flag, int i) { return (flag >> (3 + i)) & 1u; }
+__host__ __device__ __forceinline__ bool is_wall(uint32_t kind) { return kind - 1u < 3u; }
+__host__ __device__ __forceinline__ bool has_wall(uint32_t flag, int i) { return (flag >> (3 + i)) & 1u; }
 
 // lowbias32 over (value bits, cell, direction) (prelude-f16.wgsl:8-13).
-__device__ __forceinline__ uint32_t storage_hash(uint32_t bits, uint32_t i, uint32_t n) {
+__host__ __device__ __forceinline__ uint32_t storage_hash(uint32_t bits, uint32_t i, uint32_t n) {
   uint32_t r = bits ^ (n * 0x9e3779b9u) ^ (i * 0x85ebca6bu);
   r = (r ^ (r >> 16)) * 0x7feb352du;
   r = (r ^ (r >> 15)) * 0x846ca68bu;
@@ -48,38 +51,37 @@ __device__ __forceinline__ uint32_t storage_hash(uint32_t bits, uint32_t i, uint
 struct StoreF32 {
   using T = float;
   static constexpr int kBytes = 4;
-  __device__ __forceinline__ static float load(const T* p) { return *p; }
-  __device__ __forceinline__ static T encode(float g, uint32_t, uint32_t) { return g; }
+  __host__ __device__ __forceinline__ static float decode(T v) { return v; }
+  __host__ __device__ __forceinline__ static float load(const T* p) { return *p; }
+  __host__ __device__ __forceinline__ static T encode(float g, uint32_t, uint32_t) { return g; }
 };
 
 struct StoreF16 {
   using T = _Float16;
   static constexpr int kBytes = 2;
   static constexpr float kScale = 32768.0f;
-  __device__ __forceinline__ static float load(const T* p) {
-    return static_cast<float>(*p) * (1.0f / kScale);
-  }
-  __device__ __forceinline__ static T encode(float g, uint32_t i, uint32_t n) {
+  __host__ __device__ __forceinline__ static float decode(T v) { return static_cast<float>(v) * (1.0f / kScale); }
+  __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
+  __host__ __device__ __forceinline__ static T encode(float g, uint32_t i, uint32_t n) {
     const float value = g * kScale;
-    const uint32_t bits = __float_as_uint(value);
+    const uint32_t bits = as_uint(value);
     const uint32_t mag = bits & 0x7fffffffu;
     // Below 2^-14 the half is subnormal: keep the native round-to-nearest cast.
     if (mag < 0x38800000u) return static_cast<T>(value);
     const uint32_t r = storage_hash(bits, i, n);
     const uint32_t inc = (r & 0x1fffu) < (mag & 0x1fffu) ? 0x2000u : 0u;
     const uint32_t rounded = (mag & 0xffffe000u) + inc;
-    return static_cast<T>(__uint_as_float((bits & 0x80000000u) | rounded));
+    return static_cast<T>(as_float((bits & 0x80000000u) | rounded));
   }
 };
 
 struct StoreBF16 {
   using T = uint16_t;
   static constexpr int kBytes = 2;
-  __device__ __forceinline__ static float load(const T* p) {
-    return __uint_as_float(static_cast<uint32_t>(*p) << 16);
-  }
-  __device__ __forceinline__ static T encode(float g, uint32_t i, uint32_t n) {
-    const uint32_t bits = __float_as_uint(g);
+  __host__ __device__ __forceinline__ static float decode(T v) { return as_float(static_cast<uint32_t>(v) << 16); }
+  __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
+  __host__ __device__ __forceinline__ static T encode(float g, uint32_t i, uint32_t n) {
+    const uint32_t bits = as_uint(g);
     const uint32_t mag = bits & 0x7fffffffu;
     if (mag >= 0x7f800000u) return static_cast<T>((bits >> 16) | (mag > 0x7f800000u ? 0x40u : 0u));
     const uint32_t r = storage_hash(bits, i, n);
@@ -87,5 +89,27 @@ struct StoreBF16 {
     return static_cast<T>((((bits & 0x80000000u) | (mag & 0xffff0000u)) + inc) >> 16);
   }
 };
+
+// Two consecutive storage elements moved by one load or store: one dword for 16-bit
+// storage, two for f32. The address of an x-shifted pull is only element-aligned; gfx950
+// serves such loads as single unaligned dword accesses.
+template <class T>
+struct TPair {
+  T lo, hi;
+};
+
+template <class S>
+__host__ __device__ __forceinline__ void load_pair(const typename S::T* p, float& a, float& b) {
+  TPair<typename S::T> v;
+  __builtin_memcpy(&v, p, sizeof v);
+  a = S::decode(v.lo);
+  b = S::decode(v.hi);
+}
+
+template <class S>
+__host__ __device__ __forceinline__ void store_pair(typename S::T* p, typename S::T a, typename S::T b) {
+  const TPair<typename S::T> v{a, b};
+  __builtin_memcpy(p, &v, sizeof v);
+}
 
 }  // namespace pw

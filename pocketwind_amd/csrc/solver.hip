@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -120,6 +121,13 @@ void launch_init(pw_solver* s, void* out, const StepParams& p) {
 template <class S, int OP, bool LES, bool BB>
 void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const StepParams& p) {
   constexpr bool BOUZIDI = !BB;
+  if (s->kernel == PW_KERNEL_PAIR) {
+    auto k = macro ? stream_collide_pair_kernel<S, OP, LES, BOUZIDI, true>
+                   : stream_collide_pair_kernel<S, OP, LES, BOUZIDI, false>;
+    hipLaunchKernelGGL(k, dim3(blocks_for(s->cells / 2)), dim3(kBlock), 0, s->stream,
+                       (const typename S::T*)in, (typename S::T*)out, s->geometry, s->macro, p);
+    return;
+  }
   auto k = macro ? stream_collide_kernel<S, OP, LES, BOUZIDI, true> : stream_collide_kernel<S, OP, LES, BOUZIDI, false>;
   hipLaunchKernelGGL(k, dim3(blocks_for(s->cells)), dim3(kBlock), 0, s->stream,
                      (const typename S::T*)in, (typename S::T*)out, s->geometry, s->macro, p);
@@ -168,6 +176,41 @@ void launch_force_s(pw_solver* s) {
   }
   hipLaunchKernelGGL(force_reduce_kernel, dim3(1), dim3(kForceBlock), 0, s->stream, s->partials,
                      (uint32_t)s->n_partials, s->force_dev);
+}
+
+// Time each applicable step variant on the handle's own buffers and keep the fastest. The
+// populations are re-initialised afterwards, so tuning leaves no trace in the state.
+hipError_t autotune(pw_solver* s) {
+  if (s->layout.nx % 2 != 0) {
+    s->kernel = PW_KERNEL_CELL;
+    return hipSuccess;
+  }
+  const uint32_t candidates[2] = {PW_KERNEL_CELL, PW_KERNEL_PAIR};
+  float best = 0;
+  uint32_t choice = PW_KERNEL_PAIR;
+  const int reps = s->cells >= (1u << 20) ? 4 : 2;
+  for (uint32_t k : candidates) {
+    s->kernel = k;
+    launch_step(s, false, s->sp);  // warm-up
+    s->current ^= 1;
+    hipError_t e = hipEventRecord(s->t0, s->stream);
+    if (e != hipSuccess) return e;
+    for (int r = 0; r < reps; ++r) {
+      launch_step(s, false, s->sp);
+      s->current ^= 1;
+    }
+    if ((e = hipEventRecord(s->t1, s->stream)) != hipSuccess) return e;
+    if ((e = hipEventSynchronize(s->t1)) != hipSuccess) return e;
+    float ms = 0;
+    if ((e = hipEventElapsedTime(&ms, s->t0, s->t1)) != hipSuccess) return e;
+    if (best == 0 || ms < best) {
+      best = ms;
+      choice = k;
+    }
+  }
+  s->kernel = choice;
+  s->current = 0;
+  return hipGetLastError();
 }
 
 void free_all(pw_solver* s) {
@@ -299,6 +342,11 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   if ((e = hipEventCreate(&s->t0)) != hipSuccess || (e = hipEventCreate(&s->t1)) != hipSuccess) return fail(device_error(e, "event"));
   if ((e = hipMemcpyAsync(s->geometry, geometry, geometry_words * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "geometry upload"));
   if (n_boundary && (e = hipMemcpyAsync(s->boundary, boundary_cells, n_boundary * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "boundary upload"));
+  launch_init_both(s, full);
+  const char* forced = std::getenv("PW_STEP_KERNEL");
+  if (forced && std::strcmp(forced, "cell") == 0) s->kernel = PW_KERNEL_CELL;
+  else if (forced && std::strcmp(forced, "pair") == 0 && L.nx % 2 == 0) s->kernel = PW_KERNEL_PAIR;
+  else if ((e = autotune(s)) != hipSuccess) return fail(device_error(e, "step autotune"));
   launch_init_both(s, sp);
   if ((e = hipGetLastError()) != hipSuccess) return fail(device_error(e, "init launch"));
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return fail(device_error(e, "create"));
@@ -436,6 +484,38 @@ pw_status pw_solver_device_state(pw_solver* s, pw_device_state* out) {
   out->stream = (void*)s->stream;
   return PW_OK;
 }
+
+pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
+  if (pw_status st = check_handle(s)) return st;
+  if (kernel == PW_KERNEL_AUTO) {
+    PW_TRY(hipStreamSynchronize(s->stream), "tune");
+    // Tuning steps advance the state: keep a copy of the latest set and restore it.
+    const size_t bytes = pw_solver_population_bytes(s);
+    void* keep = nullptr;
+    PW_TRY(hipMalloc(&keep, bytes), "tune scratch");
+    const int cur = s->current;
+    hipError_t e = hipMemcpyAsync(keep, s->pops[cur], bytes, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = autotune(s);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->pops[cur], keep, bytes, hipMemcpyDeviceToDevice, s->stream);
+    s->current = cur;
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    hipFree(keep);
+    if (e != hipSuccess) return device_error(e, "tune");
+    return PW_OK;
+  }
+  if (kernel == PW_KERNEL_PAIR && s->layout.nx % 2 != 0) {
+    set_error("The paired step kernel needs an even nx.");
+    return PW_EINVAL;
+  }
+  if (kernel != PW_KERNEL_CELL && kernel != PW_KERNEL_PAIR) {
+    set_error("Unknown step kernel.");
+    return PW_EINVAL;
+  }
+  s->kernel = kernel;
+  return PW_OK;
+}
+
+uint32_t pw_solver_kernel(const pw_solver* s) { return s ? s->kernel : 0; }
 
 pw_status pw_solver_synchronize(pw_solver* s) {
   if (pw_status st = check_handle(s)) return st;

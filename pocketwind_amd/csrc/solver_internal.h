@@ -54,4 +54,5 @@ struct pw_solver {
   int ramp_settled = 0;
   long long first_fluid = -1;
   int destroyed = 0;
+  uint32_t kernel = 1;  // pw_kernel actually used by step()
 };

@@ -19,14 +19,14 @@ struct Stress {
 };
 
 // common.wgsl:84-88: equilibrium with the density deviation kept apart from unit density.
-__device__ __forceinline__ float geq(int i, float rho_delta, float ux, float uy, float uz,
+__host__ __device__ __forceinline__ float geq(int i, float rho_delta, float ux, float uy, float uz,
                                      float uu) {
   const float cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
   return wf(i) * (rho_delta + (1.0f + rho_delta) * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu));
 }
 
 // common.wgsl:100-113
-__device__ __forceinline__ Stress neq_stress(const float* g, const float* eq) {
+__host__ __device__ __forceinline__ Stress neq_stress(const float* g, const float* eq) {
   Stress s{0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
@@ -43,14 +43,14 @@ __device__ __forceinline__ Stress neq_stress(const float* g, const float* eq) {
 
 // common.wgsl:115-122 (closed-form Smagorinsky tau; docs/numerics.md "LES").
 template <bool LES>
-__device__ __forceinline__ float relaxation_time(const StepParams& p, const Stress& s, float rho) {
+__host__ __device__ __forceinline__ float relaxation_time(const StepParams& p, const Stress& s, float rho) {
   if (!LES) return p.tau0;
   const float q = s.xx * s.xx + s.yy * s.yy + s.zz * s.zz +
                   2.0f * (s.xy * s.xy + s.xz * s.xz + s.yz * s.yz);
   return 0.5f * (p.tau0 + sqrtf(p.tau0 * p.tau0 + p.cs_term * sqrtf(q) / rho));
 }
 
-__device__ __forceinline__ float projected(int i, const Stress& s, float third) {
+__host__ __device__ __forceinline__ float projected(int i, const Stress& s, float third) {
   const float x = cx(i), y = cy(i), z = cz(i);
   return x * x * s.xx + y * y * s.yy + z * z * s.zz + 2.0f * (x * y * s.xy + x * z * s.xz + y * z * s.yz) -
          third;
@@ -58,7 +58,7 @@ __device__ __forceinline__ float projected(int i, const Stress& s, float third) 
 
 // collide-trt.wgsl:1-15 and collide-reg.wgsl:3-18; writes post into g.
 template <int OP, bool LES>
-__device__ __forceinline__ void collide(const StepParams& p, float* g, const float* eq, float rho) {
+__host__ __device__ __forceinline__ void collide(const StepParams& p, float* g, const float* eq, float rho) {
   Stress s{0, 0, 0, 0, 0, 0};
   if (LES || OP == kOpReg) s = neq_stress(g, eq);
   const float tau = relaxation_time<LES>(p, s, rho);
@@ -85,7 +85,7 @@ __device__ __forceinline__ void collide(const StepParams& p, float* g, const flo
 }
 
 // stream_collide.wgsl:8-12: Werner-Wengle friction velocity at y = 1/2.
-__device__ __forceinline__ float friction_velocity(float nu, float speed) {
+__host__ __device__ __forceinline__ float friction_velocity(float nu, float speed) {
   const float viscous = sqrtf(nu * speed / 0.5f);
   if (0.5f * viscous / nu <= 11.81f) return viscous;
   return powf(speed * powf(nu / 0.5f, 1.0f / 7.0f) / 8.3f, 1.0f / (1.0f + 1.0f / 7.0f));
@@ -94,7 +94,7 @@ __device__ __forceinline__ float friction_velocity(float nu, float speed) {
 // stream_collide.wgsl:18-80: prescribe the smallest post-collision stress whose bounce-back
 // flux along the slip is the wall-law shear; falls back to the bulk collision.
 template <int OP>
-__device__ __forceinline__ void collide_wall(const StepParams& p, float* g, const float* eq, float rho,
+__host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float* g, const float* eq, float rho,
                                              float ux, float uy, float uz, uint32_t flag) {
   float nx = 0, ny = 0, nz = 0, exposed = 0;
 #pragma unroll
@@ -165,7 +165,7 @@ __device__ __forceinline__ void collide_wall(const StepParams& p, float* g, cons
 // xo[dx+1] + yo[dy+1] + zo[dz+1] with every axis periodic (common.wgsl:64-68).
 struct Neighbours {
   uint32_t xo[3], yo[3], zo[3];
-  __device__ __forceinline__ Neighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p) {
+  __host__ __device__ __forceinline__ Neighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p) {
     const uint32_t row = p.nx, plane = p.nx * p.ny;
     xo[0] = x == 0 ? p.nx - 1 : x - 1;
     xo[1] = x;
@@ -177,7 +177,7 @@ struct Neighbours {
     zo[1] = z * plane;
     zo[2] = (z + 1 == p.nz ? 0 : z + 1) * plane;
   }
-  __device__ __forceinline__ uint32_t at(int dx, int dy, int dz) const {
+  __host__ __device__ __forceinline__ uint32_t at(int dx, int dy, int dz) const {
     return xo[dx + 1] + yo[dy + 1] + zo[dz + 1];
   }
 };
@@ -185,7 +185,7 @@ struct Neighbours {
 // walls.wgsl:6-21: halfway bounce-back or linear Bouzidi for incoming direction i, evaluated
 // in R (f32 inside the step, f64 inside the drag reduction).
 template <class S, bool BOUZIDI, class R = float>
-__device__ __forceinline__ R wall_value(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
+__host__ __device__ __forceinline__ R wall_value(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
                                         const StepParams& p, const Neighbours& nb, uint32_t n,
                                         uint32_t flag, int i) {
   const size_t N = p.cells;
@@ -195,7 +195,7 @@ __device__ __forceinline__ R wall_value(const typename S::T* __restrict__ in, co
   const uint32_t base = geo[N + n];
   if (base == kNoQ) return reflected;
   const uint32_t rank = __builtin_popcount((flag >> 3) & ((1u << i) - 1u));
-  const R q = __uint_as_float(geo[2 * N + base + rank]);
+  const R q = pw::as_float(geo[2 * N + base + rank]);
   if (q == R(0.5)) return reflected;
   if (q < R(0.5)) {
     const uint32_t second = nb.at(cx(i), cy(i), cz(i));
@@ -206,14 +206,14 @@ __device__ __forceinline__ R wall_value(const typename S::T* __restrict__ in, co
 }
 
 template <class S>
-__device__ __forceinline__ void store_state(typename S::T* __restrict__ out, size_t N, uint32_t n, const float* g) {
+__host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__ out, size_t N, uint32_t n, const float* g) {
 #pragma unroll
   for (int i = 0; i < kQ; ++i) out[(size_t)i * N + n] = S::encode(g[i], (uint32_t)i, n);
 }
 
 // common.wgsl:145-150 with rho - 1 = 0.
 template <class S>
-__device__ __forceinline__ void store_equilibrium(typename S::T* __restrict__ out, size_t N, uint32_t n,
+__host__ __device__ __forceinline__ void store_equilibrium(typename S::T* __restrict__ out, size_t N, uint32_t n,
                                                   float ux, float uy, float uz) {
   const float uu = ux * ux + uy * uy + uz * uz;
   float g[kQ];
@@ -235,14 +235,38 @@ __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ o
   macro[n] = make_float4(1.0f, ux, 0.0f, 0.0f);
 }
 
+// Moments, equilibrium and collision of one gathered fluid cell (stream_collide.wgsl:114-133);
+// g becomes the post-collision state; returns (rho, u).
+template <int OP, bool LES>
+__host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, uint32_t flag) {
+  float rho_delta = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    rho_delta += g[i];
+    mx += cx(i) * g[i];
+    my += cy(i) * g[i];
+    mz += cz(i) * g[i];
+  }
+  const float rho = 1.0f + rho_delta;
+  const float ux = mx / rho, uy = my / rho, uz = mz / rho;
+  const float uu = ux * ux + uy * uy + uz * uz;
+  float eq[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) eq[i] = geq(i, rho_delta, ux, uy, uz, uu);
+  if (LES && (flag >> 3) != 0)
+    collide_wall<OP>(p, g, eq, rho, ux, uy, uz, flag);
+  else
+    collide<OP, LES>(p, g, eq, rho);
+  return make_float4(rho, ux, uy, uz);
+}
+
 // stream_collide.wgsl:95-136, one cell per lane.
 template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
-__global__ __launch_bounds__(256) void stream_collide_kernel(const typename S::T* __restrict__ in,
+__host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
-                                                             float4* __restrict__ macro, StepParams p) {
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= p.cells) return;
+                                                             float4* __restrict__ macro,
+                                                             const StepParams& p, uint32_t n) {
   const size_t N = p.cells;
   const uint32_t flag = geo[n];
   const uint32_t kind = flag & kTypeMask;
@@ -284,26 +308,62 @@ __global__ __launch_bounds__(256) void stream_collide_kernel(const typename S::T
       g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i)
                                : S::load(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)));
   }
-  float rho_delta = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    rho_delta += g[i];
-    mx += cx(i) * g[i];
-    my += cy(i) * g[i];
-    mz += cz(i) * g[i];
-  }
-  const float rho = 1.0f + rho_delta;
-  const float ux = mx / rho, uy = my / rho, uz = mz / rho;
-  const float uu = ux * ux + uy * uy + uz * uz;
-  float eq[kQ];
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) eq[i] = geq(i, rho_delta, ux, uy, uz, uu);
-  if (LES && (flag >> 3) != 0)
-    collide_wall<OP>(p, g, eq, rho, ux, uy, uz, flag);
-  else
-    collide<OP, LES>(p, g, eq, rho);
+  const float4 m = relax<OP, LES>(p, g, flag);
   store_state<S>(out, N, n, g);
-  if (MACRO) macro[n] = make_float4(rho, ux, uy, uz);
+  if (MACRO) macro[n] = m;
+}
+
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
+__global__ __launch_bounds__(256) void stream_collide_kernel(const typename S::T* __restrict__ in,
+                                                             typename S::T* __restrict__ out,
+                                                             const uint32_t* __restrict__ geo,
+                                                             float4* __restrict__ macro, StepParams p) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.cells) return;
+  stream_collide_cell<S, OP, LES, BOUZIDI, MACRO>(in, out, geo, macro, p, n);
+}
+
+// Two x-adjacent cells per lane. A pair whose cells are both plain fluid (no wall links) and
+// whose pulls do not wrap in x gathers and stores each direction with one wide access;
+// every other pair takes the per-cell path. Needs an even nx.
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
+__host__ __device__ __forceinline__ void stream_collide_pair(const typename S::T* __restrict__ in,
+                                                             typename S::T* __restrict__ out,
+                                                             const uint32_t* __restrict__ geo,
+                                                             float4* __restrict__ macro,
+                                                             const StepParams& p, uint32_t n) {
+  const uint2 f = *reinterpret_cast<const uint2*>(geo + n);
+  const uint32_t x = n % p.nx;
+  if ((f.x | f.y) != 0 || x == 0 || x + 2 == p.nx) {
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO>(in, out, geo, macro, p, n);
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO>(in, out, geo, macro, p, n + 1);
+    return;
+  }
+  const size_t N = p.cells;
+  const uint32_t r = n / p.nx;
+  const Neighbours nb(x, r % p.ny, r / p.ny, p);
+  float a[kQ], b[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) load_pair<S>(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)), a[i], b[i]);
+  const float4 ma = relax<OP, LES>(p, a, 0u);
+  const float4 mb = relax<OP, LES>(p, b, 0u);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i)
+    store_pair<S>(out + (size_t)i * N + n, S::encode(a[i], (uint32_t)i, n), S::encode(b[i], (uint32_t)i, n + 1));
+  if (MACRO) {
+    macro[n] = ma;
+    macro[n + 1] = mb;
+  }
+}
+
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
+__global__ __launch_bounds__(256) void stream_collide_pair_kernel(const typename S::T* __restrict__ in,
+                                                                  typename S::T* __restrict__ out,
+                                                                  const uint32_t* __restrict__ geo,
+                                                                  float4* __restrict__ macro, StepParams p) {
+  const uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (n >= p.cells) return;
+  stream_collide_pair<S, OP, LES, BOUZIDI, MACRO>(in, out, geo, macro, p, n);
 }
 
 }  // namespace pw

