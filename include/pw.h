@@ -140,6 +140,14 @@ typedef enum { PW_KERNEL_AUTO = 0, PW_KERNEL_CELL = 1, PW_KERNEL_PAIR = 2 } pw_k
 pw_status pw_solver_set_kernel(pw_solver *s, uint32_t kernel);
 uint32_t pw_solver_kernel(const pw_solver *s);
 
+/* The validation stepper stepWithForces (src/app/case-run.ts:154-169) without a host round
+ * trip per sample: queue `count` steps and, after each step whose count is a multiple of
+ * `every`, one FP64 force sample (the encodeForce semantics) into a device history, read back
+ * once at the end into `out` as 4 doubles per sample (step, fx, fy, fz). Does not touch the
+ * encodeForce/readForce slot. Blocks until done. */
+pw_status pw_solver_step_sampled(pw_solver *s, uint32_t count, uint32_t every, double *out,
+                                 size_t out_doubles);
+
 /* Wait for all queued work of the handle. */
 pw_status pw_solver_synchronize(pw_solver *s);
 
@@ -151,6 +159,46 @@ pw_status pw_solver_time_steps(pw_solver *s, uint32_t count, float *elapsed_ms);
 /* measureCopyBandwidth (src/app/bench-page.ts:204-242): device-to-device float4 copy of
  * `bytes`, `passes` times; returns GB/s of read+write traffic. */
 pw_status pw_copy_bandwidth(size_t bytes, uint32_t passes, double *gbps);
+
+/* VoxelGrid (src/geometry/voxel-grid.ts:7-15); arrays are malloc'd by pw_voxelize and
+ * released by pw_voxel_grid_free. */
+typedef struct {
+  uint32_t *geometry; /* [flags N][q offsets N][f32 q per link] */
+  size_t geometry_words;
+  uint32_t *boundary_cells; /* fluid cells with a SOLID pull neighbour, in cell order */
+  size_t n_boundary;
+  uint64_t solid_cells;
+  uint64_t frontal_cells;
+  double blockage;
+  int32_t watertight;
+  uint64_t missed_links;
+} pw_voxel_grid;
+
+/* voxelize(body, supports, layout, {floor, linkFractions}) (src/geometry/voxelize.ts:187-214,
+ * wall-links.ts:99-131) on the GPU: bit-exact flags, q offsets, link fractions and boundary
+ * list. body/supports are 9 floats per triangle in lattice cells; supports may be NULL. */
+pw_status pw_voxelize(const float *body, size_t body_tris, const float *supports,
+                      size_t support_tris, const pw_layout *layout, int floor, int link_fractions,
+                      pw_voxel_grid *out);
+void pw_voxel_grid_free(pw_voxel_grid *grid);
+
+/* Streakline particles bound to a solver's macro field and flags (createParticles,
+ * src/render/particles.ts:30-138; seeds src/render/particle-layout.ts:126-172). A record is
+ * 8 floats: pos.xyz, age, prev.xyz, generation. count may be 1 .. 2^24 (the reference caps
+ * it at 65536). Updates run on the solver's stream after its queued steps. Destroy the
+ * particles before their solver. */
+typedef struct pw_particles pw_particles;
+pw_status pw_particles_create(pw_solver *solver, uint32_t count, uint32_t seed,
+                              const double body_min[3], const double body_max[3],
+                              pw_particles **out);
+/* particles.update(encoder, steps) (src/render/particles.ts:113-124,
+ * src/render/shaders/particles_update.wgsl:46-60). */
+pw_status pw_particles_update(pw_particles *p, float steps);
+pw_status pw_particles_read(pw_particles *p, float *out, size_t out_floats);
+pw_status pw_particles_write(pw_particles *p, const float *in, size_t in_floats);
+/* The seed cross-section (y0, z0, y1, z1) and the fallback inlet node (x, y, z). */
+pw_status pw_particles_info(const pw_particles *p, float region[4], float fallback[3]);
+void pw_particles_destroy(pw_particles *p);
 
 #ifdef __cplusplus
 }

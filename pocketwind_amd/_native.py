@@ -35,6 +35,13 @@ class DeviceState(C.Structure):
 
 
 _u32p = C.POINTER(C.c_uint32)
+
+
+class VoxelGridC(C.Structure):
+    _fields_ = [("geometry", C.POINTER(C.c_uint32)), ("geometry_words", C.c_size_t),
+                ("boundary_cells", C.POINTER(C.c_uint32)), ("n_boundary", C.c_size_t),
+                ("solid_cells", C.c_uint64), ("frontal_cells", C.c_uint64),
+                ("blockage", C.c_double), ("watertight", C.c_int32), ("missed_links", C.c_uint64)]
 _lib = None
 
 # Every entry point of include/pw.h with its ctypes signature (restype, argtypes).
@@ -59,8 +66,19 @@ SIGNATURES = {
     "pw_solver_set_kernel": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pw_solver_kernel": (C.c_uint32, [C.c_void_p]),
     "pw_solver_synchronize": (C.c_int, [C.c_void_p]),
+    "pw_solver_step_sampled": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t]),
     "pw_solver_time_steps": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
     "pw_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_uint32, C.POINTER(C.c_double)]),
+    "pw_voxelize": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.POINTER(Layout),
+                              C.c_int, C.c_int, C.POINTER(VoxelGridC)]),
+    "pw_voxel_grid_free": (None, [C.POINTER(VoxelGridC)]),
+    "pw_particles_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_void_p)]),
+    "pw_particles_update": (C.c_int, [C.c_void_p, C.c_float]),
+    "pw_particles_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_particles_write": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_particles_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "pw_particles_destroy": (None, [C.c_void_p]),
 }
 
 

@@ -7,7 +7,7 @@ float64 and stored as float32, as the reference's Float32Array meshes are. ``blu
 is the 1.0 x 0.37 x 0.28 slanted body of the 256x128x128 benchmark (front edges rounded at
 0.1 L, 25-degree rear slant over the last 0.2 L), built with the same loft.
 
-``noisy_bluff_body`` subdivides that body into a fine triangle mesh (about 200k triangles)
+``noisy_bluff_body`` subdivides that body into a fine triangle mesh (217 088 triangles)
 with seeded vertex noise, the procedural mesh of the full-pipeline workload.
 """
 from __future__ import annotations
@@ -139,11 +139,12 @@ def subdivide(mesh: np.ndarray, levels: int) -> np.ndarray:
     return t.astype(np.float32).reshape(-1)
 
 
-def noisy_bluff_body(levels: int = 6, amplitude: float = 0.002, seed: int = 1234) -> np.ndarray:
+def noisy_bluff_body(levels: int = 5, segments: int = 24, amplitude: float = 0.002,
+                     seed: int = 1234) -> np.ndarray:
     """Fine slanted bluff body with seeded vertex noise, watertight: every distinct vertex
-    moves once, so shared vertices stay shared. 12 nose segments give 114 triangles; six
-    subdivisions make 114 * 4^6 = 466 944, five make 116 736."""
-    t = subdivide(bluff_body(), levels).reshape(-1, 3)
+    moves once, so shared vertices stay shared. A loft of s nose segments has 8 (s + 2) + 4
+    triangles; the default (24 segments, 5 subdivisions) gives 212 * 4^5 = 217 088."""
+    t = subdivide(bluff_body(segments=segments), levels).reshape(-1, 3)
     keys, inverse = np.unique(t, axis=0, return_inverse=True)
     rng = np.random.default_rng(seed)
     moved = keys.astype(np.float64) + rng.uniform(-amplitude, amplitude, keys.shape)

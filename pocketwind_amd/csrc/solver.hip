@@ -166,7 +166,7 @@ void launch_init_both(pw_solver* s, const StepParams& p) {
 }
 
 template <class S>
-void launch_force_s(pw_solver* s) {
+void launch_force_s(pw_solver* s, double* dest) {
   const unsigned blocks = (unsigned)s->n_partials;
   if (blocks > 0) {
     auto k = s->params.walls == PW_WALLS_BOUZIDI ? force_partials_kernel<S, true> : force_partials_kernel<S, false>;
@@ -175,7 +175,15 @@ void launch_force_s(pw_solver* s) {
                        s->partials, s->sp);
   }
   hipLaunchKernelGGL(force_reduce_kernel, dim3(1), dim3(kForceBlock), 0, s->stream, s->partials,
-                     (uint32_t)s->n_partials, s->force_dev);
+                     (uint32_t)s->n_partials, dest);
+}
+
+void launch_force(pw_solver* s, double* dest) {
+  switch (s->params.storage) {
+    case PW_STORAGE_F16: launch_force_s<StoreF16>(s, dest); break;
+    case PW_STORAGE_BF16: launch_force_s<StoreBF16>(s, dest); break;
+    default: launch_force_s<StoreF32>(s, dest); break;
+  }
 }
 
 // Time each applicable step variant on the handle's own buffers and keep the fastest. The
@@ -374,11 +382,7 @@ pw_status pw_solver_step(pw_solver* s, uint32_t count, int write_macro_on_last) 
 pw_status pw_solver_encode_force(pw_solver* s) {
   if (pw_status st = check_handle(s)) return st;
   if (s->force_encoded) return PW_OK;  // gpu-force.ts:176: an occupied slot skips the sample
-  switch (s->params.storage) {
-    case PW_STORAGE_F16: launch_force_s<StoreF16>(s); break;
-    case PW_STORAGE_BF16: launch_force_s<StoreBF16>(s); break;
-    default: launch_force_s<StoreF32>(s); break;
-  }
+  launch_force(s, s->force_dev);
   PW_TRY(hipGetLastError(), "force launch");
   PW_TRY(hipMemcpyAsync(s->force_host, s->force_dev, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream), "force readback");
   PW_TRY(hipEventRecord(s->force_event, s->stream), "force event");
@@ -520,6 +524,47 @@ uint32_t pw_solver_kernel(const pw_solver* s) { return s ? s->kernel : 0; }
 pw_status pw_solver_synchronize(pw_solver* s) {
   if (pw_status st = check_handle(s)) return st;
   PW_TRY(hipStreamSynchronize(s->stream), "synchronize");
+  return PW_OK;
+}
+
+pw_status pw_solver_step_sampled(pw_solver* s, uint32_t count, uint32_t every, double* out, size_t out_doubles) {
+  if (pw_status st = check_handle(s)) return st;
+  if (every == 0 || !out) {
+    set_error("Sampling needs a positive interval and an output buffer.");
+    return PW_EINVAL;
+  }
+  size_t samples = 0;
+  for (uint64_t k = s->steps + 1; k <= s->steps + count; ++k) samples += (k % every) == 0;
+  if (out_doubles < 4 * samples) {
+    set_error("Sample output needs 4 doubles per sample.");
+    return PW_EINVAL;
+  }
+  double* hist = nullptr;
+  if (samples) PW_TRY(hipMallocAsync((void**)&hist, samples * 3 * sizeof(double), s->stream), "sample history");
+  size_t k = 0;
+  std::vector<uint64_t> stamps;
+  for (uint32_t n = 0; n < count; ++n) {
+    if (pw_status st = pw_solver_step(s, 1, 0)) {
+      if (hist) (void)hipFreeAsync(hist, s->stream);
+      return st;
+    }
+    if (s->steps % every == 0) {
+      launch_force(s, hist + 3 * k++);
+      stamps.push_back(s->steps);
+    }
+  }
+  std::vector<double> host(3 * samples);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && samples) e = hipMemcpyAsync(host.data(), hist, host.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (hist) (void)hipFreeAsync(hist, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return device_error(e, "sampled steps");
+  for (size_t j = 0; j < samples; ++j) {
+    out[4 * j] = (double)stamps[j];
+    out[4 * j + 1] = host[3 * j];
+    out[4 * j + 2] = host[3 * j + 1];
+    out[4 * j + 3] = host[3 * j + 2];
+  }
   return PW_OK;
 }
 

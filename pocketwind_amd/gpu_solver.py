@@ -199,6 +199,16 @@ class GpuSolver:
     def synchronize(self) -> None:
         N.check(N.lib().pw_solver_synchronize(self._live()))
 
+    def step_sampled(self, count: int, every: int = FORCE_EVERY_STEPS) -> np.ndarray:
+        """Advance `count` steps, sampling the force after every `every`-th step on the device;
+        returns an (n, 4) array of (step, fx, fy, fz)."""
+        first = self.step_count + 1
+        n = sum(1 for k in range(first, first + count) if k % every == 0)
+        out = np.zeros((max(n, 1), 4), np.float64)
+        N.check(N.lib().pw_solver_step_sampled(self._live(), int(count), int(every), out.ctypes.data,
+                                               out.size))
+        return out[:n]
+
     def time_steps(self, count: int) -> float:
         """Milliseconds for `count` steps between two events on the solver's stream."""
         ms = C.c_float()

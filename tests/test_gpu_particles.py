@@ -1,0 +1,113 @@
+"""Streakline particles on the GPU against the f32 oracle restatement of
+particles_update.wgsl: seeds equal particleSeeds (particle-layout.ts:126-172), and advection
+over a solver's live macro field (interpolation, one-cell cap, respawn by hash at the inlet,
+age limit) is bit-identical to the oracle for the same macro field and flags. Also the
+reference's browser known values (tests/browser/particles.spec.ts: no particle ever sits in a
+non-fluid cell after many frames)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pocketwind_amd import bodies
+from pocketwind_amd.gpu_solver import GpuSolver, SolverParams
+from pocketwind_amd.particles import Particles
+from pocketwind_amd.voxelize import voxelize
+
+pytestmark = pytest.mark.gpu
+
+
+def sphere_case(layout=(64, 32, 32)):
+    body, _ = bodies.place_mesh(bodies.sphere_mesh(), None, 8, layout, False)
+    grid = voxelize(body, None, layout, False, link_fractions=False)
+    lo, hi = bodies.mesh_bounds(body)
+    return grid, (lo, hi)
+
+
+def test_seeds_match_oracle():
+    grid, bounds = sphere_case()
+    solver = GpuSolver.create(grid.layout, grid, SolverParams(0.6, 0.05, False))
+    try:
+        p = Particles.create(solver, bounds, 4096, 12345)
+        data, region, fallback = O.particle_seeds(grid.flags, grid.layout, bounds[0], bounds[1], 4096, 12345)
+        assert np.array_equal(p.read().reshape(-1).view(np.uint32), data.view(np.uint32))
+        r, f = p.info()
+        assert np.array_equal(r, region.astype(np.float32)) and np.array_equal(f, fallback)
+        p.destroy()
+    finally:
+        solver.destroy()
+
+
+@pytest.mark.parametrize("steps_per_frame", [1.0, 8.0, 40.0])
+def test_advection_bit_identical_to_oracle(steps_per_frame):
+    grid, bounds = sphere_case()
+    solver = GpuSolver.create(grid.layout, grid, SolverParams(0.6, 0.05, False))
+    try:
+        solver.step(200, True)
+        p = Particles.create(solver, bounds, 8192, 7)
+        state = p.read().reshape(-1)
+        region, fallback = p.info()
+        macro = solver.read_macro()
+        flags = grid.flags
+        for frame in range(60):
+            p.update(steps_per_frame)
+            state = O.particles_update(state, macro, flags, grid.layout, 7, steps_per_frame, region,
+                                       fallback[1:])
+        gpu = p.read().reshape(-1)
+        assert np.array_equal(gpu.view(np.uint32), state.view(np.uint32))
+        assert np.any(gpu.reshape(-1, 8)[:, 7] > 0), "no particle respawned"
+        p.destroy()
+    finally:
+        solver.destroy()
+
+
+def test_particles_never_rest_in_non_fluid_cells():
+    grid, bounds = sphere_case()
+    solver = GpuSolver.create(grid.layout, grid, SolverParams(0.6, 0.05, False))
+    try:
+        p = Particles.create(solver, bounds, 65536, 3)
+        kinds = grid.flags & 7
+        for frame in range(300):
+            solver.step(8, True)
+            p.update(8)
+        pos = p.read()[:, :3]
+        cells = np.floor(pos + 0.5).astype(np.int64)
+        n = cells[:, 0] + 64 * (cells[:, 1] + 32 * cells[:, 2])
+        assert np.all(kinds[n] == 0)
+        assert np.all(pos[:, 0] < 62)
+        p.destroy()
+    finally:
+        solver.destroy()
+
+
+def test_age_limit_respawns_at_inlet():
+    grid, bounds = sphere_case()
+    solver = GpuSolver.create(grid.layout, grid, SolverParams(0.6, 0.0, False))
+    try:
+        p = Particles.create(solver, bounds, 64, 1)
+        rec = p.read()
+        rec[:, 3] = 4000.0  # next frame makes age 4001 > 4000
+        p.write(rec)
+        p.update(1)
+        after = p.read()
+        assert np.all(after[:, 0] == 2) and np.all(after[:, 3] == 0) and np.all(after[:, 7] == 1)
+        p.destroy()
+    finally:
+        solver.destroy()
+
+
+def test_rejects_invalid_counts_and_regions():
+    grid, bounds = sphere_case()
+    solver = GpuSolver.create(grid.layout, grid, SolverParams(0.6, 0.05, False))
+    try:
+        for count in (0, 65537):
+            with pytest.raises(ValueError):
+                Particles.create(solver, bounds, count, 1)
+        with pytest.raises(ValueError):
+            Particles.create(solver, ((0, 30, 30), (1, 40, 40)), 1, 1)
+        big = Particles.create(solver, bounds, 1 << 20, 1, max_count=1 << 24)
+        assert big.read().shape == (1 << 20, 8)
+        big.destroy()
+    finally:
+        solver.destroy()
