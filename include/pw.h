@@ -133,10 +133,17 @@ typedef struct {
 pw_status pw_solver_device_state(pw_solver *s, pw_device_state *out);
 
 /* Step-kernel variants: one cell per lane, or two x-adjacent cells per lane moving each
- * direction with one wide access. AUTO times both on the handle's buffers (at create, or
- * when set later, restoring the state afterwards) and keeps the faster. No reference
- * counterpart: the reference has one WGSL kernel (src/lbm/shaders/stream_collide.wgsl). */
-typedef enum { PW_KERNEL_AUTO = 0, PW_KERNEL_CELL = 1, PW_KERNEL_PAIR = 2 } pw_kernel;
+ * direction with one wide access; each with cached or non-temporal population streams.
+ * AUTO times them on the handle's buffers (at create, or when set later, restoring the
+ * state afterwards) and keeps the fastest. No reference counterpart: the reference has one
+ * WGSL kernel (src/lbm/shaders/stream_collide.wgsl). */
+typedef enum {
+  PW_KERNEL_AUTO = 0,
+  PW_KERNEL_CELL = 1,
+  PW_KERNEL_PAIR = 2,
+  PW_KERNEL_CELL_NT = 3,
+  PW_KERNEL_PAIR_NT = 4
+} pw_kernel;
 pw_status pw_solver_set_kernel(pw_solver *s, uint32_t kernel);
 uint32_t pw_solver_kernel(const pw_solver *s);
 
@@ -199,6 +206,29 @@ pw_status pw_particles_write(pw_particles *p, const float *in, size_t in_floats)
 /* The seed cross-section (y0, z0, y1, z1) and the fallback inlet node (x, y, z). */
 pw_status pw_particles_info(const pw_particles *p, float region[4], float fallback[3]);
 void pw_particles_destroy(pw_particles *p);
+
+/* Batched tunnels: `count` independent tunnels of any sizes advanced by one persistent launch
+ * per step, even/odd step pairs replayed from a captured hipGraph. No reference counterpart
+ * (the reference allows one live solver, src/app/solver-slot.ts:14-44); each tunnel follows
+ * pw_solver semantics exactly and is bit-identical to the same tunnel run alone. Tunnel t
+ * takes geometries[t] (the packed geometry of pw_solver_create), boundary_cells[t], tau0[t]
+ * and u_in[t]; operator, walls, LES and storage come from `shared` (no ramp). */
+typedef struct pw_batch pw_batch;
+pw_status pw_batch_create(uint32_t count, const pw_layout *layouts, const uint32_t *const *geometries,
+                          const size_t *geometry_words, const uint32_t *const *boundary_cells,
+                          const size_t *n_boundary, const double *tau0, const double *u_in,
+                          const pw_params *shared, pw_batch **out);
+pw_status pw_batch_step(pw_batch *b, uint32_t count, int write_macro_on_last);
+/* As pw_solver_step_sampled for every tunnel: `out` receives [tunnel][sample] x
+ * (step, fx, fy, fz), 4 doubles each. */
+pw_status pw_batch_step_sampled(pw_batch *b, uint32_t count, uint32_t every, double *out,
+                                size_t out_doubles);
+pw_status pw_batch_read_macro(pw_batch *b, uint32_t tunnel, float *out, size_t out_floats);
+pw_status pw_batch_read_populations(pw_batch *b, uint32_t tunnel, void *out, size_t out_bytes);
+uint64_t pw_batch_step_count(const pw_batch *b);
+uint64_t pw_batch_cells(const pw_batch *b);
+pw_status pw_batch_time_steps(pw_batch *b, uint32_t count, float *elapsed_ms);
+void pw_batch_destroy(pw_batch *b);
 
 #ifdef __cplusplus
 }

@@ -79,22 +79,43 @@ SIGNATURES = {
     "pw_particles_write": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "pw_particles_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "pw_particles_destroy": (None, [C.c_void_p]),
+    "pw_batch_create": (C.c_int, [C.c_uint32, C.POINTER(Layout), C.POINTER(C.c_void_p),
+                                  C.POINTER(C.c_size_t), C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(Params),
+                                  C.POINTER(C.c_void_p)]),
+    "pw_batch_step": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int]),
+    "pw_batch_step_sampled": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "pw_batch_read_macro": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "pw_batch_read_populations": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "pw_batch_step_count": (C.c_uint64, [C.c_void_p]),
+    "pw_batch_cells": (C.c_uint64, [C.c_void_p]),
+    "pw_batch_time_steps": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
+    "pw_batch_destroy": (None, [C.c_void_p]),
 }
 
 
-def lib() -> C.CDLL:
+class _Library:
+    """libpocketwind.so with every entry point typed on first use."""
+
+    def __init__(self, handle: C.CDLL):
+        self._handle = handle
+
+    def __getattr__(self, name):
+        fn = getattr(self._handle, name)
+        if name in SIGNATURES:
+            fn.restype, fn.argtypes = SIGNATURES[name]
+        setattr(self, name, fn)
+        return fn
+
+
+def lib() -> "_Library":
     """Load libpocketwind.so or raise: the product has no fallback path."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing; build it with `python -m pocketwind_amd.build`.")
-        handle = C.CDLL(LIB_PATH)
-        for name, (restype, argtypes) in SIGNATURES.items():
-            fn = getattr(handle, name)
-            fn.restype = restype
-            fn.argtypes = argtypes
-        _lib = handle
+        _lib = _Library(C.CDLL(LIB_PATH))
     return _lib
 
 

@@ -5,18 +5,20 @@ this file so it travels with the source tree; it is not installed anywhere else.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import glob
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libpocketwind.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed-FP32 instructions are disabled for every kernel; the host pass ignores the feature.
-FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+COMPILE_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value", "-Wno-unused-result",
 ]
@@ -26,18 +28,36 @@ def sources() -> list[str]:
     return sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
 
 
-def build(verbose: bool = False) -> str:
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(HERE, "..", "include"), "-o", LIB + ".tmp", *sources()]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed to build libpocketwind.so")
-    warnings = [ln for ln in r.stderr.splitlines() if "packed-fp32-ops" not in ln and ln.strip()]
-    if verbose and warnings:
-        sys.stderr.write("\n".join(warnings) + "\n")
+def _compile(src: str, obj: str) -> tuple[str, subprocess.CompletedProcess]:
+    cmd = [HIPCC, *COMPILE_FLAGS, "-I", os.path.join(HERE, "..", "include"), "-c", "-o", obj, src]
+    return src, subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile every csrc/*.hip to an object in parallel, then link the shared library."""
+    srcs = sources()
+    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    with tempfile.TemporaryDirectory(prefix="pw_build_") as tmp:
+        objs = [os.path.join(tmp, os.path.basename(s) + ".o") for s in srcs]
+        with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+            results = list(pool.map(_compile, srcs, objs))
+        failed = [(src, r) for src, r in results if r.returncode != 0]
+        for src, r in results:
+            warnings = [ln for ln in r.stderr.splitlines() if "packed-fp32-ops" not in ln and ln.strip()]
+            if verbose and warnings:
+                sys.stderr.write(f"{os.path.basename(src)}:\n" + "\n".join(warnings[:40]) + "\n")
+        if failed:
+            for src, r in failed:
+                sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc failed to build libpocketwind.so")
+        link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs]
+        r = subprocess.run(link, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("linking libpocketwind.so failed")
     os.replace(LIB + ".tmp", LIB)
+    if verbose:
+        print(f"built {LIB} from {len(srcs)} sources", flush=True)
     return LIB
 
 

@@ -1,0 +1,448 @@
+// batch.hip -- many independent tunnels of uneven size stepped by one persistent launch.
+//
+// The reference runs one solver at a time (src/app/solver-slot.ts:14-44); batching has no
+// reference counterpart. Each tunnel keeps the single-solver semantics exactly (the same
+// per-cell and per-pair step code, its own tau, inlet speed, geometry, ping-pong sets and
+// macros), so a tunnel stepped in a batch is bit-identical to the same tunnel stepped alone.
+//
+// Work decomposition: the cells of all tunnels are cut into tiles of one block's worth of
+// cells; a tile table maps a tile to its tunnel. A persistent grid of blocks strides over the
+// tiles, so one launch advances every tunnel by one step. All tunnels step in lockstep, so
+// one parity bit selects every tunnel's input set; the two launches of an even/odd step pair
+// are captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "batch_step.h"
+
+namespace pw {
+namespace batch {
+
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void batch_init_kernel(const Tunnel* __restrict__ tunnels,
+                                                            const uint32_t* __restrict__ tile_tunnel,
+                                                            const uint32_t* __restrict__ tile_first,
+                                                            uint32_t n_tiles, int set) {
+  for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const uint32_t t = tile_tunnel[tile];
+    const Tunnel& tu = tunnels[t];
+    const uint32_t n = (tile - tile_first[t]) * kBlock + threadIdx.x;
+    if (n >= tu.p.cells) continue;
+    const uint32_t kind = tu.geo[n] & kTypeMask;
+    const float ux = is_wall(kind) ? 0.0f : tu.p.u_in;
+    store_equilibrium<S>((typename S::T*)tu.pops[set], tu.p.cells, n, ux, 0.0f, 0.0f);
+    tu.macro[n] = make_float4(1.0f, ux, 0.0f, 0.0f);
+  }
+}
+
+// Force partials of every tunnel: block b handles 256 entries of one tunnel's boundary list.
+template <class S, bool BOUZIDI>
+__global__ __launch_bounds__(kForceBlock) void batch_force_kernel(const Tunnel* __restrict__ tunnels,
+                                                                  const uint32_t* __restrict__ part_tunnel,
+                                                                  double* __restrict__ partials) {
+  const uint32_t t = part_tunnel[blockIdx.x];
+  const Tunnel& tu = tunnels[t];
+  const uint32_t item = (blockIdx.x - tu.partial_base) * kForceBlock + threadIdx.x;
+  double fx = 0, fy = 0, fz = 0;
+  if (item < tu.n_boundary) {
+    const uint32_t n = tu.boundary[item];
+    const uint32_t flag = tu.geo[n];
+    const StepParams& p = tu.p;
+    const uint32_t x = n % p.nx, r = n / p.nx, y = r % p.ny, z = r / p.ny;
+    const Neighbours nb(x, y, z, p);
+    double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+    cell_exchange<S, BOUZIDI>((const typename S::T*)tu.pops[0], tu.geo, p, nb, n, flag, ax, ay, az);
+    cell_exchange<S, BOUZIDI>((const typename S::T*)tu.pops[1], tu.geo, p, nb, n, flag, bx, by, bz);
+    fx = 0.5 * (ax + bx);
+    fy = 0.5 * (ay + by);
+    fz = 0.5 * (az + bz);
+  }
+  __shared__ double lds[3][kForceBlock / 64];
+  fx = wave_sum(fx);
+  fy = wave_sum(fy);
+  fz = wave_sum(fz);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) {
+    lds[0][wave] = fx;
+    lds[1][wave] = fy;
+    lds[2][wave] = fz;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double s = 0;
+    for (int w = 0; w < kForceBlock / 64; ++w) s += lds[threadIdx.x][w];
+    partials[3 * (size_t)blockIdx.x + threadIdx.x] = s;
+  }
+}
+
+// One block per tunnel reduces that tunnel's partials in the single-solver order.
+__global__ __launch_bounds__(kForceBlock) void batch_reduce_kernel(const Tunnel* __restrict__ tunnels,
+                                                                   const double* __restrict__ partials,
+                                                                   double* __restrict__ result) {
+  const Tunnel& tu = tunnels[blockIdx.x];
+  force_reduce_block(partials + 3 * (size_t)tu.partial_base, tu.n_partials, result + 3 * (size_t)blockIdx.x);
+}
+
+}  // namespace batch
+}  // namespace pw
+
+namespace pw {
+namespace batch {
+extern template void batch_step_s<StoreF32>(const BatchLaunch&, int, bool);
+extern template void batch_step_s<StoreF16>(const BatchLaunch&, int, bool);
+extern template void batch_step_s<StoreBF16>(const BatchLaunch&, int, bool);
+}  // namespace batch
+}  // namespace pw
+
+using namespace pw;
+using namespace pw::batch;
+
+struct pw_batch {
+  hipStream_t stream = nullptr;
+  pw_params params{};
+  uint32_t count = 0;
+  std::vector<pw_layout> layouts;
+  std::vector<Tunnel> host;
+  Tunnel* tunnels = nullptr;
+  uint32_t* tile_tunnel = nullptr;
+  uint32_t* tile_first = nullptr;
+  uint32_t n_tiles = 0;
+  uint32_t* init_tile_tunnel = nullptr;
+  uint32_t* init_tile_first = nullptr;
+  uint32_t n_init_tiles = 0;
+  uint32_t* part_tunnel = nullptr;
+  uint32_t n_partials = 0;
+  double* partials = nullptr;
+  std::vector<void*> allocations;
+  int pair = 0;
+  int parity = 0;
+  uint64_t steps = 0;
+  unsigned grid = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+namespace {
+
+void launch_step(pw_batch* b, int parity, bool macro) {
+  const BatchLaunch L{b->grid, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, b->pair, b->params};
+  switch (b->params.storage) {
+    case PW_STORAGE_F16: batch_step_s<StoreF16>(L, parity, macro); break;
+    case PW_STORAGE_BF16: batch_step_s<StoreBF16>(L, parity, macro); break;
+    default: batch_step_s<StoreF32>(L, parity, macro); break;
+  }
+}
+
+void launch_init(pw_batch* b) {
+  const unsigned grid = b->n_init_tiles < b->grid ? b->n_init_tiles : b->grid;
+  for (int set = 0; set < 2; ++set) {
+    switch (b->params.storage) {
+      case PW_STORAGE_F16: hipLaunchKernelGGL(batch_init_kernel<StoreF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->init_tile_tunnel, b->init_tile_first, b->n_init_tiles, set); break;
+      case PW_STORAGE_BF16: hipLaunchKernelGGL(batch_init_kernel<StoreBF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->init_tile_tunnel, b->init_tile_first, b->n_init_tiles, set); break;
+      default: hipLaunchKernelGGL(batch_init_kernel<StoreF32>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->init_tile_tunnel, b->init_tile_first, b->n_init_tiles, set); break;
+    }
+  }
+}
+
+void launch_force(pw_batch* b, double* dest) {
+  if (b->n_partials) {
+    const bool bz = b->params.walls == PW_WALLS_BOUZIDI;
+    switch (b->params.storage) {
+      case PW_STORAGE_F16: hipLaunchKernelGGL((bz ? batch_force_kernel<StoreF16, true> : batch_force_kernel<StoreF16, false>), dim3(b->n_partials), dim3(kForceBlock), 0, b->stream, b->tunnels, b->part_tunnel, b->partials); break;
+      case PW_STORAGE_BF16: hipLaunchKernelGGL((bz ? batch_force_kernel<StoreBF16, true> : batch_force_kernel<StoreBF16, false>), dim3(b->n_partials), dim3(kForceBlock), 0, b->stream, b->tunnels, b->part_tunnel, b->partials); break;
+      default: hipLaunchKernelGGL((bz ? batch_force_kernel<StoreF32, true> : batch_force_kernel<StoreF32, false>), dim3(b->n_partials), dim3(kForceBlock), 0, b->stream, b->tunnels, b->part_tunnel, b->partials); break;
+    }
+  }
+  hipLaunchKernelGGL(batch_reduce_kernel, dim3(b->count), dim3(kForceBlock), 0, b->stream, b->tunnels, b->partials, dest);
+}
+
+void free_batch(pw_batch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& g : b->graph)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (void* p : b->allocations) (void)hipFree(p);
+  if (b->t0) (void)hipEventDestroy(b->t0);
+  if (b->t1) (void)hipEventDestroy(b->t1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+template <class T>
+hipError_t dev_alloc(pw_batch* b, T** out, size_t n) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, (n ? n : 1) * sizeof(T));
+  if (e == hipSuccess) b->allocations.push_back(p);
+  *out = (T*)p;
+  return e;
+}
+
+// Capture one even/odd step pair (no macro writes) into an executable graph.
+hipError_t capture_pair(pw_batch* b, int first_parity, hipGraphExec_t* exec) {
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  launch_step(b, first_parity, false);
+  launch_step(b, first_parity ^ 1, false);
+  e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess) return e;
+  e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32_t* const* geometries,
+                          const size_t* geometry_words, const uint32_t* const* boundary_cells,
+                          const size_t* n_boundary, const double* tau0, const double* u_in,
+                          const pw_params* shared, pw_batch** out) {
+  if (!out || !layouts || !geometries || !geometry_words || !boundary_cells || !n_boundary || !tau0 || !u_in ||
+      !shared || count == 0) {
+    set_error("Null argument or empty batch.");
+    return PW_EINVAL;
+  }
+  *out = nullptr;
+  if (shared->ramp_steps != 0 || shared->wall_model != PW_WALL_LAW || shared->storage > PW_STORAGE_BF16 ||
+      shared->op > PW_OP_REG || shared->walls > PW_WALLS_BOUZIDI) {
+    set_error("Batched tunnels take operator, walls, LES and storage options without a ramp.");
+    return PW_EINVAL;
+  }
+  // Validate every tunnel exactly as a single solver would, on the host.
+  std::vector<StepParams> sps(count);
+  bool all_even = true;
+  size_t tiles = 0, init_tiles = 0, partials = 0;
+  for (uint32_t t = 0; t < count; ++t) {
+    pw_params prm = *shared;
+    prm.tau0 = tau0[t];
+    prm.u_in = u_in[t];
+    if (pw_status st = validate_tunnel(layouts[t], geometries[t], geometry_words[t], boundary_cells[t],
+                                       n_boundary[t], prm, &sps[t])) {
+      set_error("Tunnel " + std::to_string(t) + ": " + pw_last_error_message());
+      return st;
+    }
+    all_even = all_even && layouts[t].nx % 2 == 0;
+  }
+  pw_batch* b = new (std::nothrow) pw_batch();
+  if (!b) {
+    set_error("Host allocation failed.");
+    return PW_ENOMEM;
+  }
+  b->params = *shared;
+  b->count = count;
+  b->pair = all_even ? 1 : 0;
+  const uint32_t per_tile = b->pair ? 2 * kBlock : kBlock;
+  for (uint32_t t = 0; t < count; ++t) {
+    tiles += (sps[t].cells + per_tile - 1) / per_tile;
+    init_tiles += (sps[t].cells + kBlock - 1) / kBlock;
+    partials += (n_boundary[t] + kForceBlock - 1) / kForceBlock;
+  }
+  if (tiles > 0xffffffffull || partials > 0x7fffffffull) {
+    delete b;
+    set_error("Batch exceeds 32-bit tile indexing.");
+    return PW_EINVAL;
+  }
+  auto fail = [&](hipError_t e, const char* what) {
+    free_batch(b);
+    delete b;
+    return device_error(e, what);
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "batch stream");
+  if ((e = hipEventCreate(&b->t0)) != hipSuccess || (e = hipEventCreate(&b->t1)) != hipSuccess) return fail(e, "batch events");
+  const uint32_t elem = shared->storage == PW_STORAGE_F32 ? 4 : 2;
+  std::vector<uint32_t> tt, tf(count), it, itf(count), pt;
+  b->host.resize(count);
+  b->layouts.assign(layouts, layouts + count);
+  uint32_t part_base = 0;
+  for (uint32_t t = 0; t < count; ++t) {
+    Tunnel& tu = b->host[t];
+    tu.p = sps[t];
+    const size_t N = sps[t].cells;
+    for (int k = 0; k < 2; ++k) {
+      void* p = nullptr;
+      if ((e = hipMalloc(&p, (size_t)kQ * N * elem)) != hipSuccess) return fail(e, "batch populations");
+      b->allocations.push_back(p);
+      tu.pops[k] = p;
+    }
+    uint32_t *geo = nullptr, *bnd = nullptr;
+    float4* macro = nullptr;
+    if ((e = dev_alloc(b, &geo, geometry_words[t])) != hipSuccess) return fail(e, "batch geometry");
+    if ((e = dev_alloc(b, &bnd, n_boundary[t])) != hipSuccess) return fail(e, "batch boundary");
+    if ((e = dev_alloc(b, &macro, N)) != hipSuccess) return fail(e, "batch macros");
+    if ((e = hipMemcpyAsync(geo, geometries[t], geometry_words[t] * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return fail(e, "batch upload");
+    if (n_boundary[t] && (e = hipMemcpyAsync(bnd, boundary_cells[t], n_boundary[t] * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return fail(e, "batch upload");
+    tu.geo = geo;
+    tu.macro = macro;
+    tu.boundary = bnd;
+    tu.n_boundary = (uint32_t)n_boundary[t];
+    tu.partial_base = part_base;
+    tu.n_partials = (uint32_t)((n_boundary[t] + kForceBlock - 1) / kForceBlock);
+    for (uint32_t k = 0; k < tu.n_partials; ++k) pt.push_back(t);
+    part_base += tu.n_partials;
+    tf[t] = (uint32_t)tt.size();
+    for (size_t k = 0; k < (N + per_tile - 1) / per_tile; ++k) tt.push_back(t);
+    itf[t] = (uint32_t)it.size();
+    for (size_t k = 0; k < (N + kBlock - 1) / kBlock; ++k) it.push_back(t);
+  }
+  b->n_tiles = (uint32_t)tt.size();
+  b->n_init_tiles = (uint32_t)it.size();
+  b->n_partials = (uint32_t)pt.size();
+  if ((e = dev_alloc(b, &b->tunnels, count)) != hipSuccess) return fail(e, "batch table");
+  if ((e = dev_alloc(b, &b->tile_tunnel, tt.size())) != hipSuccess) return fail(e, "batch table");
+  if ((e = dev_alloc(b, &b->tile_first, count)) != hipSuccess) return fail(e, "batch table");
+  if ((e = dev_alloc(b, &b->init_tile_tunnel, it.size())) != hipSuccess) return fail(e, "batch table");
+  if ((e = dev_alloc(b, &b->init_tile_first, count)) != hipSuccess) return fail(e, "batch table");
+  if ((e = dev_alloc(b, &b->part_tunnel, pt.size())) != hipSuccess) return fail(e, "batch table");
+  if ((e = dev_alloc(b, &b->partials, 3 * (size_t)(pt.size() ? pt.size() : 1))) != hipSuccess) return fail(e, "batch partials");
+  if ((e = hipMemcpyAsync(b->tunnels, b->host.data(), count * sizeof(Tunnel), hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(b->tile_tunnel, tt.data(), tt.size() * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(b->tile_first, tf.data(), count * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(b->init_tile_tunnel, it.data(), it.size() * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(b->init_tile_first, itf.data(), count * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+      (pt.size() && (e = hipMemcpyAsync(b->part_tunnel, pt.data(), pt.size() * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess))
+    return fail(e, "batch tables");
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
+  const unsigned persistent = (unsigned)cus * 8;
+  b->grid = b->n_tiles < persistent ? b->n_tiles : persistent;
+  launch_init(b);
+  if ((e = hipGetLastError()) != hipSuccess) return fail(e, "batch init");
+  if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail(e, "batch init");
+  *out = b;
+  return PW_OK;
+}
+
+pw_status pw_batch_step(pw_batch* b, uint32_t count, int write_macro_on_last) {
+  if (!b) {
+    set_error("Null batch handle.");
+    return PW_EINVAL;
+  }
+  uint32_t done = 0;
+  hipError_t e;
+  // Whole even/odd pairs replay a captured graph; the remainder and a macro step launch directly.
+  const uint32_t direct_tail = (write_macro_on_last && count) ? 1u : 0u;
+  while (count - done - direct_tail >= 2) {
+    hipGraphExec_t& g = b->graph[b->parity];
+    if (!g && (e = capture_pair(b, b->parity, &g)) != hipSuccess) return device_error(e, "batch graph capture");
+    if ((e = hipGraphLaunch(g, b->stream)) != hipSuccess) return device_error(e, "batch graph launch");
+    done += 2;
+    b->steps += 2;
+  }
+  while (done < count) {
+    const bool macro = write_macro_on_last && done + 1 == count;
+    launch_step(b, b->parity, macro);
+    b->parity ^= 1;
+    done += 1;
+    b->steps += 1;
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return device_error(e, "batch step");
+  return PW_OK;
+}
+
+pw_status pw_batch_step_sampled(pw_batch* b, uint32_t count, uint32_t every, double* out, size_t out_doubles) {
+  if (!b || !out || every == 0) {
+    set_error("Sampling needs a batch, a positive interval and an output buffer.");
+    return PW_EINVAL;
+  }
+  size_t samples = 0;
+  for (uint64_t k = b->steps + 1; k <= b->steps + count; ++k) samples += (k % every) == 0;
+  if (out_doubles < 4 * samples * b->count) {
+    set_error("Sample output needs 4 doubles per sample and tunnel.");
+    return PW_EINVAL;
+  }
+  double* hist = nullptr;
+  hipError_t e;
+  if (samples && (e = hipMallocAsync((void**)&hist, samples * 3 * b->count * sizeof(double), b->stream)) != hipSuccess)
+    return device_error(e, "batch sample history");
+  std::vector<uint64_t> stamps;
+  uint32_t done = 0;
+  while (done < count) {
+    // Steps up to the next sample, through the graph path.
+    const uint64_t next = ((b->steps / every) + 1) * every;
+    const uint32_t chunk = (uint32_t)std::min<uint64_t>(next - b->steps, count - done);
+    if (pw_status st = pw_batch_step(b, chunk, 0)) {
+      if (hist) (void)hipFreeAsync(hist, b->stream);
+      return st;
+    }
+    done += chunk;
+    if (b->steps % every == 0) {
+      launch_force(b, hist + 3 * (size_t)b->count * stamps.size());
+      stamps.push_back(b->steps);
+    }
+  }
+  std::vector<double> host(3 * samples * b->count);
+  e = hipGetLastError();
+  if (e == hipSuccess && samples) e = hipMemcpyAsync(host.data(), hist, host.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+  if (hist) (void)hipFreeAsync(hist, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return device_error(e, "batch samples");
+  // Output layout: [tunnel][sample] x (step, fx, fy, fz).
+  for (uint32_t t = 0; t < b->count; ++t)
+    for (size_t j = 0; j < samples; ++j) {
+      double* o = out + 4 * (t * samples + j);
+      o[0] = (double)stamps[j];
+      o[1] = host[3 * (j * b->count + t)];
+      o[2] = host[3 * (j * b->count + t) + 1];
+      o[3] = host[3 * (j * b->count + t) + 2];
+    }
+  return PW_OK;
+}
+
+pw_status pw_batch_read_macro(pw_batch* b, uint32_t tunnel, float* out, size_t out_floats) {
+  if (!b || tunnel >= b->count || !out || out_floats < 4 * (size_t)b->host[tunnel].p.cells) {
+    set_error("Invalid tunnel index or output size.");
+    return PW_EINVAL;
+  }
+  hipError_t e = hipMemcpyAsync(out, b->host[tunnel].macro, 16 * (size_t)b->host[tunnel].p.cells, hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  return e == hipSuccess ? PW_OK : device_error(e, "batch macro readback");
+}
+
+pw_status pw_batch_read_populations(pw_batch* b, uint32_t tunnel, void* out, size_t out_bytes) {
+  const uint32_t elem = b && b->params.storage == PW_STORAGE_F32 ? 4 : 2;
+  if (!b || tunnel >= b->count || !out || out_bytes < (size_t)kQ * b->host[tunnel].p.cells * elem) {
+    set_error("Invalid tunnel index or output size.");
+    return PW_EINVAL;
+  }
+  hipError_t e = hipMemcpyAsync(out, b->host[tunnel].pops[b->parity], (size_t)kQ * b->host[tunnel].p.cells * elem,
+                                hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  return e == hipSuccess ? PW_OK : device_error(e, "batch population readback");
+}
+
+uint64_t pw_batch_step_count(const pw_batch* b) { return b ? b->steps : 0; }
+uint64_t pw_batch_cells(const pw_batch* b) {
+  uint64_t n = 0;
+  if (b)
+    for (const auto& t : b->host) n += t.p.cells;
+  return n;
+}
+
+pw_status pw_batch_time_steps(pw_batch* b, uint32_t count, float* elapsed_ms) {
+  if (!b || !elapsed_ms) {
+    set_error("Null argument.");
+    return PW_EINVAL;
+  }
+  hipError_t e = hipEventRecord(b->t0, b->stream);
+  if (e != hipSuccess) return device_error(e, "batch timing");
+  if (pw_status st = pw_batch_step(b, count, 0)) return st;
+  if ((e = hipEventRecord(b->t1, b->stream)) != hipSuccess || (e = hipEventSynchronize(b->t1)) != hipSuccess ||
+      (e = hipEventElapsedTime(elapsed_ms, b->t0, b->t1)) != hipSuccess)
+    return device_error(e, "batch timing");
+  return PW_OK;
+}
+
+void pw_batch_destroy(pw_batch* b) {
+  if (!b) return;
+  free_batch(b);
+  delete b;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// batch_step.h -- the persistent step kernel of batched tunnels and its launch, instantiated
+// once per storage type (batch_f32.hip, batch_f16.hip, batch_bf16.hip).
+#pragma once
+
+#include "force_kernels.h"
+
+namespace pw {
+namespace batch {
+
+constexpr int kBlock = 256;
+
+struct Tunnel {
+  StepParams p;
+  void* pops[2];
+  const uint32_t* geo;
+  float4* macro;
+  const uint32_t* boundary;
+  uint32_t n_boundary;
+  uint32_t partial_base;  // first force partial of this tunnel
+  uint32_t n_partials;
+};
+
+struct BatchLaunch {
+  unsigned grid;
+  hipStream_t stream;
+  const Tunnel* tunnels;
+  const uint32_t* tile_tunnel;
+  const uint32_t* tile_first;
+  uint32_t n_tiles;
+  int pair;
+  pw_params params;
+};
+
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool PAIR>
+__global__ __launch_bounds__(kBlock) void batch_step_kernel(const Tunnel* __restrict__ tunnels,
+                                                            const uint32_t* __restrict__ tile_tunnel,
+                                                            const uint32_t* __restrict__ tile_first,
+                                                            uint32_t n_tiles, int parity) {
+  constexpr uint32_t kCells = PAIR ? 2 * kBlock : kBlock;
+  for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const uint32_t t = tile_tunnel[tile];
+    const Tunnel& tu = tunnels[t];
+    const uint32_t n = (tile - tile_first[t]) * kCells + (PAIR ? 2 : 1) * threadIdx.x;
+    if (n >= tu.p.cells) continue;
+    const auto* in = (const typename S::T*)tu.pops[parity];
+    auto* out = (typename S::T*)tu.pops[parity ^ 1];
+    if (PAIR)
+      stream_collide_pair<S, OP, LES, BOUZIDI, MACRO, true>(in, out, tu.geo, tu.macro, tu.p, n);
+    else
+      stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, true>(in, out, tu.geo, tu.macro, tu.p, n);
+  }
+}
+
+template <class S, int OP, bool LES, bool BZ, bool MACRO>
+void batch_step_t(const BatchLaunch& L, int parity) {
+  if (L.pair)
+    hipLaunchKernelGGL((batch_step_kernel<S, OP, LES, BZ, MACRO, true>), dim3(L.grid), dim3(kBlock), 0, L.stream,
+                       L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles, parity);
+  else
+    hipLaunchKernelGGL((batch_step_kernel<S, OP, LES, BZ, MACRO, false>), dim3(L.grid), dim3(kBlock), 0, L.stream,
+                       L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles, parity);
+}
+
+template <class S, bool MACRO>
+void batch_step_m(const BatchLaunch& L, int parity) {
+  const bool reg = L.params.op == PW_OP_REG, les = L.params.les != 0, bz = L.params.walls == PW_WALLS_BOUZIDI;
+  if (reg) {
+    if (les) bz ? batch_step_t<S, kOpReg, true, true, MACRO>(L, parity) : batch_step_t<S, kOpReg, true, false, MACRO>(L, parity);
+    else bz ? batch_step_t<S, kOpReg, false, true, MACRO>(L, parity) : batch_step_t<S, kOpReg, false, false, MACRO>(L, parity);
+  } else {
+    if (les) bz ? batch_step_t<S, kOpTrt, true, true, MACRO>(L, parity) : batch_step_t<S, kOpTrt, true, false, MACRO>(L, parity);
+    else bz ? batch_step_t<S, kOpTrt, false, true, MACRO>(L, parity) : batch_step_t<S, kOpTrt, false, false, MACRO>(L, parity);
+  }
+}
+
+template <class S>
+void batch_step_s(const BatchLaunch& L, int parity, bool macro) {
+  macro ? batch_step_m<S, true>(L, parity) : batch_step_m<S, false>(L, parity);
+}
+
+}  // namespace batch
+}  // namespace pw

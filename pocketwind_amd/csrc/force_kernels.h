@@ -79,9 +79,8 @@ __global__ __launch_bounds__(kForceBlock) void force_partials_kernel(
 }
 
 // One block strides the partials in a fixed order, then reduces across its lanes.
-__global__ __launch_bounds__(kForceBlock) void force_reduce_kernel(const double* __restrict__ partials,
-                                                                   uint32_t n_partials,
-                                                                   double* __restrict__ result) {
+__device__ __forceinline__ void force_reduce_block(const double* __restrict__ partials, uint32_t n_partials,
+                                                   double* __restrict__ result) {
   double s[3] = {0, 0, 0};
   for (uint32_t k = threadIdx.x; k < n_partials; k += kForceBlock)
     for (int c = 0; c < 3; ++c) s[c] += partials[3 * (size_t)k + c];

@@ -90,26 +90,60 @@ struct StoreBF16 {
   }
 };
 
-// Two consecutive storage elements moved by one load or store: one dword for 16-bit
-// storage, two for f32. The address of an x-shifted pull is only element-aligned; gfx950
-// serves such loads as single unaligned dword accesses.
-template <class T>
-struct TPair {
-  T lo, hi;
-};
-
-template <class S>
-__host__ __device__ __forceinline__ void load_pair(const typename S::T* p, float& a, float& b) {
-  TPair<typename S::T> v;
-  __builtin_memcpy(&v, p, sizeof v);
-  a = S::decode(v.lo);
-  b = S::decode(v.hi);
+// Population streams are read once and written once per step. NT selects non-temporal
+// accesses for them (no reuse to keep in the caches); the flag and wall-path loads stay cached.
+template <bool NT, class T>
+__host__ __device__ __forceinline__ T ld(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
 }
 
-template <class S>
+template <bool NT, class T>
+__host__ __device__ __forceinline__ void st(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <class S, bool NT>
+__host__ __device__ __forceinline__ float load_pop(const typename S::T* p) {
+  return S::decode(ld<NT>(p));
+}
+
+// Two consecutive storage elements moved by one access: one dword for 16-bit storage, two
+// for f32. The address of an x-shifted pull is only element-aligned; gfx950 serves such
+// loads as single unaligned dword accesses.
+typedef uint32_t u32_align2 __attribute__((aligned(2)));
+typedef uint64_t u64_align4 __attribute__((aligned(4)));
+
+template <class T>
+__host__ __device__ __forceinline__ T lo_of(uint32_t v) { return __builtin_bit_cast(T, (uint16_t)(v & 0xffffu)); }
+template <class T>
+__host__ __device__ __forceinline__ T hi_of(uint32_t v) { return __builtin_bit_cast(T, (uint16_t)(v >> 16)); }
+
+template <class S, bool NT>
+__host__ __device__ __forceinline__ void load_pair(const typename S::T* p, float& a, float& b) {
+  using T = typename S::T;
+  if constexpr (sizeof(T) == 4) {
+    const uint64_t v = ld<NT>(reinterpret_cast<const u64_align4*>(p));
+    a = S::decode(__builtin_bit_cast(T, (uint32_t)(v & 0xffffffffu)));
+    b = S::decode(__builtin_bit_cast(T, (uint32_t)(v >> 32)));
+  } else {
+    const uint32_t v = ld<NT>(reinterpret_cast<const u32_align2*>(p));
+    a = S::decode(lo_of<T>(v));
+    b = S::decode(hi_of<T>(v));
+  }
+}
+
+template <class S, bool NT>
 __host__ __device__ __forceinline__ void store_pair(typename S::T* p, typename S::T a, typename S::T b) {
-  const TPair<typename S::T> v{a, b};
-  __builtin_memcpy(p, &v, sizeof v);
+  using T = typename S::T;
+  if constexpr (sizeof(T) == 4) {
+    const uint64_t v = (uint64_t)__builtin_bit_cast(uint32_t, a) | ((uint64_t)__builtin_bit_cast(uint32_t, b) << 32);
+    st<NT>(reinterpret_cast<uint64_t*>(p), v);
+  } else {
+    const uint32_t v = (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+    st<NT>(reinterpret_cast<uint32_t*>(p), v);
+  }
 }
 
 }  // namespace pw

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "force_kernels.h"
+#include "step_launch.h"
 
 namespace pw {
 
@@ -24,6 +25,12 @@ void set_error(const std::string& message) { g_error = message; }
 pw_status device_error(hipError_t e, const char* what) {
   set_error(std::string(what) + ": " + hipGetErrorString(e));
   return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PW_ENOMEM : PW_EDEVICE;
+}
+
+__global__ __launch_bounds__(kForceBlock) void force_reduce_kernel(const double* __restrict__ partials,
+                                                                   uint32_t n_partials,
+                                                                   double* __restrict__ result) {
+  force_reduce_block(partials, n_partials, result);
 }
 
 namespace {
@@ -118,32 +125,13 @@ void launch_init(pw_solver* s, void* out, const StepParams& p) {
                      (typename S::T*)out, s->geometry, s->macro, p);
 }
 
-template <class S, int OP, bool LES, bool BB>
-void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const StepParams& p) {
-  constexpr bool BOUZIDI = !BB;
-  if (s->kernel == PW_KERNEL_PAIR) {
-    auto k = macro ? stream_collide_pair_kernel<S, OP, LES, BOUZIDI, true>
-                   : stream_collide_pair_kernel<S, OP, LES, BOUZIDI, false>;
-    hipLaunchKernelGGL(k, dim3(blocks_for(s->cells / 2)), dim3(kBlock), 0, s->stream,
-                       (const typename S::T*)in, (typename S::T*)out, s->geometry, s->macro, p);
-    return;
-  }
-  auto k = macro ? stream_collide_kernel<S, OP, LES, BOUZIDI, true> : stream_collide_kernel<S, OP, LES, BOUZIDI, false>;
-  hipLaunchKernelGGL(k, dim3(blocks_for(s->cells)), dim3(kBlock), 0, s->stream,
-                     (const typename S::T*)in, (typename S::T*)out, s->geometry, s->macro, p);
-}
+}  // namespace
 
-template <class S>
-void launch_step_s(pw_solver* s, const void* in, void* out, bool macro, const StepParams& p) {
-  const bool reg = s->params.op == PW_OP_REG, les = s->params.les != 0, bb = s->params.walls == PW_WALLS_BB;
-  if (reg) {
-    if (les) bb ? launch_step_t<S, kOpReg, true, true>(s, in, out, macro, p) : launch_step_t<S, kOpReg, true, false>(s, in, out, macro, p);
-    else bb ? launch_step_t<S, kOpReg, false, true>(s, in, out, macro, p) : launch_step_t<S, kOpReg, false, false>(s, in, out, macro, p);
-  } else {
-    if (les) bb ? launch_step_t<S, kOpTrt, true, true>(s, in, out, macro, p) : launch_step_t<S, kOpTrt, true, false>(s, in, out, macro, p);
-    else bb ? launch_step_t<S, kOpTrt, false, true>(s, in, out, macro, p) : launch_step_t<S, kOpTrt, false, false>(s, in, out, macro, p);
-  }
-}
+extern template void launch_step_s<StoreF32>(pw_solver*, const void*, void*, bool, const StepParams&);
+extern template void launch_step_s<StoreF16>(pw_solver*, const void*, void*, bool, const StepParams&);
+extern template void launch_step_s<StoreBF16>(pw_solver*, const void*, void*, bool, const StepParams&);
+
+namespace {
 
 void launch_step(pw_solver* s, bool macro, const StepParams& p) {
   const void* in = s->pops[s->current];
@@ -189,11 +177,9 @@ void launch_force(pw_solver* s, double* dest) {
 // Time each applicable step variant on the handle's own buffers and keep the fastest. The
 // populations are re-initialised afterwards, so tuning leaves no trace in the state.
 hipError_t autotune(pw_solver* s) {
-  if (s->layout.nx % 2 != 0) {
-    s->kernel = PW_KERNEL_CELL;
-    return hipSuccess;
-  }
-  const uint32_t candidates[2] = {PW_KERNEL_CELL, PW_KERNEL_PAIR};
+  const uint32_t all[4] = {PW_KERNEL_CELL, PW_KERNEL_CELL_NT, PW_KERNEL_PAIR, PW_KERNEL_PAIR_NT};
+  const int n_candidates = s->layout.nx % 2 == 0 ? 4 : 2;
+  std::vector<uint32_t> candidates(all, all + n_candidates);
   float best = 0;
   uint32_t choice = PW_KERNEL_PAIR;
   const int reps = s->cells >= (1u << 20) ? 4 : 2;
@@ -255,6 +241,41 @@ pw_status check_handle(pw_solver* s) {
   } while (0)
 
 }  // namespace
+
+pw_status validate_tunnel(const pw_layout& L, const uint32_t* geometry, size_t geometry_words,
+                          const uint32_t* boundary_cells, size_t n_boundary, const pw_params& params,
+                          StepParams* full) {
+  if ((!geometry && geometry_words) || (!boundary_cells && n_boundary)) {
+    set_error("Null argument.");
+    return PW_EINVAL;
+  }
+  if (L.nx < 1 || L.ny < 1 || L.nz < 1) {
+    set_error("Grid dimensions must be positive safe integers.");
+    return PW_EINVAL;
+  }
+  if (params.storage > PW_STORAGE_BF16 || params.op > PW_OP_REG || params.walls > PW_WALLS_BOUZIDI ||
+      params.wall_model > PW_WALL_LEARNED) {
+    set_error("Unknown operator, wall, storage or wall-model option.");
+    return PW_EINVAL;
+  }
+  if (params.wall_model == PW_WALL_LEARNED) {
+    set_error("The learned wall closure is not available in this build.");
+    return PW_EINVAL;
+  }
+  if (!pack_params(L, params, params.u_in, full)) return PW_EINVAL;
+  if (!validate_geometry(L, geometry, geometry_words)) return PW_EINVAL;
+  const size_t N = (size_t)L.nx * L.ny * L.nz;
+  // gpu-force.ts:76-81
+  for (size_t k = 0; k < n_boundary; ++k) {
+    const uint32_t cell = boundary_cells[k];
+    if (cell >= N || (geometry[cell] & kTypeMask) != kFluid || (geometry[cell] >> 4) == 0) {
+      set_error("Boundary cells must index fluid cells with wall links.");
+      return PW_EINVAL;
+    }
+  }
+  return PW_OK;
+}
+
 }  // namespace pw
 
 using namespace pw;
@@ -273,43 +294,21 @@ int pw_device_count(void) {
 pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, size_t geometry_words,
                            const uint32_t* boundary_cells, size_t n_boundary, const pw_params* params,
                            pw_solver** out) {
-  if (!layout || !params || !out || (!geometry && geometry_words)) {
+  if (!layout || !params || !out) {
     set_error("Null argument.");
     return PW_EINVAL;
   }
   *out = nullptr;
   const pw_layout L = *layout;
-  if (L.nx < 1 || L.ny < 1 || L.nz < 1) {
-    set_error("Grid dimensions must be positive safe integers.");
-    return PW_EINVAL;
-  }
-  if (params->storage > PW_STORAGE_BF16 || params->op > PW_OP_REG || params->walls > PW_WALLS_BOUZIDI ||
-      params->wall_model > PW_WALL_LEARNED) {
-    set_error("Unknown operator, wall, storage or wall-model option.");
-    return PW_EINVAL;
-  }
-  if (params->wall_model == PW_WALL_LEARNED) {
-    set_error("The learned wall closure is not available in this build.");
-    return PW_EINVAL;
-  }
-  // gpu-solver.ts:172-177: a ramp starts from rest.
-  StepParams sp{};
-  if (!pack_params(L, *params, params->ramp_steps > 0 ? 0.0 : params->u_in, &sp)) return PW_EINVAL;
   StepParams full{};
-  if (!pack_params(L, *params, params->u_in, &full)) return PW_EINVAL;
-  if (!validate_geometry(L, geometry, geometry_words)) return PW_EINVAL;
+  if (pw_status st = validate_tunnel(L, geometry, geometry_words, boundary_cells, n_boundary, *params, &full)) return st;
+  // gpu-solver.ts:172-177: a ramp starts from rest.
+  StepParams sp = full;
+  if (params->ramp_steps > 0) sp.u_in = 0.0f;
   const size_t N = (size_t)L.nx * L.ny * L.nz;
   long long first_fluid = -1;
   for (size_t c = 0; c < N && first_fluid < 0; ++c)
     if ((geometry[c] & kTypeMask) == kFluid) first_fluid = (long long)c;
-  // gpu-force.ts:76-81
-  for (size_t k = 0; k < n_boundary; ++k) {
-    const uint32_t cell = boundary_cells[k];
-    if (cell >= N || (geometry[cell] & kTypeMask) != kFluid || (geometry[cell] >> 4) == 0) {
-      set_error("Boundary cells must index fluid cells with wall links.");
-      return PW_EINVAL;
-    }
-  }
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
     set_error("No HIP device is available.");
@@ -353,7 +352,9 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   launch_init_both(s, full);
   const char* forced = std::getenv("PW_STEP_KERNEL");
   if (forced && std::strcmp(forced, "cell") == 0) s->kernel = PW_KERNEL_CELL;
+  else if (forced && std::strcmp(forced, "cell-nt") == 0) s->kernel = PW_KERNEL_CELL_NT;
   else if (forced && std::strcmp(forced, "pair") == 0 && L.nx % 2 == 0) s->kernel = PW_KERNEL_PAIR;
+  else if (forced && std::strcmp(forced, "pair-nt") == 0 && L.nx % 2 == 0) s->kernel = PW_KERNEL_PAIR_NT;
   else if ((e = autotune(s)) != hipSuccess) return fail(device_error(e, "step autotune"));
   launch_init_both(s, sp);
   if ((e = hipGetLastError()) != hipSuccess) return fail(device_error(e, "init launch"));
@@ -507,11 +508,11 @@ pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
     if (e != hipSuccess) return device_error(e, "tune");
     return PW_OK;
   }
-  if (kernel == PW_KERNEL_PAIR && s->layout.nx % 2 != 0) {
+  if ((kernel == PW_KERNEL_PAIR || kernel == PW_KERNEL_PAIR_NT) && s->layout.nx % 2 != 0) {
     set_error("The paired step kernel needs an even nx.");
     return PW_EINVAL;
   }
-  if (kernel != PW_KERNEL_CELL && kernel != PW_KERNEL_PAIR) {
+  if (kernel > PW_KERNEL_PAIR_NT) {
     set_error("Unknown step kernel.");
     return PW_EINVAL;
   }

@@ -25,6 +25,12 @@ struct StepParams {
 void set_error(const std::string& message);
 pw_status device_error(hipError_t e, const char* what);
 
+// Host-side checks of one tunnel exactly as pw_solver_create applies them (geometry contract,
+// boundary list, f32 parameters). Fills the full-speed step parameters.
+pw_status validate_tunnel(const pw_layout& layout, const uint32_t* geometry, size_t geometry_words,
+                          const uint32_t* boundary_cells, size_t n_boundary, const pw_params& params,
+                          StepParams* full);
+
 }  // namespace pw
 
 struct pw_solver {

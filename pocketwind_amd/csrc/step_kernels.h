@@ -205,10 +205,10 @@ __host__ __device__ __forceinline__ R wall_value(const typename S::T* __restrict
   return (reflected + (R(2) * q - R(1)) * R(S::load(in + (size_t)i * N + n))) / (R(2) * q);
 }
 
-template <class S>
+template <class S, bool NT = false>
 __host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__ out, size_t N, uint32_t n, const float* g) {
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) out[(size_t)i * N + n] = S::encode(g[i], (uint32_t)i, n);
+  for (int i = 0; i < kQ; ++i) st<NT>(out + (size_t)i * N + n, S::encode(g[i], (uint32_t)i, n));
 }
 
 // common.wgsl:145-150 with rho - 1 = 0.
@@ -261,7 +261,7 @@ __host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, 
 }
 
 // stream_collide.wgsl:95-136, one cell per lane.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT = false>
 __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
@@ -298,10 +298,10 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
   }
   const Neighbours nb(x, y, z, p);
   float g[kQ];
-  g[0] = S::load(in + n);
+  g[0] = load_pop<S, NT>(in + n);
   if ((flag >> 3) == 0) {
 #pragma unroll
-    for (int i = 1; i < kQ; ++i) g[i] = S::load(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)));
+    for (int i = 1; i < kQ; ++i) g[i] = load_pop<S, NT>(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)));
   } else {
 #pragma unroll
     for (int i = 1; i < kQ; ++i)
@@ -309,24 +309,24 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
                                : S::load(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)));
   }
   const float4 m = relax<OP, LES>(p, g, flag);
-  store_state<S>(out, N, n, g);
+  store_state<S, NT>(out, N, n, g);
   if (MACRO) macro[n] = m;
 }
 
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT>
 __global__ __launch_bounds__(256) void stream_collide_kernel(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
                                                              float4* __restrict__ macro, StepParams p) {
   const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= p.cells) return;
-  stream_collide_cell<S, OP, LES, BOUZIDI, MACRO>(in, out, geo, macro, p, n);
+  stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n);
 }
 
 // Two x-adjacent cells per lane. A pair whose cells are both plain fluid (no wall links) and
 // whose pulls do not wrap in x gathers and stores each direction with one wide access;
 // every other pair takes the per-cell path. Needs an even nx.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT = false>
 __host__ __device__ __forceinline__ void stream_collide_pair(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
@@ -335,8 +335,8 @@ __host__ __device__ __forceinline__ void stream_collide_pair(const typename S::T
   const uint2 f = *reinterpret_cast<const uint2*>(geo + n);
   const uint32_t x = n % p.nx;
   if ((f.x | f.y) != 0 || x == 0 || x + 2 == p.nx) {
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO>(in, out, geo, macro, p, n);
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO>(in, out, geo, macro, p, n + 1);
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n);
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n + 1);
     return;
   }
   const size_t N = p.cells;
@@ -344,26 +344,26 @@ __host__ __device__ __forceinline__ void stream_collide_pair(const typename S::T
   const Neighbours nb(x, r % p.ny, r / p.ny, p);
   float a[kQ], b[kQ];
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) load_pair<S>(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)), a[i], b[i]);
+  for (int i = 0; i < kQ; ++i) load_pair<S, NT>(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)), a[i], b[i]);
   const float4 ma = relax<OP, LES>(p, a, 0u);
   const float4 mb = relax<OP, LES>(p, b, 0u);
 #pragma unroll
   for (int i = 0; i < kQ; ++i)
-    store_pair<S>(out + (size_t)i * N + n, S::encode(a[i], (uint32_t)i, n), S::encode(b[i], (uint32_t)i, n + 1));
+    store_pair<S, NT>(out + (size_t)i * N + n, S::encode(a[i], (uint32_t)i, n), S::encode(b[i], (uint32_t)i, n + 1));
   if (MACRO) {
     macro[n] = ma;
     macro[n + 1] = mb;
   }
 }
 
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT>
 __global__ __launch_bounds__(256) void stream_collide_pair_kernel(const typename S::T* __restrict__ in,
                                                                   typename S::T* __restrict__ out,
                                                                   const uint32_t* __restrict__ geo,
                                                                   float4* __restrict__ macro, StepParams p) {
   const uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
   if (n >= p.cells) return;
-  stream_collide_pair<S, OP, LES, BOUZIDI, MACRO>(in, out, geo, macro, p, n);
+  stream_collide_pair<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n);
 }
 
 }  // namespace pw

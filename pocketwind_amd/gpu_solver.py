@@ -185,7 +185,7 @@ class GpuSolver:
         N.check(N.lib().pw_solver_device_state(self._live(), C.byref(out)))
         return out
 
-    KERNELS = {"auto": 0, "cell": 1, "pair": 2}
+    KERNELS = {"auto": 0, "cell": 1, "pair": 2, "cell-nt": 3, "pair-nt": 4}
 
     def set_kernel(self, kernel: str) -> None:
         """Force a step-kernel variant ('cell', 'pair') or re-tune ('auto')."""
@@ -194,7 +194,7 @@ class GpuSolver:
     @property
     def kernel(self) -> str:
         code = int(N.lib().pw_solver_kernel(self._live()))
-        return {1: "cell", 2: "pair"}.get(code, "auto")
+        return {1: "cell", 2: "pair", 3: "cell-nt", 4: "pair-nt"}.get(code, "auto")
 
     def synchronize(self) -> None:
         N.check(N.lib().pw_solver_synchronize(self._live()))
