@@ -6,7 +6,7 @@
 
 #include <string>
 
-#include "../../include/pw.h"
+#include "pw.h"
 
 namespace pw {
 

@@ -107,7 +107,8 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
     }
   float sx = ux, sy = uy, sz = uz;
   const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
-  if (nl > 1e-3f) {
+  const bool projected_slip = nl > 1e-3f;
+  if (projected_slip) {
     const float ax = nx / nl, ay = ny / nl, az = nz / nl;
     const float d = ux * ax + uy * ay + uz * az;
     sx -= d * ax;
@@ -130,7 +131,10 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
     const int j = opp(i);
     const float x = cx(j), y = cy(j), z = cz(j);
     const float along = x * tx + y * ty + z * tz;
-    flux += 2.0f * along * (eq[j] + wf(j));
+    // The weight part of the flux, sum 2 along_j w_j = -2 n.t, is exactly zero once the slip
+    // has had its normal component removed; evaluated in f32 it would only add the rounding
+    // residual of t's normal component, amplified by 1 / |slip| when the slip is small.
+    flux += 2.0f * along * (projected_slip ? eq[j] : eq[j] + wf(j));
     const float s = 9.0f * wf(j) * along;
     b.xx += s * (x * x - 1.0f / 3.0f);
     b.yy += s * (y * y - 1.0f / 3.0f);

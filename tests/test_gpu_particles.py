@@ -56,7 +56,8 @@ def test_advection_bit_identical_to_oracle(steps_per_frame):
                                        fallback[1:])
         gpu = p.read().reshape(-1)
         assert np.array_equal(gpu.view(np.uint32), state.view(np.uint32))
-        assert np.any(gpu.reshape(-1, 8)[:, 7] > 0), "no particle respawned"
+        if steps_per_frame * 60 * 0.05 > 64:  # far enough to cross the tunnel: respawns happened
+            assert np.any(gpu.reshape(-1, 8)[:, 7] > 0), "no particle respawned"
         p.destroy()
     finally:
         solver.destroy()
@@ -105,7 +106,7 @@ def test_rejects_invalid_counts_and_regions():
             with pytest.raises(ValueError):
                 Particles.create(solver, bounds, count, 1)
         with pytest.raises(ValueError):
-            Particles.create(solver, ((0, 30, 30), (1, 40, 40)), 1, 1)
+            Particles.create(solver, ((0, 40, 40), (1, 50, 50)), 1, 1)
         big = Particles.create(solver, bounds, 1 << 20, 1, max_count=1 << 24)
         assert big.read().shape == (1 << 20, 8)
         big.destroy()
