@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-#define PW_ABI_VERSION 1
+#define PW_ABI_VERSION 2
 
 /* Errors (src/lbm/gpu-resources.ts:3-27, gpu-force.ts:192-205). EINVAL is the reference's
  * RangeError, ENOMEM its AllocationFailure, EBUSY its 'busy' readback. */
@@ -132,20 +132,20 @@ typedef struct {
 } pw_device_state;
 pw_status pw_solver_device_state(pw_solver *s, pw_device_state *out);
 
-/* Step-kernel variants: one cell per lane, or two x-adjacent cells per lane moving each
- * direction with one wide access; each with cached or non-temporal population streams.
- * AUTO times them on the handle's buffers (at create, or when set later, restoring the
- * state afterwards) and keeps the fastest. No reference counterpart: the reference has one
- * WGSL kernel (src/lbm/shaders/stream_collide.wgsl). */
+/* Step-kernel variants: one cell per lane with cached or non-temporal population streams.
+ * AUTO times both at block sizes 64, 128 and 256 on the handle's buffers (at create, or when
+ * set later, restoring the state afterwards) and keeps the fastest; an explicit variant runs
+ * at block size 256. No reference counterpart: the reference has one WGSL kernel
+ * (src/lbm/shaders/stream_collide.wgsl) with a fixed workgroup size. */
 typedef enum {
   PW_KERNEL_AUTO = 0,
   PW_KERNEL_CELL = 1,
-  PW_KERNEL_PAIR = 2,
-  PW_KERNEL_CELL_NT = 3,
-  PW_KERNEL_PAIR_NT = 4
+  PW_KERNEL_CELL_NT = 2
 } pw_kernel;
 pw_status pw_solver_set_kernel(pw_solver *s, uint32_t kernel);
 uint32_t pw_solver_kernel(const pw_solver *s);
+/* Threads per block of the step launches. */
+uint32_t pw_solver_block_size(const pw_solver *s);
 
 /* The validation stepper stepWithForces (src/app/case-run.ts:154-169) without a host round
  * trip per sample: queue `count` steps and, after each step whose count is a multiple of

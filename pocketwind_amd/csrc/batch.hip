@@ -2,7 +2,7 @@
 //
 // The reference runs one solver at a time (src/app/solver-slot.ts:14-44); batching has no
 // reference counterpart. Each tunnel keeps the single-solver semantics exactly (the same
-// per-cell and per-pair step code, its own tau, inlet speed, geometry, ping-pong sets and
+// per-cell step code, its own tau, inlet speed, geometry, ping-pong sets and
 // macros), so a tunnel stepped in a batch is bit-identical to the same tunnel stepped alone.
 //
 // Work decomposition: the cells of all tunnels are cut into tiles of one block's worth of
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(const Tunnel* __rest
     if (n >= tu.p.cells) continue;
     const uint32_t kind = tu.geo[n] & kTypeMask;
     const float ux = is_wall(kind) ? 0.0f : tu.p.u_in;
-    store_equilibrium<S>((typename S::T*)tu.pops[set], tu.p.cells, n, ux, 0.0f, 0.0f);
+    store_equilibrium<S>((typename S::T*)tu.pops[set], tu.p.cells, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
     tu.macro[n] = make_float4(1.0f, ux, 0.0f, 0.0f);
   }
 }
@@ -54,8 +54,9 @@ __global__ __launch_bounds__(kForceBlock) void batch_force_kernel(const Tunnel* 
     const uint32_t n = tu.boundary[item];
     const uint32_t flag = tu.geo[n];
     const StepParams& p = tu.p;
-    const uint32_t x = n % p.nx, r = n / p.nx, y = r % p.ny, z = r / p.ny;
-    const Neighbours nb(x, y, z, p);
+    uint32_t x, y, z;
+    cell_coords(n, p, x, y, z);
+    const Neighbours<> nb(x, y, z, p);
     double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
     cell_exchange<S, BOUZIDI>((const typename S::T*)tu.pops[0], tu.geo, p, nb, n, flag, ax, ay, az);
     cell_exchange<S, BOUZIDI>((const typename S::T*)tu.pops[1], tu.geo, p, nb, n, flag, bx, by, bz);
@@ -113,14 +114,11 @@ struct pw_batch {
   uint32_t* tile_tunnel = nullptr;
   uint32_t* tile_first = nullptr;
   uint32_t n_tiles = 0;
-  uint32_t* init_tile_tunnel = nullptr;
-  uint32_t* init_tile_first = nullptr;
-  uint32_t n_init_tiles = 0;
   uint32_t* part_tunnel = nullptr;
   uint32_t n_partials = 0;
   double* partials = nullptr;
   std::vector<void*> allocations;
-  int pair = 0;
+  uint32_t* seeds = nullptr;  // two step-count words, see batch_step_kernel
   int parity = 0;
   uint64_t steps = 0;
   unsigned grid = 0;
@@ -131,7 +129,7 @@ struct pw_batch {
 namespace {
 
 void launch_step(pw_batch* b, int parity, bool macro) {
-  const BatchLaunch L{b->grid, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, b->pair, b->params};
+  const BatchLaunch L{b->grid, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, b->seeds, b->params};
   switch (b->params.storage) {
     case PW_STORAGE_F16: batch_step_s<StoreF16>(L, parity, macro); break;
     case PW_STORAGE_BF16: batch_step_s<StoreBF16>(L, parity, macro); break;
@@ -140,12 +138,12 @@ void launch_step(pw_batch* b, int parity, bool macro) {
 }
 
 void launch_init(pw_batch* b) {
-  const unsigned grid = b->n_init_tiles < b->grid ? b->n_init_tiles : b->grid;
+  const unsigned grid = b->grid;
   for (int set = 0; set < 2; ++set) {
     switch (b->params.storage) {
-      case PW_STORAGE_F16: hipLaunchKernelGGL(batch_init_kernel<StoreF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->init_tile_tunnel, b->init_tile_first, b->n_init_tiles, set); break;
-      case PW_STORAGE_BF16: hipLaunchKernelGGL(batch_init_kernel<StoreBF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->init_tile_tunnel, b->init_tile_first, b->n_init_tiles, set); break;
-      default: hipLaunchKernelGGL(batch_init_kernel<StoreF32>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->init_tile_tunnel, b->init_tile_first, b->n_init_tiles, set); break;
+      case PW_STORAGE_F16: hipLaunchKernelGGL(batch_init_kernel<StoreF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); break;
+      case PW_STORAGE_BF16: hipLaunchKernelGGL(batch_init_kernel<StoreBF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); break;
+      default: hipLaunchKernelGGL(batch_init_kernel<StoreF32>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); break;
     }
   }
 }
@@ -216,8 +214,7 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   }
   // Validate every tunnel exactly as a single solver would, on the host.
   std::vector<StepParams> sps(count);
-  bool all_even = true;
-  size_t tiles = 0, init_tiles = 0, partials = 0;
+  size_t tiles = 0, partials = 0;
   for (uint32_t t = 0; t < count; ++t) {
     pw_params prm = *shared;
     prm.tau0 = tau0[t];
@@ -227,7 +224,6 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
       set_error("Tunnel " + std::to_string(t) + ": " + pw_last_error_message());
       return st;
     }
-    all_even = all_even && layouts[t].nx % 2 == 0;
   }
   pw_batch* b = new (std::nothrow) pw_batch();
   if (!b) {
@@ -236,11 +232,8 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   }
   b->params = *shared;
   b->count = count;
-  b->pair = all_even ? 1 : 0;
-  const uint32_t per_tile = b->pair ? 2 * kBlock : kBlock;
   for (uint32_t t = 0; t < count; ++t) {
-    tiles += (sps[t].cells + per_tile - 1) / per_tile;
-    init_tiles += (sps[t].cells + kBlock - 1) / kBlock;
+    tiles += (sps[t].cells + kBlock - 1) / kBlock;
     partials += (n_boundary[t] + kForceBlock - 1) / kForceBlock;
   }
   if (tiles > 0xffffffffull || partials > 0x7fffffffull) {
@@ -257,7 +250,7 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "batch stream");
   if ((e = hipEventCreate(&b->t0)) != hipSuccess || (e = hipEventCreate(&b->t1)) != hipSuccess) return fail(e, "batch events");
   const uint32_t elem = shared->storage == PW_STORAGE_F32 ? 4 : 2;
-  std::vector<uint32_t> tt, tf(count), it, itf(count), pt;
+  std::vector<uint32_t> tt, tf(count), pt;
   b->host.resize(count);
   b->layouts.assign(layouts, layouts + count);
   uint32_t part_base = 0;
@@ -287,25 +280,21 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
     for (uint32_t k = 0; k < tu.n_partials; ++k) pt.push_back(t);
     part_base += tu.n_partials;
     tf[t] = (uint32_t)tt.size();
-    for (size_t k = 0; k < (N + per_tile - 1) / per_tile; ++k) tt.push_back(t);
-    itf[t] = (uint32_t)it.size();
-    for (size_t k = 0; k < (N + kBlock - 1) / kBlock; ++k) it.push_back(t);
+    for (size_t k = 0; k < (N + kBlock - 1) / kBlock; ++k) tt.push_back(t);
   }
   b->n_tiles = (uint32_t)tt.size();
-  b->n_init_tiles = (uint32_t)it.size();
   b->n_partials = (uint32_t)pt.size();
   if ((e = dev_alloc(b, &b->tunnels, count)) != hipSuccess) return fail(e, "batch table");
   if ((e = dev_alloc(b, &b->tile_tunnel, tt.size())) != hipSuccess) return fail(e, "batch table");
   if ((e = dev_alloc(b, &b->tile_first, count)) != hipSuccess) return fail(e, "batch table");
-  if ((e = dev_alloc(b, &b->init_tile_tunnel, it.size())) != hipSuccess) return fail(e, "batch table");
-  if ((e = dev_alloc(b, &b->init_tile_first, count)) != hipSuccess) return fail(e, "batch table");
   if ((e = dev_alloc(b, &b->part_tunnel, pt.size())) != hipSuccess) return fail(e, "batch table");
   if ((e = dev_alloc(b, &b->partials, 3 * (size_t)(pt.size() ? pt.size() : 1))) != hipSuccess) return fail(e, "batch partials");
+  if ((e = dev_alloc(b, &b->seeds, 2)) != hipSuccess) return fail(e, "batch seeds");
+  const uint32_t seeds0[2] = {0, 0};
   if ((e = hipMemcpyAsync(b->tunnels, b->host.data(), count * sizeof(Tunnel), hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
       (e = hipMemcpyAsync(b->tile_tunnel, tt.data(), tt.size() * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
       (e = hipMemcpyAsync(b->tile_first, tf.data(), count * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(b->init_tile_tunnel, it.data(), it.size() * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(b->init_tile_first, itf.data(), count * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(b->seeds, seeds0, sizeof seeds0, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
       (pt.size() && (e = hipMemcpyAsync(b->part_tunnel, pt.data(), pt.size() * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess))
     return fail(e, "batch tables");
   int cus = 256;

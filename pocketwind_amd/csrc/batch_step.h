@@ -27,38 +27,35 @@ struct BatchLaunch {
   const uint32_t* tile_tunnel;
   const uint32_t* tile_first;
   uint32_t n_tiles;
-  int pair;
+  uint32_t* seeds;  // device: seeds[parity] is the step count a launch of that parity runs
   pw_params params;
 };
 
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool PAIR>
+// One step of every tunnel. The rounding seed is the step count, read on the device so a
+// replayed graph advances it: a launch of parity k reads seeds[k] and writes seeds[k ^ 1] for
+// the next launch, which in stream order starts after this one ends.
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
 __global__ __launch_bounds__(kBlock) void batch_step_kernel(const Tunnel* __restrict__ tunnels,
                                                             const uint32_t* __restrict__ tile_tunnel,
                                                             const uint32_t* __restrict__ tile_first,
-                                                            uint32_t n_tiles, int parity) {
-  constexpr uint32_t kCells = PAIR ? 2 * kBlock : kBlock;
+                                                            uint32_t n_tiles, int parity, uint32_t* seeds) {
+  const uint32_t seed = seeds[parity];
+  if (blockIdx.x == 0 && threadIdx.x == 0) seeds[parity ^ 1] = seed + 1;
   for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     const uint32_t t = tile_tunnel[tile];
     const Tunnel& tu = tunnels[t];
-    const uint32_t n = (tile - tile_first[t]) * kCells + (PAIR ? 2 : 1) * threadIdx.x;
+    const uint32_t n = (tile - tile_first[t]) * kBlock + threadIdx.x;
     if (n >= tu.p.cells) continue;
     const auto* in = (const typename S::T*)tu.pops[parity];
     auto* out = (typename S::T*)tu.pops[parity ^ 1];
-    if (PAIR)
-      stream_collide_pair<S, OP, LES, BOUZIDI, MACRO, true>(in, out, tu.geo, tu.macro, tu.p, n);
-    else
-      stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, true>(in, out, tu.geo, tu.macro, tu.p, n);
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, true>(in, out, tu.geo, tu.macro, tu.p, n, seed);
   }
 }
 
 template <class S, int OP, bool LES, bool BZ, bool MACRO>
 void batch_step_t(const BatchLaunch& L, int parity) {
-  if (L.pair)
-    hipLaunchKernelGGL((batch_step_kernel<S, OP, LES, BZ, MACRO, true>), dim3(L.grid), dim3(kBlock), 0, L.stream,
-                       L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles, parity);
-  else
-    hipLaunchKernelGGL((batch_step_kernel<S, OP, LES, BZ, MACRO, false>), dim3(L.grid), dim3(kBlock), 0, L.stream,
-                       L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles, parity);
+  hipLaunchKernelGGL((batch_step_kernel<S, OP, LES, BZ, MACRO>), dim3(L.grid), dim3(kBlock), 0, L.stream,
+                     L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles, parity, L.seeds);
 }
 
 template <class S, bool MACRO>

@@ -22,13 +22,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <class S, bool BOUZIDI>
 __device__ __forceinline__ void cell_exchange(const typename S::T* __restrict__ state,
                                               const uint32_t* __restrict__ geo, const StepParams& p,
-                                              const Neighbours& nb, uint32_t n, uint32_t flag,
+                                              const Neighbours<>& nb, uint32_t n, uint32_t flag,
                                               double& fx, double& fy, double& fz) {
   const size_t N = p.cells;
 #pragma unroll
   for (int i = 1; i < kQ; ++i) {
     if (!has_wall(flag, i)) continue;
-    const uint32_t source = nb.at(-cx(i), -cy(i), -cz(i));
+    const uint32_t source = nb.index(-cx(i), -cy(i), -cz(i));
     if ((geo[source] & kTypeMask) != kSolid) continue;
     const int j = opp(i);
     const double outgoing = S::load(state + (size_t)j * N + n);
@@ -50,8 +50,9 @@ __global__ __launch_bounds__(kForceBlock) void force_partials_kernel(
   if (item < n_boundary) {
     const uint32_t n = boundary[item];
     const uint32_t flag = geo[n];
-    const uint32_t x = n % p.nx, r = n / p.nx, y = r % p.ny, z = r / p.ny;
-    const Neighbours nb(x, y, z, p);
+    uint32_t x, y, z;
+    cell_coords(n, p, x, y, z);
+    const Neighbours<> nb(x, y, z, p);
     double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
     cell_exchange<S, BOUZIDI>(state_a, geo, p, nb, n, flag, ax, ay, az);
     cell_exchange<S, BOUZIDI>(state_b, geo, p, nb, n, flag, bx, by, bz);

@@ -1,9 +1,9 @@
 // lattice.h -- D3Q19 constants and population storage codecs for gfx950 kernels.
 //
 // Velocity order and weights follow src/lbm/lattice.ts:2-52 (opposite pairs adjacent,
-// i odd <-> i + 1); the codecs follow src/lbm/shaders/prelude-f16.wgsl:7-22 (f16 deviation
-// storage scaled by 2^15 with lowbias32 proportional rounding). bf16 storage is this
-// library's own format: the same hash decides the 16 dropped mantissa bits, no scale.
+// i odd <-> i + 1); the f16 codec keeps the format of src/lbm/shaders/prelude-f16.wgsl:5-22
+// (deviation scaled by 2^15, proportional stochastic rounding of the dropped bits). bf16
+// storage is this library's own format (deviation unscaled, the same rounding).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,55 +39,96 @@ constexpr uint32_t kNoQ = 0xffffffffu;
 __host__ __device__ __forceinline__ bool is_wall(uint32_t kind) { return kind - 1u < 3u; }
 __host__ __device__ __forceinline__ bool has_wall(uint32_t flag, int i) { return (flag >> (3 + i)) & 1u; }
 
-// lowbias32 over (value bits, cell, direction) (prelude-f16.wgsl:8-13).
-__host__ __device__ __forceinline__ uint32_t storage_hash(uint32_t bits, uint32_t i, uint32_t n) {
-  uint32_t r = bits ^ (n * 0x9e3779b9u) ^ (i * 0x85ebca6bu);
+// lowbias32 (public-domain hash-prospector constants; the same mixer as prelude-f16.wgsl:8-13).
+__host__ __device__ __forceinline__ uint32_t hash32(uint32_t r) {
   r = (r ^ (r >> 16)) * 0x7feb352du;
   r = (r ^ (r >> 15)) * 0x846ca68bu;
   return r ^ (r >> 16);
 }
 
-// Storage codecs: load() returns the f32 deviation g = f - w; store() takes g.
+// Random words for the stochastic rounding of one cell's 19 stores at one step: one hash per
+// cell and step, then a Weyl sequence over the directions, so every direction's word is
+// uniformly distributed and a cell draws fresh words each step.
+__host__ __device__ __forceinline__ uint32_t cell_random(uint32_t n, uint32_t seed) {
+  return hash32(n ^ (seed * 0x85ebca6bu));
+}
+__host__ __device__ __forceinline__ uint32_t direction_random(uint32_t h, int i) {
+  return h + (uint32_t)i * 0x9e3779b9u;
+}
+
+// Seed of the initial (reset) state; step k uses seed k.
+constexpr uint32_t kInitSeed = 0xffffffffu;
+
+// Stochastic rounding of an f32 to the format's dropped mantissa bits: round away from zero
+// with probability equal to the dropped fraction, chosen by the random word r. On the device
+// this is one v_cvt_sr_* instruction; the host pass (CPU checks of this kernel source) adds
+// r's low bits to the dropped bits and truncates, which has the same distribution.
+__host__ __device__ __forceinline__ _Float16 round_f16_sr(float v, uint32_t r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+  half2_t h = __builtin_nondeterministic_value(h);
+  h = __builtin_amdgcn_cvt_sr_f16_f32(h, v, r, false);
+  return h[0];
+#else
+  const uint32_t bits = as_uint(v), mag = bits & 0x7fffffffu;
+  if (mag >= 0x7f800000u) return static_cast<_Float16>(v);
+  if (mag < 0x38800000u) {  // f16 subnormal range: steps of 2^-24
+    const float steps = as_float(mag) * 16777216.0f;
+    const float low = __builtin_floorf(steps);
+    const bool up = (float)(r & 0xffffffu) * (1.0f / 16777216.0f) < steps - low;
+    const float m = (low + (up ? 1.0f : 0.0f)) * (1.0f / 16777216.0f);
+    return static_cast<_Float16>((bits >> 31) ? -m : m);
+  }
+  const uint32_t rounded = (mag + (r & 0x1fffu)) & 0xffffe000u;
+  return static_cast<_Float16>(as_float((bits & 0x80000000u) | rounded));
+#endif
+}
+
+__host__ __device__ __forceinline__ uint16_t round_bf16_sr(float v, uint32_t r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef short short2_t __attribute__((ext_vector_type(2)));
+  short2_t h = __builtin_nondeterministic_value(h);
+  h = __builtin_amdgcn_cvt_sr_bf16_f32(h, v, r, false);
+  return static_cast<uint16_t>(h[0]);
+#else
+  const uint32_t bits = as_uint(v), mag = bits & 0x7fffffffu;
+  if (mag >= 0x7f800000u) return static_cast<uint16_t>((bits >> 16) | (mag > 0x7f800000u ? 0x40u : 0u));
+  const uint32_t rounded = (mag + (r & 0xffffu)) & 0xffff0000u;
+  return static_cast<uint16_t>(((bits & 0x80000000u) | rounded) >> 16);
+#endif
+}
+
+// Storage codecs: decode() returns the f32 deviation g = f - w; encode() takes g and the
+// store's random word.
 struct StoreF32 {
   using T = float;
   static constexpr int kBytes = 4;
+  static constexpr int kShift = 2;
   __host__ __device__ __forceinline__ static float decode(T v) { return v; }
   __host__ __device__ __forceinline__ static float load(const T* p) { return *p; }
-  __host__ __device__ __forceinline__ static T encode(float g, uint32_t, uint32_t) { return g; }
+  __host__ __device__ __forceinline__ static T encode(float g, uint32_t) { return g; }
 };
 
+// f16 deviations scaled by 2^15 (prelude-f16.wgsl:5-22 fixes the format; the rounding's
+// random source is this library's own).
 struct StoreF16 {
   using T = _Float16;
   static constexpr int kBytes = 2;
+  static constexpr int kShift = 1;
   static constexpr float kScale = 32768.0f;
   __host__ __device__ __forceinline__ static float decode(T v) { return static_cast<float>(v) * (1.0f / kScale); }
   __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
-  __host__ __device__ __forceinline__ static T encode(float g, uint32_t i, uint32_t n) {
-    const float value = g * kScale;
-    const uint32_t bits = as_uint(value);
-    const uint32_t mag = bits & 0x7fffffffu;
-    // Below 2^-14 the half is subnormal: keep the native round-to-nearest cast.
-    if (mag < 0x38800000u) return static_cast<T>(value);
-    const uint32_t r = storage_hash(bits, i, n);
-    const uint32_t inc = (r & 0x1fffu) < (mag & 0x1fffu) ? 0x2000u : 0u;
-    const uint32_t rounded = (mag & 0xffffe000u) + inc;
-    return static_cast<T>(as_float((bits & 0x80000000u) | rounded));
-  }
+  __host__ __device__ __forceinline__ static T encode(float g, uint32_t r) { return round_f16_sr(g * kScale, r); }
 };
 
+// bf16 deviations, unscaled (this library's own format).
 struct StoreBF16 {
   using T = uint16_t;
   static constexpr int kBytes = 2;
+  static constexpr int kShift = 1;
   __host__ __device__ __forceinline__ static float decode(T v) { return as_float(static_cast<uint32_t>(v) << 16); }
   __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
-  __host__ __device__ __forceinline__ static T encode(float g, uint32_t i, uint32_t n) {
-    const uint32_t bits = as_uint(g);
-    const uint32_t mag = bits & 0x7fffffffu;
-    if (mag >= 0x7f800000u) return static_cast<T>((bits >> 16) | (mag > 0x7f800000u ? 0x40u : 0u));
-    const uint32_t r = storage_hash(bits, i, n);
-    const uint32_t inc = (r & 0xffffu) < (mag & 0xffffu) ? 0x10000u : 0u;
-    return static_cast<T>((((bits & 0x80000000u) | (mag & 0xffff0000u)) + inc) >> 16);
-  }
+  __host__ __device__ __forceinline__ static T encode(float g, uint32_t r) { return round_bf16_sr(g, r); }
 };
 
 // Population streams are read once and written once per step. NT selects non-temporal
@@ -107,43 +148,6 @@ __host__ __device__ __forceinline__ void st(T* p, T v) {
 template <class S, bool NT>
 __host__ __device__ __forceinline__ float load_pop(const typename S::T* p) {
   return S::decode(ld<NT>(p));
-}
-
-// Two consecutive storage elements moved by one access: one dword for 16-bit storage, two
-// for f32. The address of an x-shifted pull is only element-aligned; gfx950 serves such
-// loads as single unaligned dword accesses.
-typedef uint32_t u32_align2 __attribute__((aligned(2)));
-typedef uint64_t u64_align4 __attribute__((aligned(4)));
-
-template <class T>
-__host__ __device__ __forceinline__ T lo_of(uint32_t v) { return __builtin_bit_cast(T, (uint16_t)(v & 0xffffu)); }
-template <class T>
-__host__ __device__ __forceinline__ T hi_of(uint32_t v) { return __builtin_bit_cast(T, (uint16_t)(v >> 16)); }
-
-template <class S, bool NT>
-__host__ __device__ __forceinline__ void load_pair(const typename S::T* p, float& a, float& b) {
-  using T = typename S::T;
-  if constexpr (sizeof(T) == 4) {
-    const uint64_t v = ld<NT>(reinterpret_cast<const u64_align4*>(p));
-    a = S::decode(__builtin_bit_cast(T, (uint32_t)(v & 0xffffffffu)));
-    b = S::decode(__builtin_bit_cast(T, (uint32_t)(v >> 32)));
-  } else {
-    const uint32_t v = ld<NT>(reinterpret_cast<const u32_align2*>(p));
-    a = S::decode(lo_of<T>(v));
-    b = S::decode(hi_of<T>(v));
-  }
-}
-
-template <class S, bool NT>
-__host__ __device__ __forceinline__ void store_pair(typename S::T* p, typename S::T a, typename S::T b) {
-  using T = typename S::T;
-  if constexpr (sizeof(T) == 4) {
-    const uint64_t v = (uint64_t)__builtin_bit_cast(uint32_t, a) | ((uint64_t)__builtin_bit_cast(uint32_t, b) << 32);
-    st<NT>(reinterpret_cast<uint64_t*>(p), v);
-  } else {
-    const uint32_t v = (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
-    st<NT>(reinterpret_cast<uint32_t*>(p), v);
-  }
 }
 
 }  // namespace pw

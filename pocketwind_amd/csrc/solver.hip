@@ -107,6 +107,10 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
   out->tau0 = 3.0f * nu + 0.5f;
   out->cs_term = (float)(18 * M_SQRT2 * 0.1 * 0.1);
   out->lambda = 3.0f / 16.0f;
+  out->inv_tau0 = 1.0f / out->tau0;
+  out->wm0 = 1.0f / (out->lambda / (out->tau0 - 0.5f) + 0.5f);
+  out->div_nx = make_fast_div(L.nx);
+  out->div_ny = make_fast_div(L.ny);
   return true;
 }
 
@@ -127,19 +131,21 @@ void launch_init(pw_solver* s, void* out, const StepParams& p) {
 
 }  // namespace
 
-extern template void launch_step_s<StoreF32>(pw_solver*, const void*, void*, bool, const StepParams&);
-extern template void launch_step_s<StoreF16>(pw_solver*, const void*, void*, bool, const StepParams&);
-extern template void launch_step_s<StoreBF16>(pw_solver*, const void*, void*, bool, const StepParams&);
+extern template void launch_step_s<StoreF32>(pw_solver*, const void*, void*, bool, const StepParams&, uint32_t);
+extern template void launch_step_s<StoreF16>(pw_solver*, const void*, void*, bool, const StepParams&, uint32_t);
+extern template void launch_step_s<StoreBF16>(pw_solver*, const void*, void*, bool, const StepParams&, uint32_t);
 
 namespace {
 
+// One step from the current set into the other; the step count seeds the stores' rounding.
 void launch_step(pw_solver* s, bool macro, const StepParams& p) {
   const void* in = s->pops[s->current];
   void* out = s->pops[s->current ^ 1];
+  const uint32_t seed = (uint32_t)s->steps;
   switch (s->params.storage) {
-    case PW_STORAGE_F16: launch_step_s<StoreF16>(s, in, out, macro, p); break;
-    case PW_STORAGE_BF16: launch_step_s<StoreBF16>(s, in, out, macro, p); break;
-    default: launch_step_s<StoreF32>(s, in, out, macro, p); break;
+    case PW_STORAGE_F16: launch_step_s<StoreF16>(s, in, out, macro, p, seed); break;
+    case PW_STORAGE_BF16: launch_step_s<StoreBF16>(s, in, out, macro, p, seed); break;
+    default: launch_step_s<StoreF32>(s, in, out, macro, p, seed); break;
   }
 }
 
@@ -174,35 +180,39 @@ void launch_force(pw_solver* s, double* dest) {
   }
 }
 
-// Time each applicable step variant on the handle's own buffers and keep the fastest. The
-// populations are re-initialised afterwards, so tuning leaves no trace in the state.
+// Time each step variant at each block size on the handle's own buffers and keep the
+// fastest. The caller re-initialises (or restores) the populations afterwards, so tuning
+// leaves no trace in the state.
 hipError_t autotune(pw_solver* s) {
-  const uint32_t all[4] = {PW_KERNEL_CELL, PW_KERNEL_CELL_NT, PW_KERNEL_PAIR, PW_KERNEL_PAIR_NT};
-  const int n_candidates = s->layout.nx % 2 == 0 ? 4 : 2;
-  std::vector<uint32_t> candidates(all, all + n_candidates);
+  const uint32_t kernels[2] = {PW_KERNEL_CELL, PW_KERNEL_CELL_NT};
+  const uint32_t blocks[3] = {64, 128, 256};
   float best = 0;
-  uint32_t choice = PW_KERNEL_PAIR;
+  uint32_t choice = PW_KERNEL_CELL_NT, choice_block = 256;
   const int reps = s->cells >= (1u << 20) ? 4 : 2;
-  for (uint32_t k : candidates) {
-    s->kernel = k;
-    launch_step(s, false, s->sp);  // warm-up
-    s->current ^= 1;
-    hipError_t e = hipEventRecord(s->t0, s->stream);
-    if (e != hipSuccess) return e;
-    for (int r = 0; r < reps; ++r) {
-      launch_step(s, false, s->sp);
+  for (uint32_t k : kernels)
+    for (uint32_t b : blocks) {
+      s->kernel = k;
+      s->block = b;
+      launch_step(s, false, s->sp);  // warm-up
       s->current ^= 1;
+      hipError_t e = hipEventRecord(s->t0, s->stream);
+      if (e != hipSuccess) return e;
+      for (int r = 0; r < reps; ++r) {
+        launch_step(s, false, s->sp);
+        s->current ^= 1;
+      }
+      if ((e = hipEventRecord(s->t1, s->stream)) != hipSuccess) return e;
+      if ((e = hipEventSynchronize(s->t1)) != hipSuccess) return e;
+      float ms = 0;
+      if ((e = hipEventElapsedTime(&ms, s->t0, s->t1)) != hipSuccess) return e;
+      if (best == 0 || ms < best) {
+        best = ms;
+        choice = k;
+        choice_block = b;
+      }
     }
-    if ((e = hipEventRecord(s->t1, s->stream)) != hipSuccess) return e;
-    if ((e = hipEventSynchronize(s->t1)) != hipSuccess) return e;
-    float ms = 0;
-    if ((e = hipEventElapsedTime(&ms, s->t0, s->t1)) != hipSuccess) return e;
-    if (best == 0 || ms < best) {
-      best = ms;
-      choice = k;
-    }
-  }
   s->kernel = choice;
+  s->block = choice_block;
   s->current = 0;
   return hipGetLastError();
 }
@@ -353,8 +363,6 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   const char* forced = std::getenv("PW_STEP_KERNEL");
   if (forced && std::strcmp(forced, "cell") == 0) s->kernel = PW_KERNEL_CELL;
   else if (forced && std::strcmp(forced, "cell-nt") == 0) s->kernel = PW_KERNEL_CELL_NT;
-  else if (forced && std::strcmp(forced, "pair") == 0 && L.nx % 2 == 0) s->kernel = PW_KERNEL_PAIR;
-  else if (forced && std::strcmp(forced, "pair-nt") == 0 && L.nx % 2 == 0) s->kernel = PW_KERNEL_PAIR_NT;
   else if ((e = autotune(s)) != hipSuccess) return fail(device_error(e, "step autotune"));
   launch_init_both(s, sp);
   if ((e = hipGetLastError()) != hipSuccess) return fail(device_error(e, "init launch"));
@@ -508,19 +516,17 @@ pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
     if (e != hipSuccess) return device_error(e, "tune");
     return PW_OK;
   }
-  if ((kernel == PW_KERNEL_PAIR || kernel == PW_KERNEL_PAIR_NT) && s->layout.nx % 2 != 0) {
-    set_error("The paired step kernel needs an even nx.");
-    return PW_EINVAL;
-  }
-  if (kernel > PW_KERNEL_PAIR_NT) {
+  if (kernel > PW_KERNEL_CELL_NT) {
     set_error("Unknown step kernel.");
     return PW_EINVAL;
   }
   s->kernel = kernel;
+  s->block = kStepBlock;
   return PW_OK;
 }
 
 uint32_t pw_solver_kernel(const pw_solver* s) { return s ? s->kernel : 0; }
+uint32_t pw_solver_block_size(const pw_solver* s) { return s ? s->block : 0; }
 
 pw_status pw_solver_synchronize(pw_solver* s) {
   if (pw_status st = check_handle(s)) return st;

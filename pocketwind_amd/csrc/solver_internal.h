@@ -10,16 +10,41 @@
 
 namespace pw {
 
+// Division of a u32 by a fixed divisor d as a multiply-high and shifts (the round-up
+// method with a 33-bit magic number): n / d = (t + ((n - t) >> 1)) >> s with t = mulhi(n, m),
+// exact for every u32 n; d == 1 is the identity.
+struct FastDiv {
+  uint32_t d, m, s;
+};
+
+__host__ __device__ __forceinline__ uint32_t fast_div(uint32_t n, const FastDiv& f) {
+  if (f.d == 1) return n;
+  const uint32_t t = (uint32_t)(((uint64_t)n * f.m) >> 32);
+  return (t + ((n - t) >> 1)) >> f.s;
+}
+
+inline FastDiv make_fast_div(uint32_t d) {
+  if (d <= 1) return FastDiv{1, 0, 0};
+  uint32_t l = 0;
+  while (((uint64_t)1 << l) < d) ++l;  // ceil(log2 d)
+  const uint64_t m = (((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1;
+  return FastDiv{d, (uint32_t)m, l - 1};
+}
+
 // Per-launch scalars of the step kernel (the reference's 48-byte uniform, struct Params in
-// src/lbm/shaders/common.wgsl:31-46), passed by value so a ramped step needs no buffer write.
+// src/lbm/shaders/common.wgsl:31-46, plus constants this kernel precomputes), passed by
+// value so a ramped step needs no buffer write.
 struct StepParams {
   uint32_t nx, ny, nz;
   uint32_t cells;
-  float u_in;     // inlet speed this step (ramped)
-  float nu;       // f32 viscosity, (tau0 - 1/2) / 3 rounded as the reference packs it
-  float tau0;     // 3 nu + 1/2 evaluated in f32, as relaxation_time does
-  float cs_term;  // 18 sqrt(2) Cs^2
-  float lambda;   // TRT magic parameter 3/16
+  float u_in;      // inlet speed this step (ramped)
+  float nu;        // f32 viscosity, (tau0 - 1/2) / 3 rounded as the reference packs it
+  float tau0;      // 3 nu + 1/2 evaluated in f32, as relaxation_time does
+  float cs_term;   // 18 sqrt(2) Cs^2
+  float lambda;    // TRT magic parameter 3/16
+  float inv_tau0;  // 1 / tau0 in f32: the relaxation rate without LES
+  float wm0;       // TRT odd rate 1 / (lambda / (tau0 - 1/2) + 1/2) in f32
+  FastDiv div_nx, div_ny;
 };
 
 void set_error(const std::string& message);
@@ -61,4 +86,5 @@ struct pw_solver {
   long long first_fluid = -1;
   int destroyed = 0;
   uint32_t kernel = 1;  // pw_kernel actually used by step()
+  uint32_t block = 256;  // threads per step block
 };

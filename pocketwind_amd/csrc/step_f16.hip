@@ -2,5 +2,5 @@
 #include "step_launch.h"
 
 namespace pw {
-template void launch_step_s<StoreF16>(pw_solver*, const void*, void*, bool, const StepParams&);
+template void launch_step_s<StoreF16>(pw_solver*, const void*, void*, bool, const StepParams&, uint32_t);
 }  // namespace pw

@@ -5,6 +5,19 @@
 // of one plane and its stores to one plane are unit-stride across the wavefront.
 // Semantics follow src/lbm/shaders/stream_collide.wgsl:95-136 (and the collision,
 // wall and outlet helpers it assembles); arithmetic is f32 as in the reference.
+//
+// The collision is evaluated in pair form. Directions i = 2p + 1 and j = i + 1 are
+// opposite (c_j = -c_i), so
+//   * the moments need only the pair sums g_i + g_j (density, second moment) and the pair
+//     differences g_i - g_j (momentum);
+//   * the equilibria share an even part a_p and flip an odd part b_p: eq_i = a_p + b_p,
+//     eq_j = a_p - b_p;
+//   * the non-equilibrium stress sum_i c c (g_i - eq_i) is the second moment of g minus its
+//     closed form at equilibrium, (rho - 1)/3 I + rho u u (exact for D3Q19, whose weights
+//     are isotropic to fourth order);
+//   * the regularised post-collision state is even in c, so one correction per pair.
+// This is the reference's arithmetic regrouped: the same quantities in f32, summed in
+// another order.
 #pragma once
 
 #include "lattice.h"
@@ -14,40 +27,113 @@ namespace pw {
 
 enum { kOpTrt = 0, kOpReg = 1 };
 
+// Reciprocal and square root to about one ulp (v_rcp_f32 / v_sqrt_f32) on the device; the
+// host pass (CPU checks of this kernel source) uses the correctly rounded operations.
+__host__ __device__ __forceinline__ float fast_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+__host__ __device__ __forceinline__ float fast_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return __builtin_sqrtf(x);
+#endif
+}
+
 struct Stress {
   float xx, yy, zz, xy, xz, yz;
 };
 
 // common.wgsl:84-88: equilibrium with the density deviation kept apart from unit density.
 __host__ __device__ __forceinline__ float geq(int i, float rho_delta, float ux, float uy, float uz,
-                                     float uu) {
+                                              float uu) {
   const float cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
   return wf(i) * (rho_delta + (1.0f + rho_delta) * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu));
 }
 
-// common.wgsl:100-113
-__host__ __device__ __forceinline__ Stress neq_stress(const float* g, const float* eq) {
+constexpr int kPairs = 9;
+
+// Moments and pair-form equilibrium of one gathered cell (stream_collide.wgsl:115-126).
+struct CellState {
+  float rd, rho, inv_rho, ux, uy, uz;
+  float eq0;
+  float a[kPairs], b[kPairs];      // eq_{2p+1} = a + b, eq_{2p+2} = a - b
+  float sum[kPairs], dif[kPairs];  // g_{2p+1} + g_{2p+2}, g_{2p+1} - g_{2p+2}
+};
+
+__host__ __device__ __forceinline__ void cell_state(const float* g, CellState& c) {
+#pragma unroll
+  for (int p = 0; p < kPairs; ++p) {
+    c.sum[p] = g[2 * p + 1] + g[2 * p + 2];
+    c.dif[p] = g[2 * p + 1] - g[2 * p + 2];
+  }
+  float rd = g[0], mx = 0.0f, my = 0.0f, mz = 0.0f;
+#pragma unroll
+  for (int p = 0; p < kPairs; ++p) {
+    const int i = 2 * p + 1;
+    rd += c.sum[p];
+    if (cx(i)) mx += cx(i) * c.dif[p];
+    if (cy(i)) my += cy(i) * c.dif[p];
+    if (cz(i)) mz += cz(i) * c.dif[p];
+  }
+  c.rd = rd;
+  c.rho = 1.0f + rd;
+  c.inv_rho = fast_rcp(c.rho);
+  c.ux = mx * c.inv_rho;
+  c.uy = my * c.inv_rho;
+  c.uz = mz * c.inv_rho;
+  const float h = 1.5f * (c.ux * c.ux + c.uy * c.uy + c.uz * c.uz);
+  c.eq0 = wf(0) * (rd - c.rho * h);
+#pragma unroll
+  for (int p = 0; p < kPairs; ++p) {
+    const int i = 2 * p + 1;
+    const float cu = cx(i) * c.ux + cy(i) * c.uy + cz(i) * c.uz;
+    const float rw = wf(i) * c.rho;
+    c.a[p] = rw * (4.5f * cu * cu - h) + wf(i) * rd;
+    c.b[p] = (3.0f * rw) * cu;
+  }
+}
+
+__host__ __device__ __forceinline__ float eq_of(const CellState& c, int i) {
+  if (i == 0) return c.eq0;
+  const int p = (i - 1) / 2;
+  return (i & 1) ? c.a[p] + c.b[p] : c.a[p] - c.b[p];
+}
+
+// common.wgsl:100-113: non-equilibrium stress, here as the second moment of g minus its
+// equilibrium value (rho - 1)/3 I + rho u u.
+__host__ __device__ __forceinline__ Stress neq_stress(const CellState& c) {
   Stress s{0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    const float d = g[i] - eq[i];
-    if (cx(i)) s.xx += d;
-    if (cy(i)) s.yy += d;
-    if (cz(i)) s.zz += d;
-    if (cx(i) * cy(i)) s.xy += cx(i) * cy(i) * d;
-    if (cx(i) * cz(i)) s.xz += cx(i) * cz(i) * d;
-    if (cy(i) * cz(i)) s.yz += cy(i) * cz(i) * d;
+  for (int p = 0; p < kPairs; ++p) {
+    const int i = 2 * p + 1;
+    const float v = c.sum[p];
+    if (cx(i)) s.xx += v;
+    if (cy(i)) s.yy += v;
+    if (cz(i)) s.zz += v;
+    if (cx(i) * cy(i)) s.xy += cx(i) * cy(i) * v;
+    if (cx(i) * cz(i)) s.xz += cx(i) * cz(i) * v;
+    if (cy(i) * cz(i)) s.yz += cy(i) * cz(i) * v;
   }
+  const float third = c.rd * (1.0f / 3.0f);
+  const float rx = c.rho * c.ux, ry = c.rho * c.uy, rz = c.rho * c.uz;
+  s.xx -= rx * c.ux + third;
+  s.yy -= ry * c.uy + third;
+  s.zz -= rz * c.uz + third;
+  s.xy -= rx * c.uy;
+  s.xz -= rx * c.uz;
+  s.yz -= ry * c.uz;
   return s;
 }
 
 // common.wgsl:115-122 (closed-form Smagorinsky tau; docs/numerics.md "LES").
-template <bool LES>
-__host__ __device__ __forceinline__ float relaxation_time(const StepParams& p, const Stress& s, float rho) {
-  if (!LES) return p.tau0;
-  const float q = s.xx * s.xx + s.yy * s.yy + s.zz * s.zz +
-                  2.0f * (s.xy * s.xy + s.xz * s.xz + s.yz * s.yz);
-  return 0.5f * (p.tau0 + sqrtf(p.tau0 * p.tau0 + p.cs_term * sqrtf(q) / rho));
+__host__ __device__ __forceinline__ float les_tau(const StepParams& p, const Stress& s, float inv_rho) {
+  const float q = s.xx * s.xx + s.yy * s.yy + s.zz * s.zz + 2.0f * (s.xy * s.xy + s.xz * s.xz + s.yz * s.yz);
+  return 0.5f * (p.tau0 + fast_sqrt(p.tau0 * p.tau0 + p.cs_term * fast_sqrt(q) * inv_rho));
 }
 
 __host__ __device__ __forceinline__ float projected(int i, const Stress& s, float third) {
@@ -56,30 +142,36 @@ __host__ __device__ __forceinline__ float projected(int i, const Stress& s, floa
          third;
 }
 
-// collide-trt.wgsl:1-15 and collide-reg.wgsl:3-18; writes post into g.
+// collide-trt.wgsl:1-15 and collide-reg.wgsl:3-18; writes the post-collision state into g.
 template <int OP, bool LES>
-__host__ __device__ __forceinline__ void collide(const StepParams& p, float* g, const float* eq, float rho) {
+__host__ __device__ __forceinline__ void collide(const StepParams& p, float* g, const CellState& c) {
   Stress s{0, 0, 0, 0, 0, 0};
-  if (LES || OP == kOpReg) s = neq_stress(g, eq);
-  const float tau = relaxation_time<LES>(p, s, rho);
-  const float wp = 1.0f / tau;
+  if (LES || OP == kOpReg) s = neq_stress(c);
+  const float tau = LES ? les_tau(p, s, c.inv_rho) : p.tau0;
+  const float wp = LES ? fast_rcp(tau) : p.inv_tau0;
   if (OP == kOpReg) {
-    const float third = (s.xx + s.yy + s.zz) / 3.0f;
+    const float third = (s.xx + s.yy + s.zz) * (1.0f / 3.0f);
     const float k = (1.0f - wp) * 4.5f;
+    g[0] = c.eq0 - k * wf(0) * third;
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) g[i] = eq[i] + k * wf(i) * projected(i, s, third);
+    for (int q = 0; q < kPairs; ++q) {
+      const int i = 2 * q + 1;
+      const float t = c.a[q] + k * wf(i) * projected(i, s, third);
+      g[i] = t + c.b[q];
+      g[i + 1] = t - c.b[q];
+    }
   } else {
-    const float wm = 1.0f / (p.lambda / (tau - 0.5f) + 0.5f);
-    g[0] = g[0] - wp * (g[0] - eq[0]);
+    const float wm = LES ? fast_rcp(p.lambda * fast_rcp(tau - 0.5f) + 0.5f) : p.wm0;
+    g[0] = g[0] - wp * (g[0] - c.eq0);
 #pragma unroll
-    for (int i = 1; i < kQ; i += 2) {
-      const int j = i + 1;
-      const float even = 0.5f * wp * (g[i] + g[j] - eq[i] - eq[j]);
-      const float odd = 0.5f * wm * (g[i] - g[j] - eq[i] + eq[j]);
+    for (int q = 0; q < kPairs; ++q) {
+      const int i = 2 * q + 1;
+      const float even = wp * (0.5f * c.sum[q] - c.a[q]);
+      const float odd = wm * (0.5f * c.dif[q] - c.b[q]);
       const float gi = g[i] - even - odd;
-      const float gj = g[j] - even + odd;
+      const float gj = g[i + 1] - even + odd;
       g[i] = gi;
-      g[j] = gj;
+      g[i + 1] = gj;
     }
   }
 }
@@ -94,8 +186,8 @@ __host__ __device__ __forceinline__ float friction_velocity(float nu, float spee
 // stream_collide.wgsl:18-80: prescribe the smallest post-collision stress whose bounce-back
 // flux along the slip is the wall-law shear; falls back to the bulk collision.
 template <int OP>
-__host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float* g, const float* eq, float rho,
-                                             float ux, float uy, float uz, uint32_t flag) {
+__host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float* g, const CellState& c,
+                                                     uint32_t flag) {
   float nx = 0, ny = 0, nz = 0, exposed = 0;
 #pragma unroll
   for (int i = 1; i < kQ; ++i)
@@ -105,24 +197,24 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
       nz += wf(i) * cz(i);
       exposed += wf(i);
     }
-  float sx = ux, sy = uy, sz = uz;
+  float sx = c.ux, sy = c.uy, sz = c.uz;
   const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
   const bool projected_slip = nl > 1e-3f;
   if (projected_slip) {
     const float ax = nx / nl, ay = ny / nl, az = nz / nl;
-    const float d = ux * ax + uy * ay + uz * az;
+    const float d = c.ux * ax + c.uy * ay + c.uz * az;
     sx -= d * ax;
     sy -= d * ay;
     sz -= d * az;
   }
   const float speed = sqrtf(sx * sx + sy * sy + sz * sz);
   if (speed < 1e-7f) {
-    collide<OP, true>(p, g, eq, rho);
+    collide<OP, true>(p, g, c);
     return;
   }
   const float tx = sx / speed, ty = sy / speed, tz = sz / speed;
   const float ut = friction_velocity(p.nu, speed);
-  const float shear = 6.0f * exposed * rho * ut * ut;
+  const float shear = 6.0f * exposed * c.rho * ut * ut;
   float flux = 0;
   Stress b{0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -134,7 +226,8 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
     // The weight part of the flux, sum 2 along_j w_j = -2 n.t, is exactly zero once the slip
     // has had its normal component removed; evaluated in f32 it would only add the rounding
     // residual of t's normal component, amplified by 1 / |slip| when the slip is small.
-    flux += 2.0f * along * (projected_slip ? eq[j] : eq[j] + wf(j));
+    const float eqj = eq_of(c, j);
+    flux += 2.0f * along * (projected_slip ? eqj : eqj + wf(j));
     const float s = 9.0f * wf(j) * along;
     b.xx += s * (x * x - 1.0f / 3.0f);
     b.yy += s * (y * y - 1.0f / 3.0f);
@@ -145,7 +238,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
   }
   const float norm = b.xx * b.xx + b.yy * b.yy + b.zz * b.zz + 2.0f * (b.xy * b.xy + b.xz * b.xz + b.yz * b.yz);
   if (norm < 0.05f) {
-    collide<OP, true>(p, g, eq, rho);
+    collide<OP, true>(p, g, c);
     return;
   }
   const float scale = 4.5f * (shear - flux) / norm;
@@ -154,44 +247,67 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
   bool admissible = true;
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
-    post[i] = eq[i] + scale * wf(i) * projected(i, b, third);
+    post[i] = eq_of(c, i) + scale * wf(i) * projected(i, b, third);
     admissible = admissible && (post[i] + wf(i) >= 0.0f);
   }
   if (!admissible) {
-    collide<OP, true>(p, g, eq, rho);
+    collide<OP, true>(p, g, c);
     return;
   }
 #pragma unroll
   for (int i = 0; i < kQ; ++i) g[i] = post[i];
 }
 
-// Wrapped neighbour offsets of a cell: the flat index of (x + dx, y + dy, z + dz) is
-// xo[dx+1] + yo[dy+1] + zo[dz+1] with every axis periodic (common.wgsl:64-68).
+// Cell coordinates of a flat index.
+__host__ __device__ __forceinline__ void cell_coords(uint32_t n, const StepParams& p, uint32_t& x, uint32_t& y,
+                                                     uint32_t& z) {
+  const uint32_t r = fast_div(n, p.div_nx);
+  x = n - r * p.nx;
+  z = fast_div(r, p.div_ny);
+  y = r - z * p.ny;
+}
+
+// Wrapped neighbour offsets of a cell, scaled by 2^SHIFT (element bytes for the streaming
+// loads, 1 for flat indices): the offset of (x + dx, y + dy, z + dz) is
+// xo[dx+1] + yo[dy+1] + zo[dz+1] with every axis periodic (common.wgsl:64-68). Population
+// planes are below 4 GiB (the cell count is checked at create), so a byte offset fits a u32.
+template <int SHIFT = 0>
 struct Neighbours {
   uint32_t xo[3], yo[3], zo[3];
   __host__ __device__ __forceinline__ Neighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p) {
     const uint32_t row = p.nx, plane = p.nx * p.ny;
-    xo[0] = x == 0 ? p.nx - 1 : x - 1;
-    xo[1] = x;
-    xo[2] = x + 1 == p.nx ? 0 : x + 1;
-    yo[0] = (y == 0 ? p.ny - 1 : y - 1) * row;
-    yo[1] = y * row;
-    yo[2] = (y + 1 == p.ny ? 0 : y + 1) * row;
-    zo[0] = (z == 0 ? p.nz - 1 : z - 1) * plane;
-    zo[1] = z * plane;
-    zo[2] = (z + 1 == p.nz ? 0 : z + 1) * plane;
+    xo[0] = (x == 0 ? p.nx - 1 : x - 1) << SHIFT;
+    xo[1] = x << SHIFT;
+    xo[2] = (x + 1 == p.nx ? 0 : x + 1) << SHIFT;
+    yo[0] = ((y == 0 ? p.ny - 1 : y - 1) * row) << SHIFT;
+    yo[1] = (y * row) << SHIFT;
+    yo[2] = ((y + 1 == p.ny ? 0 : y + 1) * row) << SHIFT;
+    zo[0] = ((z == 0 ? p.nz - 1 : z - 1) * plane) << SHIFT;
+    zo[1] = (z * plane) << SHIFT;
+    zo[2] = ((z + 1 == p.nz ? 0 : z + 1) * plane) << SHIFT;
   }
   __host__ __device__ __forceinline__ uint32_t at(int dx, int dy, int dz) const {
     return xo[dx + 1] + yo[dy + 1] + zo[dz + 1];
   }
+  // Flat cell index of a neighbour.
+  __host__ __device__ __forceinline__ uint32_t index(int dx, int dy, int dz) const { return at(dx, dy, dz) >> SHIFT; }
 };
+
+template <class T>
+__host__ __device__ __forceinline__ const T* byte_offset(const T* base, uint32_t bytes) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + bytes);
+}
+template <class T>
+__host__ __device__ __forceinline__ T* byte_offset(T* base, uint32_t bytes) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + bytes);
+}
 
 // walls.wgsl:6-21: halfway bounce-back or linear Bouzidi for incoming direction i, evaluated
 // in R (f32 inside the step, f64 inside the drag reduction).
-template <class S, bool BOUZIDI, class R = float>
+template <class S, bool BOUZIDI, class R = float, int SHIFT>
 __host__ __device__ __forceinline__ R wall_value(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
-                                        const StepParams& p, const Neighbours& nb, uint32_t n,
-                                        uint32_t flag, int i) {
+                                                 const StepParams& p, const Neighbours<SHIFT>& nb, uint32_t n,
+                                                 uint32_t flag, int i) {
   const size_t N = p.cells;
   const int j = opp(i);
   const R reflected = S::load(in + (size_t)j * N + n);
@@ -202,28 +318,32 @@ __host__ __device__ __forceinline__ R wall_value(const typename S::T* __restrict
   const R q = pw::as_float(geo[2 * N + base + rank]);
   if (q == R(0.5)) return reflected;
   if (q < R(0.5)) {
-    const uint32_t second = nb.at(cx(i), cy(i), cz(i));
+    const uint32_t second = nb.index(cx(i), cy(i), cz(i));
     if ((geo[second] & kTypeMask) != kFluid) return reflected;
     return R(2) * q * reflected + (R(1) - R(2) * q) * R(S::load(in + (size_t)j * N + second));
   }
   return (reflected + (R(2) * q - R(1)) * R(S::load(in + (size_t)i * N + n))) / (R(2) * q);
 }
 
+// Stores of one cell's 19 values; `self` is the cell's byte offset in a plane and h its
+// random word for this step.
 template <class S, bool NT = false>
-__host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__ out, size_t N, uint32_t n, const float* g) {
+__host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__ out, size_t N, uint32_t self,
+                                                     const float* g, uint32_t h) {
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) st<NT>(out + (size_t)i * N + n, S::encode(g[i], (uint32_t)i, n));
+  for (int i = 0; i < kQ; ++i)
+    st<NT>(byte_offset(out + (size_t)i * N, self), S::encode(g[i], direction_random(h, i)));
 }
 
 // common.wgsl:145-150 with rho - 1 = 0.
-template <class S>
+template <class S, bool NT = false>
 __host__ __device__ __forceinline__ void store_equilibrium(typename S::T* __restrict__ out, size_t N, uint32_t n,
-                                                  float ux, float uy, float uz) {
+                                                           float ux, float uy, float uz, uint32_t h) {
   const float uu = ux * ux + uy * uy + uz * uz;
   float g[kQ];
 #pragma unroll
   for (int i = 0; i < kQ; ++i) g[i] = geq(i, 0.0f, ux, uy, uz, uu);
-  store_state<S>(out, N, n, g);
+  store_state<S, NT>(out, N, n << S::kShift, g, h);
 }
 
 // init.wgsl:1-8: walls at rest, every other cell at the inflow equilibrium.
@@ -235,7 +355,7 @@ __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ o
   if (n >= p.cells) return;
   const uint32_t kind = geo[n] & kTypeMask;
   const float ux = is_wall(kind) ? 0.0f : p.u_in;
-  store_equilibrium<S>(out, p.cells, n, ux, 0.0f, 0.0f);
+  store_equilibrium<S>(out, p.cells, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
   macro[n] = make_float4(1.0f, ux, 0.0f, 0.0f);
 }
 
@@ -243,42 +363,29 @@ __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ o
 // g becomes the post-collision state; returns (rho, u).
 template <int OP, bool LES>
 __host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, uint32_t flag) {
-  float rho_delta = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    rho_delta += g[i];
-    mx += cx(i) * g[i];
-    my += cy(i) * g[i];
-    mz += cz(i) * g[i];
-  }
-  const float rho = 1.0f + rho_delta;
-  const float ux = mx / rho, uy = my / rho, uz = mz / rho;
-  const float uu = ux * ux + uy * uy + uz * uz;
-  float eq[kQ];
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) eq[i] = geq(i, rho_delta, ux, uy, uz, uu);
+  CellState c;
+  cell_state(g, c);
   if (LES && (flag >> 3) != 0)
-    collide_wall<OP>(p, g, eq, rho, ux, uy, uz, flag);
+    collide_wall<OP>(p, g, c, flag);
   else
-    collide<OP, LES>(p, g, eq, rho);
-  return make_float4(rho, ux, uy, uz);
+    collide<OP, LES>(p, g, c);
+  return make_float4(c.rho, c.ux, c.uy, c.uz);
 }
 
-// stream_collide.wgsl:95-136, one cell per lane.
+// stream_collide.wgsl:95-136, one cell per lane; `seed` selects the step's rounding words.
 template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT = false>
 __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
                                                              float4* __restrict__ macro,
-                                                             const StepParams& p, uint32_t n) {
+                                                             const StepParams& p, uint32_t n, uint32_t seed) {
   const size_t N = p.cells;
   const uint32_t flag = geo[n];
   const uint32_t kind = flag & kTypeMask;
   if (is_wall(kind)) return;
-  const uint32_t x = n % p.nx;
-  const uint32_t r = n / p.nx;
-  const uint32_t y = r % p.ny;
-  const uint32_t z = r / p.ny;
+  uint32_t x, y, z;
+  cell_coords(n, p, x, y, z);
+  const uint32_t h = cell_random(n, seed);
   if (kind == kEq) {
     float ux = p.u_in, uy = 0.0f, uz = 0.0f;
     if (x == p.nx - 1 && x > 0) {
@@ -296,24 +403,26 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
       uy = my / mass;
       uz = mz / mass;
     }
-    store_equilibrium<S>(out, N, n, ux, uy, uz);
+    store_equilibrium<S, NT>(out, N, n, ux, uy, uz, h);
     if (MACRO) macro[n] = make_float4(1.0f, ux, uy, uz);
     return;
   }
-  const Neighbours nb(x, y, z, p);
+  const Neighbours<S::kShift> nb(x, y, z, p);
+  const uint32_t self = n << S::kShift;
   float g[kQ];
-  g[0] = load_pop<S, NT>(in + n);
+  g[0] = load_pop<S, NT>(byte_offset(in, self));
   if ((flag >> 3) == 0) {
 #pragma unroll
-    for (int i = 1; i < kQ; ++i) g[i] = load_pop<S, NT>(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)));
+    for (int i = 1; i < kQ; ++i)
+      g[i] = load_pop<S, NT>(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
   } else {
 #pragma unroll
     for (int i = 1; i < kQ; ++i)
       g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i)
-                               : S::load(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)));
+                               : S::load(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
   }
   const float4 m = relax<OP, LES>(p, g, flag);
-  store_state<S, NT>(out, N, n, g);
+  store_state<S, NT>(out, N, self, g, h);
   if (MACRO) macro[n] = m;
 }
 
@@ -321,53 +430,11 @@ template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT>
 __global__ __launch_bounds__(256) void stream_collide_kernel(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
-                                                             float4* __restrict__ macro, StepParams p) {
+                                                             float4* __restrict__ macro, StepParams p,
+                                                             uint32_t seed) {
   const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= p.cells) return;
-  stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n);
-}
-
-// Two x-adjacent cells per lane. A pair whose cells are both plain fluid (no wall links) and
-// whose pulls do not wrap in x gathers and stores each direction with one wide access;
-// every other pair takes the per-cell path. Needs an even nx.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT = false>
-__host__ __device__ __forceinline__ void stream_collide_pair(const typename S::T* __restrict__ in,
-                                                             typename S::T* __restrict__ out,
-                                                             const uint32_t* __restrict__ geo,
-                                                             float4* __restrict__ macro,
-                                                             const StepParams& p, uint32_t n) {
-  const uint2 f = *reinterpret_cast<const uint2*>(geo + n);
-  const uint32_t x = n % p.nx;
-  if ((f.x | f.y) != 0 || x == 0 || x + 2 == p.nx) {
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n);
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n + 1);
-    return;
-  }
-  const size_t N = p.cells;
-  const uint32_t r = n / p.nx;
-  const Neighbours nb(x, r % p.ny, r / p.ny, p);
-  float a[kQ], b[kQ];
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) load_pair<S, NT>(in + (size_t)i * N + nb.at(-cx(i), -cy(i), -cz(i)), a[i], b[i]);
-  const float4 ma = relax<OP, LES>(p, a, 0u);
-  const float4 mb = relax<OP, LES>(p, b, 0u);
-#pragma unroll
-  for (int i = 0; i < kQ; ++i)
-    store_pair<S, NT>(out + (size_t)i * N + n, S::encode(a[i], (uint32_t)i, n), S::encode(b[i], (uint32_t)i, n + 1));
-  if (MACRO) {
-    macro[n] = ma;
-    macro[n + 1] = mb;
-  }
-}
-
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT>
-__global__ __launch_bounds__(256) void stream_collide_pair_kernel(const typename S::T* __restrict__ in,
-                                                                  typename S::T* __restrict__ out,
-                                                                  const uint32_t* __restrict__ geo,
-                                                                  float4* __restrict__ macro, StepParams p) {
-  const uint32_t n = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-  if (n >= p.cells) return;
-  stream_collide_pair<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n);
+  stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n, seed);
 }
 
 }  // namespace pw

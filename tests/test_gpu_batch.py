@@ -41,7 +41,7 @@ def test_batch_equals_single_tunnels(precision):
         batch.step(1, True)
         for k, (grid, tau, u) in enumerate(tunnels):
             s = GpuSolver.create(grid.layout, grid, SolverParams(tau, u, False, precision=precision))
-            s.set_kernel("pair-nt")  # every layout here has an even nx, so the batch pairs cells
+            s.set_kernel("cell-nt")  # the batch's per-cell step with non-temporal streams
             single = s.step_sampled(203, 8)
             s.step(1, True)
             assert np.array_equal(single, samples[k]), f"tunnel {k} force samples differ"
