@@ -1,21 +1,35 @@
 #!/usr/bin/env python3
-"""Throughput of the D3Q19 step on one MI355X: MLUPS and % of HBM roofline.
+"""Throughput of the D3Q19 step on one MI355X: MLUPS and % of HBM roofline, with Cd.
 
-A "step" is one fused pull-stream + collide pass over the whole 256x128x128 lattice with the
-populations already resident in HBM. Timing: W untimed warm-up steps, then K steps between
-two HIP events recorded on the solver's own stream (the stream the step kernel runs on);
-every step is one launch of the step kernel, so the per-launch duration is the timed span
-divided by K. Algorithmic bytes per cell update follow the reference's BYTES_PER_CELL_STEP
-(src/app/bench-page.ts:44-45): 19 populations read + 19 written + one 4-byte flag, i.e.
-156 B at f32 and 80 B at f16/bf16 storage.
+Workload (the metric's configuration): the bluff body of 1.0 x 0.37 x 0.28 L (front edges
+rounded at 0.1 L, 25 degree rear slant over the last 0.2 L), L = 64 cells, nose at 0.25 nx,
+centred, in a 256x128x128 D3Q19 tunnel with EQ inlet, pressure outlet, periodic y/z,
+halfway bounce-back, regularised collision + Smagorinsky (Cs 0.1) with the wall-law
+boundary collision, Re 320 on L (nu 0.01, tau0 0.53), u_in 0.05. The grid is voxelised on
+the device. The reference's own throughput grid (all fluid, fully periodic,
+src/app/bench-page.ts:245-260) is timed beside it.
 
-The CPU baseline is the FP64 C restatement of the reference CPU solver (the oracle) on a
-bounded sample of the same workload, single-threaded like the reference's JS path.
+A "step" is one fused pull-stream + collide pass over the whole lattice with the populations
+resident in HBM: exactly one launch of the step kernel. Timing: W untimed warm-up steps,
+then K steps between two HIP events recorded on the solver's own stream (the stream the step
+kernel runs on), median of three repeats; the per-launch duration is the timed span / K.
+Algorithmic bytes per cell update follow the reference's BYTES_PER_CELL_STEP
+(src/app/bench-page.ts:44-45): 19 populations read + 19 written + one 4-byte flag, 156 B at
+f32 and 80 B at f16/bf16 storage, over every cell of the lattice.
+
+Cd: six flow-throughs (6 ceil(nx / u_in) steps) with an FP64 force sample every 8 steps,
+mean over the last two (src/validation/cases.ts:294-320), Cd = fx / (1/2 u^2 A) with A the
+voxel frontal area; reported for each storage precision on the same grid.
+
+The CPU baseline is the FP64 C restatement of the reference CPU solver (the oracle) stepping
+the same body grid single-threaded, like the reference's JS path, for a bounded number of
+steps.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -28,6 +42,9 @@ sys.path.insert(0, ROOT)
 METRIC = "MLUPS & % HBM roofline, D3Q19 256x128x128; learned+fp16 vs plain fp32 at Cd±1%"
 HBM_PEAK_GBPS = 8000.0
 LAYOUT = (256, 128, 128)
+BODY_LENGTH = 64
+U_IN = 0.05
+TAU0 = 3 * (U_IN * BODY_LENGTH / 320) + 0.5  # Re 320 on L: nu = 0.01, tau0 = 0.53
 
 
 def bytes_per_cell(precision: str) -> int:
@@ -40,6 +57,18 @@ def empty_grid(layout):
     geometry = np.zeros(2 * cells, np.uint32)
     geometry[cells:] = NO_Q
     return VoxelGrid(*layout, geometry)
+
+
+def body_grid(layout=LAYOUT, length=BODY_LENGTH):
+    from pocketwind_amd import bodies
+    from pocketwind_amd.voxelize import voxelize
+    body, _ = bodies.place_mesh(bodies.bluff_body(), None, length, layout, False)
+    return voxelize(body, None, layout, False, link_fractions=False)
+
+
+def params_for(precision: str):
+    from pocketwind_amd.gpu_solver import SolverParams
+    return SolverParams(TAU0, U_IN, True, operator="reg", precision=precision)
 
 
 def time_solver(grid, params, steps, warmup, repeats=3):
@@ -56,69 +85,98 @@ def time_solver(grid, params, steps, warmup, repeats=3):
             ms = solver.time_steps(steps)
             wall = (time.perf_counter() - wall0) * 1e3
             runs.append((ms, wall))
-        macro = solver.read_macro()
-        finite = bool(np.all(np.isfinite(macro)))
+        solver.step(1, True)
+        finite = bool(np.all(np.isfinite(solver.read_macro())))
+        variant = f"{solver.kernel}/{solver.block_size}"
     finally:
         solver.destroy()
     runs.sort()
     ms, wall = runs[len(runs) // 2]
-    return ms, wall, finite
+    launch_ms = ms / steps
+    cells = grid.cells
+    return {"ms": ms, "wall_ms": wall, "launch_us": launch_ms * 1e3,
+            "mlups": cells * steps / (ms * 1e3), "finite": finite, "variant": variant,
+            "gbps": bytes_per_cell(params.precision) * cells / (launch_ms * 1e6)}
 
 
-def cpu_baseline(seconds_target: float = 15.0):
-    """Oracle (FP64, one thread) on a 64x32x32 sample of the same periodic REG+LES tunnel."""
+def body_cd(grid, precision: str):
+    from pocketwind_amd import drag
+    from pocketwind_amd.gpu_solver import GpuSolver
+    solver = GpuSolver.create(grid.layout, grid, params_for(precision))
+    try:
+        flow = drag.flow_through_steps(grid.nx, U_IN)
+        samples = solver.step_sampled(6 * flow, 8)
+        fx, _, _ = drag.window_mean_force(samples, 6 * flow, 2 * flow)
+    finally:
+        solver.destroy()
+    return fx * drag.cd_per_lattice_force(grid.frontal_cells, U_IN)
+
+
+def cpu_baseline(grid, seconds_target: float = 15.0):
+    """Oracle (FP64, one thread) stepping the same body grid for about `seconds_target`."""
     import oracle
-    layout = (64, 32, 32)
-    cells = layout[0] * layout[1] * layout[2]
-    flags = np.zeros(cells, np.uint32)
-    cpu = oracle.Solver(layout, 0.53, oracle.REG, True, flags=flags,
-                        equilibrium_state=(1.0, (0.05, 0.0, 0.0)), threads=1)
-    cpu.step(2)
+    cells = grid.cells
+    cpu = oracle.Solver(grid.layout, TAU0, oracle.REG, True, flags=grid.flags,
+                        equilibrium_state=(1.0, (U_IN, 0.0, 0.0)), threads=1)
+    cpu.step(1)
     steps, elapsed = 0, 0.0
     t0 = time.perf_counter()
-    while elapsed < seconds_target and steps < 100000:
-        cpu.step(4)
-        steps += 4
+    while elapsed < seconds_target:
+        cpu.step(1)
+        steps += 1
         elapsed = time.perf_counter() - t0
     mlups = cells * steps / elapsed / 1e6
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
-            "sample": f"FP64 C oracle, REG+LES periodic {layout[0]}x{layout[1]}x{layout[2]}, "
-                      f"{steps} steps in {elapsed:.1f} s, one thread"}
+            "sample": f"FP64 C oracle (restated CpuSolver), the same 256x128x128 bluff-body REG+LES "
+                      f"tunnel, {steps} steps in {elapsed:.1f} s on one thread"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--only", default=None,
+                    help="comma-separated storage precisions (f16, f32, bf16) to time on the body "
+                         "grid, then one short copy probe, and nothing else (profiling runs)")
+    ap.add_argument("--no-cd", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs one tunnel shard per GPU and launches "
-              "no multi-GPU work (--gpus must be 1).", file=sys.stderr)
+        print("bench.py measures one GPU: this build runs one tunnel per GPU and launches no "
+              "multi-GPU work (--gpus must be 1).", file=sys.stderr)
         return 2
 
     from pocketwind_amd import _native
-    from pocketwind_amd.gpu_solver import SolverParams, copy_bandwidth
+    from pocketwind_amd.gpu_solver import copy_bandwidth
     if _native.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
 
-    grid = empty_grid(LAYOUT)
-    cells = grid.cells
-    results = {}
-    for precision in ("f32", "f16"):
-        params = SolverParams(0.53, 0.05, True, operator="reg", precision=precision)
-        ms, wall, finite = time_solver(grid, params, args.steps, args.warmup)
-        launch_ms = ms / args.steps
-        mlups = cells * args.steps / (ms * 1e3)
-        gbps = bytes_per_cell(precision) * cells / (launch_ms * 1e6)
-        results[precision] = dict(ms=ms, wall=wall, mlups=mlups, gbps=gbps, launch_ms=launch_ms,
-                                  finite=finite)
-        print(f"# {precision}: {mlups:.1f} MLUPS, {launch_ms * 1e3:.2f} us/step, {gbps:.0f} GB/s, "
-              f"wall {wall / args.steps * 1e3:.2f} us/step, finite={finite}", file=sys.stderr, flush=True)
-    copy_gbps = copy_bandwidth(256 << 20, 40)
-    head = results["f16"]
+    grid = body_grid()
+    precisions = args.only.split(",") if args.only else ["f16", "f32", "bf16"]
+    if any(p not in ("f16", "f32", "bf16") for p in precisions):
+        print("--only takes f16, f32 and bf16", file=sys.stderr)
+        return 2
+    body = {}
+    for precision in precisions:
+        r = time_solver(grid, params_for(precision), args.steps, args.warmup)
+        body[precision] = r
+        print(f"# body {precision}: {r['mlups']:.1f} MLUPS, {r['launch_us']:.2f} us/launch, "
+              f"{r['gbps']:.0f} GB/s, wall {r['wall_ms'] / args.steps * 1e3:.2f} us/step, "
+              f"{r['variant']}, finite={r['finite']}", file=sys.stderr, flush=True)
+    if args.only:
+        print(f"# copy probe: {copy_bandwidth(512 << 20, 4):.0f} GB/s (512 MiB, non-temporal float4)",
+              file=sys.stderr, flush=True)
+        return 0
+    empty = {}
+    for precision in ("f16", "f32"):
+        r = time_solver(empty_grid(LAYOUT), params_for(precision), args.steps, args.warmup)
+        empty[precision] = r
+        print(f"# empty {precision}: {r['mlups']:.1f} MLUPS, {r['launch_us']:.2f} us/launch, "
+              f"{r['gbps']:.0f} GB/s, {r['variant']}", file=sys.stderr, flush=True)
+    copy_gbps = copy_bandwidth(512 << 20, 40)
+    head = body["f16"]
     line = {
         "metric": METRIC,
         "value": round(head["mlups"], 1),
@@ -126,25 +184,39 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(head["launch_ms"], 5),
+        "ms_per_step": round(head["launch_us"] / 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 arithmetic, f16 storage",
         "data": "synthetic",
-        "config": {"workload": "empty periodic D3Q19 tunnel 256x128x128 (the reference bench grid), "
-                               "REG + Smagorinsky Cs 0.1, tau0 0.53, u_in 0.05",
-                   "precision": "f16 deviation storage", "parallelism": "single GPU"},
+        "config": {"workload": "bluff body L=64 (1.0x0.37x0.28 L, rounded nose, 25 deg slant) in a "
+                               "256x128x128 D3Q19 tunnel, REG + Smagorinsky Cs 0.1 + wall law, Re 320 "
+                               "(tau0 0.53), u_in 0.05, device-voxelised",
+                   "precision": "f16 deviation storage", "wall_closure": "wall law",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(head["gbps"], 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(head["gbps"] / HBM_PEAK_GBPS, 4), "traffic": None},
-        "fp32": {"value": round(results["f32"]["mlups"], 1), "ms_per_step": round(results["f32"]["launch_ms"], 5),
-                 "achieved_gbps": round(results["f32"]["gbps"], 1),
-                 "frac": round(results["f32"]["gbps"] / HBM_PEAK_GBPS, 4)},
-        "speedup_f16_vs_f32": round(head["mlups"] / results["f32"]["mlups"], 3),
+        "f32": {"value": round(body["f32"]["mlups"], 1), "ms_per_step": round(body["f32"]["launch_us"] / 1e3, 5),
+                "achieved_gbps": round(body["f32"]["gbps"], 1),
+                "frac": round(body["f32"]["gbps"] / HBM_PEAK_GBPS, 4)},
+        "bf16": {"value": round(body["bf16"]["mlups"], 1), "achieved_gbps": round(body["bf16"]["gbps"], 1)},
+        "speedup_f16_vs_f32": round(head["mlups"] / body["f32"]["mlups"], 3),
+        "empty_grid": {p: {"value": round(r["mlups"], 1), "achieved_gbps": round(r["gbps"], 1)}
+                       for p, r in empty.items()},
+        "variants": {p: r["variant"] for p, r in body.items()},
         "copy_gbps": round(copy_gbps, 1),
     }
+    if not args.no_cd:
+        cds = {}
+        for precision in ("f16", "f32", "bf16"):
+            cds[precision] = body_cd(grid, precision)
+            print(f"# Cd {precision}: {cds[precision]:.6f}", file=sys.stderr, flush=True)
+        line["cd"] = {p: round(v, 6) for p, v in cds.items()}
+        line["cd"]["f16_vs_f32_pct"] = round(100 * (cds["f16"] / cds["f32"] - 1), 4)
+        line["cd"]["bf16_vs_f32_pct"] = round(100 * (cds["bf16"] / cds["f32"] - 1), 4)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"] = cpu_baseline(grid)
     print(json.dumps(line))
     return 0
 

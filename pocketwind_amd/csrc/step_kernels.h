@@ -335,14 +335,26 @@ __host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__
     st<NT>(byte_offset(out + (size_t)i * N, self), S::encode(g[i], direction_random(h, i)));
 }
 
-// common.wgsl:145-150 with rho - 1 = 0.
+// common.wgsl:145-150 with rho - 1 = 0, in pair form: eq_i = w (4.5 cu^2 - 1.5 u^2) + 3 w cu.
+__host__ __device__ __forceinline__ void equilibrium_values(float ux, float uy, float uz, float* g) {
+  const float h15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+  g[0] = wf(0) * (0.0f - h15);  // +0 at rest, as the stored rest state of walls
+#pragma unroll
+  for (int q = 0; q < kPairs; ++q) {
+    const int i = 2 * q + 1;
+    const float cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+    const float a = wf(i) * (4.5f * cu * cu - h15);
+    const float b = (3.0f * wf(i)) * cu;
+    g[i] = a + b;
+    g[i + 1] = a - b;
+  }
+}
+
 template <class S, bool NT = false>
 __host__ __device__ __forceinline__ void store_equilibrium(typename S::T* __restrict__ out, size_t N, uint32_t n,
                                                            float ux, float uy, float uz, uint32_t h) {
-  const float uu = ux * ux + uy * uy + uz * uz;
   float g[kQ];
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) g[i] = geq(i, 0.0f, ux, uy, uz, uu);
+  equilibrium_values(ux, uy, uz, g);
   store_state<S, NT>(out, N, n << S::kShift, g, h);
 }
 
@@ -372,6 +384,42 @@ __host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, 
   return make_float4(c.rho, c.ux, c.uy, c.uz);
 }
 
+// stream_collide.wgsl:83-107: an EQ cell takes the equilibrium at unit density and the
+// inlet speed or, on the outlet plane x = nx - 1, the upstream cell's velocity (pressure
+// outlet). Writes the post-step values into g and returns (1, u).
+template <class S>
+__host__ __device__ __forceinline__ float4 eq_state(const typename S::T* __restrict__ in, const StepParams& p,
+                                                    uint32_t n, uint32_t x, float* g) {
+  float ux = p.u_in, uy = 0.0f, uz = 0.0f;
+  if (x == p.nx - 1 && x > 0) {
+    const size_t N = p.cells;
+    float mass = 1.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const float v = S::load(in + (size_t)i * N + (n - 1));
+      mass += v;
+      mx += cx(i) * v;
+      my += cy(i) * v;
+      mz += cz(i) * v;
+    }
+    const float inv = fast_rcp(mass);
+    ux = mx * inv;
+    uy = my * inv;
+    uz = mz * inv;
+  }
+  equilibrium_values(ux, uy, uz, g);
+  return make_float4(1.0f, ux, uy, uz);
+}
+
+// Walls keep the rest state (all deviations zero; stream_collide.wgsl:96 skips them). They
+// store it with their wavefront anyway: a wavefront whose lanes all store writes whole cache
+// lines, while a skipped lane leaves a partly written line that HBM must read, merge and
+// write back.
+__host__ __device__ __forceinline__ void rest_state(float* g) {
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) g[i] = 0.0f;
+}
+
 // stream_collide.wgsl:95-136, one cell per lane; `seed` selects the step's rounding words.
 template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT = false>
 __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T* __restrict__ in,
@@ -382,59 +430,139 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
   const size_t N = p.cells;
   const uint32_t flag = geo[n];
   const uint32_t kind = flag & kTypeMask;
-  if (is_wall(kind)) return;
   uint32_t x, y, z;
   cell_coords(n, p, x, y, z);
-  const uint32_t h = cell_random(n, seed);
-  if (kind == kEq) {
-    float ux = p.u_in, uy = 0.0f, uz = 0.0f;
-    if (x == p.nx - 1 && x > 0) {
-      // stream_collide.wgsl:83-93: pressure outlet takes the upstream velocity.
-      float mass = 1.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) {
-        const float v = S::load(in + (size_t)i * N + (n - 1));
-        mass += v;
-        mx += cx(i) * v;
-        my += cy(i) * v;
-        mz += cz(i) * v;
-      }
-      ux = mx / mass;
-      uy = my / mass;
-      uz = mz / mass;
-    }
-    store_equilibrium<S, NT>(out, N, n, ux, uy, uz, h);
-    if (MACRO) macro[n] = make_float4(1.0f, ux, uy, uz);
-    return;
-  }
-  const Neighbours<S::kShift> nb(x, y, z, p);
   const uint32_t self = n << S::kShift;
   float g[kQ];
-  g[0] = load_pop<S, NT>(byte_offset(in, self));
-  if ((flag >> 3) == 0) {
+  float4 m;
+  if (kind == kFluid) {
+    const Neighbours<S::kShift> nb(x, y, z, p);
+    g[0] = load_pop<S, NT>(byte_offset(in, self));
+    if ((flag >> 3) == 0) {
 #pragma unroll
-    for (int i = 1; i < kQ; ++i)
-      g[i] = load_pop<S, NT>(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
+      for (int i = 1; i < kQ; ++i)
+        g[i] = load_pop<S, NT>(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
+    } else {
+#pragma unroll
+      for (int i = 1; i < kQ; ++i)
+        g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i)
+                                 : S::load(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
+    }
+    m = relax<OP, LES>(p, g, flag);
+  } else if (kind == kEq) {
+    m = eq_state<S>(in, p, n, x, g);
   } else {
-#pragma unroll
-    for (int i = 1; i < kQ; ++i)
-      g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i)
-                               : S::load(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
+    rest_state(g);
+    m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
-  const float4 m = relax<OP, LES>(p, g, flag);
-  store_state<S, NT>(out, N, self, g, h);
-  if (MACRO) macro[n] = m;
+  store_state<S, NT>(out, N, self, g, cell_random(n, seed));
+  if (MACRO && !is_wall(kind)) macro[n] = m;
 }
+
+// Raw storage bits of one element, zero-extended to 32 bits, and their decoding.
+template <class S, bool NT>
+__device__ __forceinline__ uint32_t load_bits(const typename S::T* p) {
+  const typename S::T v = ld<NT>(p);
+  if constexpr (sizeof(typename S::T) == 4) return __builtin_bit_cast(uint32_t, v);
+  else return (uint32_t)__builtin_bit_cast(uint16_t, v);
+}
+template <class S>
+__device__ __forceinline__ float decode_bits(uint32_t b) {
+  using T = typename S::T;
+  if constexpr (sizeof(T) == 4) return S::decode(__builtin_bit_cast(T, b));
+  else return S::decode(__builtin_bit_cast(T, (uint16_t)b));
+}
+
+// Gather of a wavefront whose 64 cells lie in one x-row (nx a multiple of 64). Each plane is
+// read with aligned, unit-stride loads of the pull's source row, so every cache line of the
+// plane is fetched once. An x-shifted pull then takes its neighbour lane's value with a DPP
+// wavefront shift (wave_shr:1 moves lane l - 1 to lane l, wave_shl:1 lane l + 1 to lane l);
+// the lane at the wavefront's edge takes the value one load brings for the whole wavefront.
+// All 64 lanes must be active.
+template <class S, bool NT>
+__device__ __forceinline__ void gather_rows(const typename S::T* __restrict__ in, const StepParams& p, uint32_t x,
+                                            uint32_t y, uint32_t z, uint32_t lane, float* g) {
+  constexpr int SH = S::kShift;
+  const size_t N = p.cells;
+  const uint32_t row = p.nx, plane = p.nx * p.ny;
+  const uint32_t yo[3] = {((y == 0 ? p.ny - 1 : y - 1) * row) << SH, (y * row) << SH,
+                          ((y + 1 == p.ny ? 0 : y + 1) * row) << SH};
+  const uint32_t zo[3] = {((z == 0 ? p.nz - 1 : z - 1) * plane) << SH, (z * plane) << SH,
+                          ((z + 1 == p.nz ? 0 : z + 1) * plane) << SH};
+  const uint32_t x0 = x - lane;
+  const uint32_t left = (x0 == 0 ? p.nx - 1 : x0 - 1) << SH;     // source of lane 0 when c_x = +1
+  const uint32_t right = (x0 + 64 == p.nx ? 0 : x0 + 64) << SH;  // source of lane 63 when c_x = -1
+  const uint32_t xs = x << SH;
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const typename S::T* base = in + (size_t)i * N;
+    const uint32_t r = yo[1 - cy(i)] + zo[1 - cz(i)];
+    if (cx(i) == 0) {
+      g[i] = decode_bits<S>(load_bits<S, NT>(byte_offset(base, r + xs)));
+    } else {
+      const uint32_t own = load_bits<S, NT>(byte_offset(base, r + xs));
+      const uint32_t edge = load_bits<S, NT>(byte_offset(base, r + (cx(i) > 0 ? left : right)));
+      const int v = cx(i) > 0 ? __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x138, 0xf, 0xf, false)
+                              : __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x130, 0xf, 0xf, false);
+      g[i] = decode_bits<S>((uint32_t)v);
+    }
+  }
+}
+
+// stream_collide_cell for a wavefront of one x-row: the whole wavefront gathers (gather_rows),
+// then fluid lanes replace wall-link directions and collide, EQ and wall lanes take their
+// fixed states, and every lane stores. A wavefront of walls only skips the step.
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT>
+__device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __restrict__ in,
+                                                        typename S::T* __restrict__ out,
+                                                        const uint32_t* __restrict__ geo,
+                                                        float4* __restrict__ macro, const StepParams& p,
+                                                        uint32_t n, uint32_t seed) {
+  const uint32_t flag = geo[n];
+  const uint32_t kind = flag & kTypeMask;
+  if (__all(is_wall(kind))) return;
+  uint32_t x, y, z;
+  cell_coords(n, p, x, y, z);
+  float g[kQ];
+  gather_rows<S, NT>(in, p, x, y, z, threadIdx.x & 63u, g);
+  float4 m;
+  if (kind == kFluid) {
+    if ((flag >> 3) != 0) {
+      const Neighbours<> nb(x, y, z, p);
+#pragma unroll
+      for (int i = 1; i < kQ; ++i)
+        if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i);
+    }
+    m = relax<OP, LES>(p, g, flag);
+  } else if (kind == kEq) {
+    m = eq_state<S>(in, p, n, x, g);
+  } else {
+    rest_state(g);
+    m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+  }
+  store_state<S, NT>(out, p.cells, n << S::kShift, g, cell_random(n, seed));
+  if (MACRO && !is_wall(kind)) macro[n] = m;
+}
+
+// Blocks are renumbered so that each of the 8 XCDs (which receive blocks round-robin) steps
+// one contiguous eighth of the lattice: the cache lines that neighbouring rows and
+// wavefronts share stay in one XCD's L2. The grid holds 8 per_xcd blocks.
+constexpr uint32_t kXcds = 8;
 
 template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT>
 __global__ __launch_bounds__(256) void stream_collide_kernel(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
                                                              float4* __restrict__ macro, StepParams p,
-                                                             uint32_t seed) {
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= p.cells) return;
-  stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n, seed);
+                                                             uint32_t seed, uint32_t per_xcd) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t logical = (b % kXcds) * per_xcd + b / kXcds;
+  const uint32_t n = logical * blockDim.x + threadIdx.x;
+  if (n >= p.cells) return;  // whole wavefronts when nx is a multiple of 64
+  if (p.nx % 64 == 0)
+    stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n, seed);
+  else
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n, seed);
 }
 
 }  // namespace pw

@@ -15,9 +15,11 @@ void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const St
   const bool nt = s->kernel == PW_KERNEL_CELL_NT;
   auto k = macro ? (nt ? stream_collide_kernel<S, OP, LES, BOUZIDI, true, true> : stream_collide_kernel<S, OP, LES, BOUZIDI, true, false>)
                  : (nt ? stream_collide_kernel<S, OP, LES, BOUZIDI, false, true> : stream_collide_kernel<S, OP, LES, BOUZIDI, false, false>);
-  const unsigned block = s->block;
-  hipLaunchKernelGGL(k, dim3((unsigned)((s->cells + block - 1) / block)), dim3(block), 0, s->stream,
-                     (const typename S::T*)in, (typename S::T*)out, s->geometry, s->macro, p, seed);
+  const uint32_t block = s->block;
+  const uint64_t blocks = (s->cells + block - 1) / block;
+  const uint32_t per_xcd = (uint32_t)((blocks + kXcds - 1) / kXcds);
+  hipLaunchKernelGGL(k, dim3(kXcds * per_xcd), dim3(block), 0, s->stream, (const typename S::T*)in,
+                     (typename S::T*)out, s->geometry, s->macro, p, seed, per_xcd);
 }
 
 template <class S>
