@@ -139,6 +139,8 @@ def main():
     ap.add_argument("--only", default=None,
                     help="comma-separated storage precisions (f16, f32, bf16) to time on the body "
                          "grid, then one short copy probe, and nothing else (profiling runs)")
+    ap.add_argument("--grid", choices=["body", "empty"], default="body",
+                    help="grid of an --only run: the bluff body or the reference's all-fluid grid")
     ap.add_argument("--no-cd", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -153,7 +155,7 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         return 1
 
-    grid = body_grid()
+    grid = empty_grid(LAYOUT) if (args.only and args.grid == "empty") else body_grid()
     precisions = args.only.split(",") if args.only else ["f16", "f32", "bf16"]
     if any(p not in ("f16", "f32", "bf16") for p in precisions):
         print("--only takes f16, f32 and bf16", file=sys.stderr)
@@ -162,7 +164,7 @@ def main():
     for precision in precisions:
         r = time_solver(grid, params_for(precision), args.steps, args.warmup)
         body[precision] = r
-        print(f"# body {precision}: {r['mlups']:.1f} MLUPS, {r['launch_us']:.2f} us/launch, "
+        print(f"# {args.grid if args.only else 'body'} {precision}: {r['mlups']:.1f} MLUPS, {r['launch_us']:.2f} us/launch, "
               f"{r['gbps']:.0f} GB/s, wall {r['wall_ms'] / args.steps * 1e3:.2f} us/step, "
               f"{r['variant']}, finite={r['finite']}", file=sys.stderr, flush=True)
     if args.only:

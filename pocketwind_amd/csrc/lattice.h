@@ -119,6 +119,19 @@ struct StoreF16 {
   __host__ __device__ __forceinline__ static float decode(T v) { return static_cast<float>(v) * (1.0f / kScale); }
   __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
   __host__ __device__ __forceinline__ static T encode(float g, uint32_t r) { return round_f16_sr(g * kScale, r); }
+  // Two values rounded into the low and high halves of one register.
+  __host__ __device__ __forceinline__ static uint32_t encode_pair(float a, uint32_t ra, float b, uint32_t rb) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+    half2_t h = __builtin_nondeterministic_value(h);
+    h = __builtin_amdgcn_cvt_sr_f16_f32(h, a * kScale, ra, false);
+    h = __builtin_amdgcn_cvt_sr_f16_f32(h, b * kScale, rb, true);
+    return __builtin_bit_cast(uint32_t, h);
+#else
+    return (uint32_t)__builtin_bit_cast(uint16_t, encode(a, ra)) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, encode(b, rb)) << 16);
+#endif
+  }
 };
 
 // bf16 deviations, unscaled (this library's own format).
@@ -129,6 +142,17 @@ struct StoreBF16 {
   __host__ __device__ __forceinline__ static float decode(T v) { return as_float(static_cast<uint32_t>(v) << 16); }
   __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
   __host__ __device__ __forceinline__ static T encode(float g, uint32_t r) { return round_bf16_sr(g, r); }
+  __host__ __device__ __forceinline__ static uint32_t encode_pair(float a, uint32_t ra, float b, uint32_t rb) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef short short2_t __attribute__((ext_vector_type(2)));
+    short2_t h = __builtin_nondeterministic_value(h);
+    h = __builtin_amdgcn_cvt_sr_bf16_f32(h, a, ra, false);
+    h = __builtin_amdgcn_cvt_sr_bf16_f32(h, b, rb, true);
+    return __builtin_bit_cast(uint32_t, h);
+#else
+    return (uint32_t)encode(a, ra) | ((uint32_t)encode(b, rb) << 16);
+#endif
+  }
 };
 
 // Population streams are read once and written once per step. NT selects non-temporal

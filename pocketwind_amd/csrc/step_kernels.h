@@ -473,35 +473,41 @@ __device__ __forceinline__ float decode_bits(uint32_t b) {
   else return S::decode(__builtin_bit_cast(T, (uint16_t)b));
 }
 
-// Gather of a wavefront whose 64 cells lie in one x-row (nx a multiple of 64). Each plane is
-// read with aligned, unit-stride loads of the pull's source row, so every cache line of the
-// plane is fetched once. An x-shifted pull then takes its neighbour lane's value with a DPP
-// wavefront shift (wave_shr:1 moves lane l - 1 to lane l, wave_shl:1 lane l + 1 to lane l);
-// the lane at the wavefront's edge takes the value one load brings for the whole wavefront.
-// All 64 lanes must be active.
+// A wavefront whose 64 cells lie in one x-row (nx a multiple of 64) shares the row: its
+// first cell n0, coordinates x0, y, z, the source rows of its pulls and its store bases are
+// wavefront-uniform and live in scalar registers, so loads and stores address with the lane's
+// byte offset alone.
+struct RowWave {
+  uint32_t n0, x0, y, z;
+};
+
+// Gather of a row wavefront. Each plane is read with aligned, unit-stride loads of the pull's
+// source row, so every cache line of the plane is fetched once. An x-shifted pull then takes
+// its neighbour lane's value with a DPP wavefront shift (wave_shr:1 moves lane l - 1 to lane
+// l, wave_shl:1 lane l + 1 to lane l); the lane at the wavefront's edge takes the value one
+// load brings for the whole wavefront. All 64 lanes must be active.
 template <class S, bool NT>
-__device__ __forceinline__ void gather_rows(const typename S::T* __restrict__ in, const StepParams& p, uint32_t x,
-                                            uint32_t y, uint32_t z, uint32_t lane, float* g) {
+__device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in, const StepParams& p,
+                                           const RowWave& w, uint32_t lane, float* g) {
   constexpr int SH = S::kShift;
   const size_t N = p.cells;
   const uint32_t row = p.nx, plane = p.nx * p.ny;
+  const uint32_t y = w.y, z = w.z, x0 = w.x0;
   const uint32_t yo[3] = {((y == 0 ? p.ny - 1 : y - 1) * row) << SH, (y * row) << SH,
                           ((y + 1 == p.ny ? 0 : y + 1) * row) << SH};
   const uint32_t zo[3] = {((z == 0 ? p.nz - 1 : z - 1) * plane) << SH, (z * plane) << SH,
                           ((z + 1 == p.nz ? 0 : z + 1) * plane) << SH};
-  const uint32_t x0 = x - lane;
   const uint32_t left = (x0 == 0 ? p.nx - 1 : x0 - 1) << SH;     // source of lane 0 when c_x = +1
   const uint32_t right = (x0 + 64 == p.nx ? 0 : x0 + 64) << SH;  // source of lane 63 when c_x = -1
-  const uint32_t xs = x << SH;
+  const uint32_t xs = lane << SH;
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
-    const typename S::T* base = in + (size_t)i * N;
-    const uint32_t r = yo[1 - cy(i)] + zo[1 - cz(i)];
+    const typename S::T* src = byte_offset(in + (size_t)i * N, yo[1 - cy(i)] + zo[1 - cz(i)]);
+    const uint32_t own = load_bits<S, NT>(byte_offset(src, (x0 << SH) + xs));
     if (cx(i) == 0) {
-      g[i] = decode_bits<S>(load_bits<S, NT>(byte_offset(base, r + xs)));
+      g[i] = decode_bits<S>(own);
     } else {
-      const uint32_t own = load_bits<S, NT>(byte_offset(base, r + xs));
-      const uint32_t edge = load_bits<S, NT>(byte_offset(base, r + (cx(i) > 0 ? left : right)));
+      const uint32_t edge = load_bits<S, NT>(byte_offset(src, cx(i) > 0 ? left : right));
       const int v = cx(i) > 0 ? __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x138, 0xf, 0xf, false)
                               : __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x130, 0xf, 0xf, false);
       g[i] = decode_bits<S>((uint32_t)v);
@@ -509,9 +515,56 @@ __device__ __forceinline__ void gather_rows(const typename S::T* __restrict__ in
   }
 }
 
-// stream_collide_cell for a wavefront of one x-row: the whole wavefront gathers (gather_rows),
-// then fluid lanes replace wall-link directions and collide, EQ and wall lanes take their
-// fixed states, and every lane stores. A wavefront of walls only skips the step.
+// Base of plane i at cell n0 as an opaque wavefront-uniform global pointer. Kept opaque so
+// the compiler addresses each store as scalar base + lane offset rather than chaining 64-bit
+// vector adds from one plane to the next.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* plane_base(T* out, size_t N, int i, uint32_t n0) {
+  T* p = out + (size_t)i * N + n0;
+  asm("" : "+s"(p));
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+// Stores of a row wavefront: plane i of cell n0 + lane. 16-bit formats round two directions
+// into one register and store its halves.
+template <class S, bool NT>
+__device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_t N, uint32_t n0, uint32_t lane,
+                                          const float* g, uint32_t h) {
+  using T = typename S::T;
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const float v = S::encode(g[i], direction_random(h, i));
+      auto* b = plane_base(out, N, i, n0);
+      if constexpr (NT) __builtin_nontemporal_store(v, b + lane);
+      else b[lane] = v;
+    }
+  } else {
+    uint16_t* o = reinterpret_cast<uint16_t*>(out);
+    const uint16_t v0 = __builtin_bit_cast(uint16_t, S::encode(g[0], direction_random(h, 0)));
+    auto* b0 = plane_base(o, N, 0, n0);
+    if constexpr (NT) __builtin_nontemporal_store(v0, b0 + lane);
+    else b0[lane] = v0;
+#pragma unroll
+    for (int q = 0; q < kPairs; ++q) {
+      const int i = 2 * q + 1;
+      const uint32_t v = S::encode_pair(g[i], direction_random(h, i), g[i + 1], direction_random(h, i + 1));
+      auto* bi = plane_base(o, N, i, n0);
+      auto* bj = plane_base(o, N, i + 1, n0);
+      if constexpr (NT) {
+        __builtin_nontemporal_store((uint16_t)(v & 0xffffu), bi + lane);
+        __builtin_nontemporal_store((uint16_t)(v >> 16), bj + lane);
+      } else {
+        bi[lane] = (uint16_t)(v & 0xffffu);
+        bj[lane] = (uint16_t)(v >> 16);
+      }
+    }
+  }
+}
+
+// stream_collide_cell for a row wavefront: the whole wavefront gathers (gather_row), then fluid
+// lanes replace wall-link directions and collide, EQ and wall lanes take their fixed states,
+// and every lane stores. A wavefront of walls only skips the step.
 template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT>
 __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __restrict__ in,
                                                         typename S::T* __restrict__ out,
@@ -521,14 +574,17 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
   const uint32_t flag = geo[n];
   const uint32_t kind = flag & kTypeMask;
   if (__all(is_wall(kind))) return;
-  uint32_t x, y, z;
-  cell_coords(n, p, x, y, z);
+  const uint32_t lane = threadIdx.x & 63u;
+  RowWave w;
+  w.n0 = __builtin_amdgcn_readfirstlane(n - lane);
+  cell_coords(w.n0, p, w.x0, w.y, w.z);
   float g[kQ];
-  gather_rows<S, NT>(in, p, x, y, z, threadIdx.x & 63u, g);
+  gather_row<S, NT>(in, p, w, lane, g);
+  const uint32_t x = w.x0 + lane;
   float4 m;
   if (kind == kFluid) {
     if ((flag >> 3) != 0) {
-      const Neighbours<> nb(x, y, z, p);
+      const Neighbours<> nb(x, w.y, w.z, p);
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
         if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i);
@@ -540,7 +596,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     rest_state(g);
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
-  store_state<S, NT>(out, p.cells, n << S::kShift, g, cell_random(n, seed));
+  store_row<S, NT>(out, p.cells, w.n0, lane, g, cell_random(n, seed));
   if (MACRO && !is_wall(kind)) macro[n] = m;
 }
 
