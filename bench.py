@@ -99,6 +99,17 @@ def time_solver(grid, params, steps, warmup, repeats=3):
             "gbps": bytes_per_cell(params.precision) * cells / (launch_ms * 1e6)}
 
 
+def measured_traffic(precision: str, variant: str):
+    """HBM bytes per launch that rocprof counters gave for this storage and step variant
+    (profiles/step_traffic.json), or None when that variant was not profiled."""
+    path = os.path.join(ROOT, "profiles", "step_traffic.json")
+    try:
+        table = json.load(open(path))["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+    return table.get(f"{precision}/{variant.split('/')[0]}")
+
+
 def body_cd(grid, precision: str):
     from pocketwind_amd import drag
     from pocketwind_amd.gpu_solver import GpuSolver
@@ -198,7 +209,8 @@ def main():
                    "precision": "f16 deviation storage", "wall_closure": "wall law",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(head["gbps"], 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(head["gbps"] / HBM_PEAK_GBPS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(head["gbps"] / HBM_PEAK_GBPS, 4),
+                     "traffic": measured_traffic("f16", head["variant"])},
         "f32": {"value": round(body["f32"]["mlups"], 1), "ms_per_step": round(body["f32"]["launch_us"] / 1e3, 5),
                 "achieved_gbps": round(body["f32"]["gbps"], 1),
                 "frac": round(body["f32"]["gbps"] / HBM_PEAK_GBPS, 4)},

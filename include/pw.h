@@ -132,15 +132,17 @@ typedef struct {
 } pw_device_state;
 pw_status pw_solver_device_state(pw_solver *s, pw_device_state *out);
 
-/* Step-kernel variants: one cell per lane with cached or non-temporal population streams.
- * AUTO times both at block sizes 64, 128 and 256 on the handle's buffers (at create, or when
- * set later, restoring the state afterwards) and keeps the fastest; an explicit variant runs
- * at block size 256. No reference counterpart: the reference has one WGSL kernel
+/* Step-kernel variants: one cell per lane with cached population streams, non-temporal
+ * loads and stores, or cached loads with non-temporal stores. AUTO times all three at block
+ * sizes 64, 128 and 256 on the handle's buffers (at create, or when set later, restoring the
+ * state afterwards) and keeps the fastest; an explicit variant runs at block size 256. No
+ * reference counterpart: the reference has one WGSL kernel
  * (src/lbm/shaders/stream_collide.wgsl) with a fixed workgroup size. */
 typedef enum {
   PW_KERNEL_AUTO = 0,
   PW_KERNEL_CELL = 1,
-  PW_KERNEL_CELL_NT = 2
+  PW_KERNEL_CELL_NT = 2,
+  PW_KERNEL_CELL_NT_STORES = 3
 } pw_kernel;
 pw_status pw_solver_set_kernel(pw_solver *s, uint32_t kernel);
 uint32_t pw_solver_kernel(const pw_solver *s);

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(const Tunnel* __rest
     if (n >= tu.p.cells) continue;
     const auto* in = (const typename S::T*)tu.pops[parity];
     auto* out = (typename S::T*)tu.pops[parity ^ 1];
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, true>(in, out, tu.geo, tu.macro, tu.p, n, seed);
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, kNtAll>(in, out, tu.geo, tu.macro, tu.p, n, seed);
   }
 }
 

@@ -184,7 +184,7 @@ void launch_force(pw_solver* s, double* dest) {
 // fastest. The caller re-initialises (or restores) the populations afterwards, so tuning
 // leaves no trace in the state.
 hipError_t autotune(pw_solver* s) {
-  const uint32_t kernels[2] = {PW_KERNEL_CELL, PW_KERNEL_CELL_NT};
+  const uint32_t kernels[3] = {PW_KERNEL_CELL, PW_KERNEL_CELL_NT, PW_KERNEL_CELL_NT_STORES};
   const uint32_t blocks[3] = {64, 128, 256};
   float best = 0;
   uint32_t choice = PW_KERNEL_CELL_NT, choice_block = 256;
@@ -363,6 +363,7 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   const char* forced = std::getenv("PW_STEP_KERNEL");
   if (forced && std::strcmp(forced, "cell") == 0) s->kernel = PW_KERNEL_CELL;
   else if (forced && std::strcmp(forced, "cell-nt") == 0) s->kernel = PW_KERNEL_CELL_NT;
+  else if (forced && std::strcmp(forced, "cell-nt-stores") == 0) s->kernel = PW_KERNEL_CELL_NT_STORES;
   else if ((e = autotune(s)) != hipSuccess) return fail(device_error(e, "step autotune"));
   launch_init_both(s, sp);
   if ((e = hipGetLastError()) != hipSuccess) return fail(device_error(e, "init launch"));
@@ -516,7 +517,7 @@ pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
     if (e != hipSuccess) return device_error(e, "tune");
     return PW_OK;
   }
-  if (kernel > PW_KERNEL_CELL_NT) {
+  if (kernel > PW_KERNEL_CELL_NT_STORES) {
     set_error("Unknown step kernel.");
     return PW_EINVAL;
   }

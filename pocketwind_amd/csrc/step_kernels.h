@@ -420,8 +420,12 @@ __host__ __device__ __forceinline__ void rest_state(float* g) {
   for (int i = 0; i < kQ; ++i) g[i] = 0.0f;
 }
 
+// Cache policy of the population streams: cached, non-temporal loads and stores, or cached
+// loads with non-temporal stores. Flags and wall-path loads are always cached.
+enum { kCached = 0, kNtAll = 1, kNtStores = 2 };
+
 // stream_collide.wgsl:95-136, one cell per lane; `seed` selects the step's rounding words.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT = false>
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL = kCached>
 __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
@@ -437,11 +441,11 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
   float4 m;
   if (kind == kFluid) {
     const Neighbours<S::kShift> nb(x, y, z, p);
-    g[0] = load_pop<S, NT>(byte_offset(in, self));
+    g[0] = load_pop<S, POL == kNtAll>(byte_offset(in, self));
     if ((flag >> 3) == 0) {
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
-        g[i] = load_pop<S, NT>(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
+        g[i] = load_pop<S, POL == kNtAll>(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
     } else {
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
@@ -455,7 +459,7 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
     rest_state(g);
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
-  store_state<S, NT>(out, N, self, g, cell_random(n, seed));
+  store_state<S, POL != kCached>(out, N, self, g, cell_random(n, seed));
   if (MACRO && !is_wall(kind)) macro[n] = m;
 }
 
@@ -565,7 +569,7 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_
 // stream_collide_cell for a row wavefront: the whole wavefront gathers (gather_row), then fluid
 // lanes replace wall-link directions and collide, EQ and wall lanes take their fixed states,
 // and every lane stores. A wavefront of walls only skips the step.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT>
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL>
 __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __restrict__ in,
                                                         typename S::T* __restrict__ out,
                                                         const uint32_t* __restrict__ geo,
@@ -579,7 +583,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
   w.n0 = __builtin_amdgcn_readfirstlane(n - lane);
   cell_coords(w.n0, p, w.x0, w.y, w.z);
   float g[kQ];
-  gather_row<S, NT>(in, p, w, lane, g);
+  gather_row<S, POL == kNtAll>(in, p, w, lane, g);
   const uint32_t x = w.x0 + lane;
   float4 m;
   if (kind == kFluid) {
@@ -596,7 +600,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     rest_state(g);
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
-  store_row<S, NT>(out, p.cells, w.n0, lane, g, cell_random(n, seed));
+  store_row<S, POL != kCached>(out, p.cells, w.n0, lane, g, cell_random(n, seed));
   if (MACRO && !is_wall(kind)) macro[n] = m;
 }
 
@@ -605,7 +609,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
 // wavefronts share stay in one XCD's L2. The grid holds 8 per_xcd blocks.
 constexpr uint32_t kXcds = 8;
 
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, bool NT>
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL>
 __global__ __launch_bounds__(256) void stream_collide_kernel(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
@@ -616,9 +620,9 @@ __global__ __launch_bounds__(256) void stream_collide_kernel(const typename S::T
   const uint32_t n = logical * blockDim.x + threadIdx.x;
   if (n >= p.cells) return;  // whole wavefronts when nx is a multiple of 64
   if (p.nx % 64 == 0)
-    stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n, seed);
+    stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL>(in, out, geo, macro, p, n, seed);
   else
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, NT>(in, out, geo, macro, p, n, seed);
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL>(in, out, geo, macro, p, n, seed);
 }
 
 }  // namespace pw

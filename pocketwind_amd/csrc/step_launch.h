@@ -3,6 +3,8 @@
 // (step_f32.hip, step_f16.hip, step_bf16.hip) so the builds run in parallel.
 #pragma once
 
+#include <type_traits>
+
 #include "step_kernels.h"
 
 namespace pw {
@@ -12,9 +14,15 @@ constexpr int kStepBlock = 256;  // largest block of a step launch (the kernels'
 template <class S, int OP, bool LES, bool BB>
 void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const StepParams& p, uint32_t seed) {
   constexpr bool BOUZIDI = !BB;
-  const bool nt = s->kernel == PW_KERNEL_CELL_NT;
-  auto k = macro ? (nt ? stream_collide_kernel<S, OP, LES, BOUZIDI, true, true> : stream_collide_kernel<S, OP, LES, BOUZIDI, true, false>)
-                 : (nt ? stream_collide_kernel<S, OP, LES, BOUZIDI, false, true> : stream_collide_kernel<S, OP, LES, BOUZIDI, false, false>);
+  auto pick = [&](auto with_macro) {
+    constexpr bool M = decltype(with_macro)::value;
+    switch (s->kernel) {
+      case PW_KERNEL_CELL_NT: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtAll>;
+      case PW_KERNEL_CELL_NT_STORES: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtStores>;
+      default: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kCached>;
+    }
+  };
+  auto k = macro ? pick(std::true_type{}) : pick(std::false_type{});
   const uint32_t block = s->block;
   const uint64_t blocks = (s->cells + block - 1) / block;
   const uint32_t per_xcd = (uint32_t)((blocks + kXcds - 1) / kXcds);

@@ -185,17 +185,17 @@ class GpuSolver:
         N.check(N.lib().pw_solver_device_state(self._live(), C.byref(out)))
         return out
 
-    KERNELS = {"auto": 0, "cell": 1, "cell-nt": 2}
+    KERNELS = {"auto": 0, "cell": 1, "cell-nt": 2, "cell-nt-stores": 3}
 
     def set_kernel(self, kernel: str) -> None:
-        """Force a step-kernel variant ('cell', 'cell-nt') at block size 256, or re-tune
-        variant and block size ('auto')."""
+        """Force a step-kernel variant ('cell', 'cell-nt', 'cell-nt-stores') at block size 256,
+        or re-tune variant and block size ('auto')."""
         N.check(N.lib().pw_solver_set_kernel(self._live(), self.KERNELS[kernel]))
 
     @property
     def kernel(self) -> str:
         code = int(N.lib().pw_solver_kernel(self._live()))
-        return {1: "cell", 2: "cell-nt"}.get(code, "auto")
+        return {1: "cell", 2: "cell-nt", 3: "cell-nt-stores"}.get(code, "auto")
 
     @property
     def block_size(self) -> int:

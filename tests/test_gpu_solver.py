@@ -73,7 +73,7 @@ def assert_resting_walls(grid, macro, pops):
     assert np.abs(pops[:, walls].astype(np.float64)).max() == 0
 
 
-@pytest.mark.parametrize("kernel", ["cell", "cell-nt"])
+@pytest.mark.parametrize("kernel", ["cell", "cell-nt", "cell-nt-stores"])
 @pytest.mark.parametrize("operator,les,walls", CONFIGS)
 def test_f32_matches_oracle_on_seeded_blob(operator, les, walls, kernel):
     grid = blob_grid(walls)
@@ -124,7 +124,7 @@ LOW_PRECISION_BARS = {
 }
 
 
-@pytest.mark.parametrize("kernel", ["cell", "cell-nt"])
+@pytest.mark.parametrize("kernel", ["cell", "cell-nt", "cell-nt-stores"])
 @pytest.mark.parametrize("precision", ["f32", "f16", "bf16"])
 def test_low_precision_storage_stays_close(precision, kernel):
     grid = blob_grid("bb")
