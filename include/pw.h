@@ -232,6 +232,36 @@ uint64_t pw_batch_cells(const pw_batch *b);
 pw_status pw_batch_time_steps(pw_batch *b, uint32_t count, float *elapsed_ms);
 void pw_batch_destroy(pw_batch *b);
 
+/* Learned wall closure network: x (12 features) -> 64 -> 64 -> 3 with SiLU activations, bf16
+ * matrix-core products accumulated in f32. No reference counterpart: the reference's wall
+ * treatment is the wall law of src/lbm/shaders/stream_collide.wgsl:1-80, which stays the
+ * closure's fallback. Parameters are one f32 vector: W1 (64 x 12, row-major), b1 (64), W2
+ * (64 x 64), b2 (64), W3 (3 x 64), b3 (3). All arrays here are host arrays; a handle is not
+ * thread-safe. */
+#define PW_MLP_INPUTS 12
+#define PW_MLP_HIDDEN 64
+#define PW_MLP_OUTPUTS 3
+#define PW_MLP_PARAMS 5187
+typedef struct pw_mlp pw_mlp;
+pw_status pw_mlp_create(pw_mlp **out); /* all parameters zero */
+void pw_mlp_destroy(pw_mlp *m);
+/* Sets the f32 parameters (n == PW_MLP_PARAMS) and clears the Adam state. */
+pw_status pw_mlp_set_params(pw_mlp *m, const float *params, size_t n);
+pw_status pw_mlp_get_params(pw_mlp *m, float *params, size_t n);
+/* y (rows x 3) = network(x (rows x 12)). */
+pw_status pw_mlp_forward(pw_mlp *m, const float *x, size_t rows, float *y);
+/* Gradient of the mean squared error over all rows with respect to the parameters, and the
+ * loss (may be NULL). Does not change the parameters. */
+pw_status pw_mlp_gradients(pw_mlp *m, const float *x, const float *t, size_t rows, float *grads,
+                           float *loss);
+/* Training set (copied to the device). */
+pw_status pw_mlp_set_data(pw_mlp *m, const float *x, const float *t, size_t rows);
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8) at learning rate lr, each on a minibatch of
+ * `batch` rows drawn uniformly with replacement (seeded), or on the whole set when batch is
+ * 0; loss_history (may be NULL) receives each step's minibatch loss. */
+pw_status pw_mlp_train(pw_mlp *m, uint32_t steps, uint32_t batch, float lr, uint32_t seed,
+                       float *loss_history);
+
 #ifdef __cplusplus
 }
 #endif
