@@ -254,6 +254,28 @@ pw_status pw_mlp_forward(pw_mlp *m, const float *x, size_t rows, float *y);
  * loss (may be NULL). Does not change the parameters. */
 pw_status pw_mlp_gradients(pw_mlp *m, const float *x, const float *t, size_t rows, float *grads,
                            float *loss);
+/* Learned wall closure in a solver created with wall_model PW_WALL_LEARNED (which needs LES).
+ * A wall cell is a fluid cell with a wall link; the wall cells are listed in cell order. Before
+ * every step a closure kernel evaluates the attached network on the 12 features of each wall
+ * cell (see the features call) and the LES wall collision uses the resulting wall shear
+ * stress along the slip in place of the wall law's rho u_tau^2; until a network is attached,
+ * and for cells whose normalised features leave [-limit, limit], the wall law stays. */
+size_t pw_solver_wall_cell_count(const pw_solver *s);
+pw_status pw_solver_wall_cells(pw_solver *s, uint32_t *out, size_t n);
+/* Per wall cell of the latest state, 15 floats: velocity / u_ref, velocity / u_ref of the
+ * fluid cell one step along the rounded wall normal, mean wall-link fraction, log10 nu, unit
+ * wall normal, 6 x the wall links' weight sum, then the wall law's stress vector. */
+pw_status pw_solver_closure_features(pw_solver *s, float u_ref, float *out, size_t n_floats);
+/* Momentum-exchange force on the body from each wall cell (links to SOLID only, mean of the
+ * two latest states, the per-cell terms of the encodeForce sum), 3 doubles per wall cell. */
+pw_status pw_solver_wall_forces(pw_solver *s, double *out, size_t n_doubles);
+/* Attaches a copy of `net` with its normalisation: 12 feature means, 12 feature scales
+ * (normalised = (feature - mean) x scale), output scale (stress = output x scale), limit,
+ * velocity unit u_ref (27 floats). */
+pw_status pw_solver_set_closure(pw_solver *s, const pw_mlp *net, const float *norm, size_t n_norm);
+/* The last closure evaluation: 4 floats per wall cell, stress xyz and 1 (used) or 0 (wall
+ * law). */
+pw_status pw_solver_closure_output(pw_solver *s, float *out, size_t n_floats);
 /* Training set (copied to the device). */
 pw_status pw_mlp_set_data(pw_mlp *m, const float *x, const float *t, size_t rows);
 /* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8) at learning rate lr, each on a minibatch of

@@ -92,6 +92,12 @@ SIGNATURES = {
     "pw_batch_cells": (C.c_uint64, [C.c_void_p]),
     "pw_batch_time_steps": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
     "pw_batch_destroy": (None, [C.c_void_p]),
+    "pw_solver_wall_cell_count": (C.c_size_t, [C.c_void_p]),
+    "pw_solver_wall_cells": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_solver_closure_features": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t]),
+    "pw_solver_wall_forces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_solver_set_closure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_solver_closure_output": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "pw_mlp_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "pw_mlp_destroy": (None, [C.c_void_p]),
     "pw_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -92,3 +92,149 @@ class Mlp:
         losses = np.empty(steps, np.float32)
         N.check(N.lib().pw_mlp_train(self._h, steps, batch, C.c_float(lr), seed, losses.ctypes.data))
         return losses
+
+
+# --- Training data from this solver's own coarse/fine runs ---------------------------------
+
+N_NORM = 27  # 12 means, 12 scales, output scale, limit, velocity unit
+
+
+def random_body(seed: int) -> np.ndarray:
+    """Seeded rounded-box / slanted body (bodies.bluff_body with drawn proportions)."""
+    from . import bodies
+    rng = np.random.default_rng(seed)
+    return bodies.bluff_body(width=rng.uniform(0.28, 0.45), height=rng.uniform(0.22, 0.36),
+                             radius=rng.uniform(0.03, 0.14), slant_length=rng.uniform(0.1, 0.3),
+                             slant_deg=rng.uniform(0.0, 30.0))
+
+
+def _tunnel(mesh, layout, length, nu, u_in, wall_model="law", precision="f32"):
+    from . import bodies
+    from .gpu_solver import GpuSolver, SolverParams
+    from .voxelize import voxelize
+    body, _ = bodies.place_mesh(mesh, None, length, layout, False)
+    grid = voxelize(body, None, layout, False, link_fractions=True)
+    # Halfway bounce-back: the wall function (stream_collide.wgsl:18-80) assumes it; with
+    # Bouzidi walls it drives spurious near-wall velocities (the reference restatement too).
+    # The voxelizer's link fractions still reach the closure as its q feature.
+    params = SolverParams(3 * nu + 0.5, u_in, True, operator="reg", walls="bb", precision=precision,
+                          wall_model=wall_model)
+    solver = GpuSolver.create(layout, grid, params)
+    if not isinstance(solver, GpuSolver):
+        raise MemoryError(solver.message)
+    return solver, grid
+
+
+def _averaged(solver, warmup, window, snapshots, read):
+    solver.step(warmup, False)
+    every = max(1, window // snapshots)
+    acc = None
+    for _ in range(snapshots):
+        solver.step(every, False)
+        v = read()
+        acc = v.astype(np.float64) if acc is None else acc + v
+    return acc / snapshots
+
+
+def _coords(cells, layout):
+    nx, ny, _ = layout
+    cells = np.asarray(cells, np.int64)
+    return np.stack([cells % nx, (cells // nx) % ny, cells // (nx * ny)], 1)
+
+
+def restrict_stress(coarse_cells, coarse_layout, normals, fine_cells, fine_layout, fine_forces, fine_normals,
+                    fine_exposed, radius: int = 2, align: float = 0.9):
+    """Wall shear stress the fine run puts on the patch of wall of each coarse wall cell: over
+    the fine wall cells within `radius` fine cells of the coarse cell's image (2X) whose wall
+    normal is within acos(align) of the coarse one, the sum of their forces with each one's
+    normal component removed (then the coarse normal's), over the sum of their exposed weights
+    (6 sum w = faces). Returns (stress, support), support the fine face count."""
+    fc = _coords(fine_cells, fine_layout)
+    index = {tuple(c): k for k, c in enumerate(fc)}
+    cc = 2 * _coords(coarse_cells, coarse_layout)
+    ff = np.asarray(fine_forces, np.float64)
+    fn = np.asarray(fine_normals, np.float64)
+    tangential = ff - np.sum(ff * fn, 1, keepdims=True) * fn
+    stress = np.zeros((len(coarse_cells), 3))
+    support = np.zeros(len(coarse_cells))
+    span = range(-radius, radius + 1)
+    nfx, nfy, nfz = fine_layout
+    for k, (x, y, z) in enumerate(cc):
+        n = normals[k]
+        f = np.zeros(3)
+        area = 0.0
+        for dz in span:
+            for dy in span:
+                for dx in span:
+                    j = index.get(((x + dx) % nfx, (y + dy) % nfy, (z + dz) % nfz))
+                    if j is not None and np.dot(fn[j], n) >= align:
+                        f += tangential[j]
+                        area += fine_exposed[j]
+        if area > 0:
+            f = f - np.dot(f, n) * n
+            stress[k] = f / area
+            support[k] = area
+    return stress, support
+
+
+def coarse_fine_pair(seed: int, coarse_layout=(48, 24, 24), length: int = 16, re_range=(100.0, 400.0),
+                     u_in: float = 0.05, flow_throughs: float = 4.0, snapshots: int = 30):
+    """One training body: the closure features of its coarse run and the wall shear stress
+    its 2x finer run puts on the same wall, both averaged over the last flow-through.
+    Returns dict(features (n x 12), law (n x 3), target (n x 3), coarse_stress (n x 3))."""
+    rng = np.random.default_rng(seed + 7919)
+    re = rng.uniform(*re_range)
+    nu = u_in * length / re
+    mesh = random_body(seed)
+    fine_layout = tuple(2 * v for v in coarse_layout)
+    flow = int(np.ceil(coarse_layout[0] / u_in))
+    warm = int((flow_throughs - 1) * flow)
+    solver, _ = _tunnel(mesh, coarse_layout, length, nu, u_in)
+    try:
+        rec = _averaged(solver, warm, flow, snapshots, lambda: solver.closure_features(u_in))
+        cells = solver.wall_cells()
+        coarse_forces = _averaged(solver, 0, flow // 4, max(1, snapshots // 4), solver.wall_forces)
+    finally:
+        solver.destroy()
+    solver, _ = _tunnel(mesh, fine_layout, 2 * length, 2 * nu, u_in)
+    try:
+        fine_forces = _averaged(solver, 2 * warm, 2 * flow, snapshots, solver.wall_forces)
+        fine_cells = solver.wall_cells()
+        fine_rec = solver.closure_features(u_in)
+    finally:
+        solver.destroy()
+    normals = rec[:, 8:11]
+    normals = normals / np.maximum(np.linalg.norm(normals, axis=1, keepdims=True), 1e-12)
+    target, support = restrict_stress(cells, coarse_layout, normals, fine_cells, fine_layout, fine_forces,
+                                      fine_rec[:, 8:11], fine_rec[:, 11].astype(np.float64))
+    area = rec[:, 11].astype(np.float64)
+    cf = coarse_forces - np.sum(coarse_forces * normals, 1, keepdims=True) * normals
+    keep = support > 0
+    return dict(features=rec[keep, :12], law=rec[keep, 12:15], target=target[keep],
+                coarse_stress=(cf / area[:, None])[keep], re=re)
+
+
+def fit_normalization(features, targets, limit: float = 6.0, u_ref: float = 0.05) -> np.ndarray:
+    mean = features.mean(0)
+    std = features.std(0)
+    scale = np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-6), 1.0)
+    out = float(np.sqrt(np.mean(targets ** 2))) or 1.0
+    return np.concatenate([mean, scale, [out, limit, u_ref]]).astype(np.float32)
+
+
+def normalise(features, norm):
+    return (features - norm[:12]) * norm[12:24]
+
+
+def train_closure(features, targets, steps: int = 4000, batch: int = 512, lr: float = 2e-3, seed: int = 0,
+                  norm=None):
+    """Fits the network on the device. Returns (net, norm, loss history)."""
+    norm = fit_normalization(features, targets) if norm is None else norm
+    net = Mlp(init_params(seed))
+    net.set_data(normalise(features, norm), targets / norm[24])
+    history = net.train(steps, batch, lr, seed=seed)
+    return net, norm, history
+
+
+def predict(net, norm, features) -> np.ndarray:
+    return net.forward(normalise(features, norm)) * norm[24]

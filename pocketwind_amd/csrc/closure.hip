@@ -1,0 +1,169 @@
+// closure.hip -- the learned wall closure inside a solver: its per-step launch and the
+// feature, output and wall-force readbacks that training and evaluation use.
+//
+// No reference counterpart (the reference's wall treatment is the wall law of
+// src/lbm/shaders/stream_collide.wgsl:1-80, which stays the fallback).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "closure.h"
+
+namespace pw {
+
+using closure::Norm;
+
+namespace {
+
+inline unsigned closure_blocks(size_t n) { return (unsigned)((n + 63) / 64); }
+
+template <class S>
+void launch_closure_s(pw_solver* s, const void* state, const void* net, float4* tau, float* records) {
+  Norm norm;
+  std::memcpy(&norm, s->closure_norm, sizeof norm);
+  if (!records && norm.u_ref == 0.0f) norm.u_ref = 1.0f;
+  const bool bz = s->params.walls == PW_WALLS_BOUZIDI;
+  auto k = bz ? closure::closure_kernel<S, true> : closure::closure_kernel<S, false>;
+  hipLaunchKernelGGL(k, dim3(closure_blocks(s->n_wall)), dim3(64), 0, s->stream, (const typename S::T*)state,
+                     s->geometry, s->sp, s->wall_cells, (uint32_t)s->n_wall, (const mlp::Packed*)net, norm, tau,
+                     records);
+}
+
+void launch_closure_any(pw_solver* s, const void* state, const void* net, float4* tau, float* records) {
+  switch (s->params.storage) {
+    case PW_STORAGE_F16: launch_closure_s<StoreF16>(s, state, net, tau, records); break;
+    case PW_STORAGE_BF16: launch_closure_s<StoreBF16>(s, state, net, tau, records); break;
+    default: launch_closure_s<StoreF32>(s, state, net, tau, records); break;
+  }
+}
+
+template <class S>
+void launch_wall_forces_s(pw_solver* s, double* out) {
+  const bool bz = s->params.walls == PW_WALLS_BOUZIDI;
+  auto k = bz ? closure::wall_force_kernel<S, true> : closure::wall_force_kernel<S, false>;
+  hipLaunchKernelGGL(k, dim3((unsigned)((s->n_wall + 255) / 256)), dim3(256), 0, s->stream,
+                     (const typename S::T*)s->pops[0], (const typename S::T*)s->pops[1], s->geometry, s->sp,
+                     s->wall_cells, (uint32_t)s->n_wall, out);
+}
+
+pw_status check(pw_solver* s) {
+  if (!s || s->destroyed) {
+    set_error("Null or destroyed solver handle.");
+    return PW_EINVAL;
+  }
+  return PW_OK;
+}
+
+}  // namespace
+
+void launch_closure(pw_solver* s, StepParams& p) {
+  if (!s->closure_on || s->n_wall == 0) return;
+  launch_closure_any(s, s->pops[s->current], s->closure_net, s->closure_tau, nullptr);
+  p.closure_tau = s->closure_tau;
+  p.closure_slot = s->closure_slot;
+}
+
+}  // namespace pw
+
+using namespace pw;
+
+extern "C" {
+
+size_t pw_solver_wall_cell_count(const pw_solver* s) { return (s && !s->destroyed) ? s->n_wall : 0; }
+
+pw_status pw_solver_wall_cells(pw_solver* s, uint32_t* out, size_t n) {
+  if (pw_status st = check(s)) return st;
+  if (!out || n < s->n_wall) {
+    set_error("Wall cell output needs pw_solver_wall_cell_count entries.");
+    return PW_EINVAL;
+  }
+  hipError_t e = s->n_wall ? hipMemcpyAsync(out, s->wall_cells, s->n_wall * 4, hipMemcpyDeviceToHost, s->stream) : hipSuccess;
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  return e == hipSuccess ? PW_OK : device_error(e, "wall cell readback");
+}
+
+pw_status pw_solver_closure_features(pw_solver* s, float u_ref, float* out, size_t n_floats) {
+  if (pw_status st = check(s)) return st;
+  if (!out || n_floats < s->n_wall * closure::kRecord || !(u_ref > 0.0f)) {
+    set_error("Feature output needs 15 floats per wall cell and a positive velocity unit.");
+    return PW_EINVAL;
+  }
+  if (s->n_wall == 0) return PW_OK;
+  float* records = nullptr;
+  hipError_t e = hipMallocAsync((void**)&records, s->n_wall * closure::kRecord * sizeof(float), s->stream);
+  if (e != hipSuccess) return device_error(e, "feature buffer");
+  const float keep = s->closure_norm[2 * closure::kFeatures + 2];
+  s->closure_norm[2 * closure::kFeatures + 2] = u_ref;
+  launch_closure_any(s, s->pops[s->current], nullptr, nullptr, records);
+  s->closure_norm[2 * closure::kFeatures + 2] = keep;
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, records, s->n_wall * closure::kRecord * sizeof(float), hipMemcpyDeviceToHost, s->stream);
+  (void)hipFreeAsync(records, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  return e == hipSuccess ? PW_OK : device_error(e, "features");
+}
+
+pw_status pw_solver_closure_output(pw_solver* s, float* out, size_t n_floats) {
+  if (pw_status st = check(s)) return st;
+  if (!s->closure_tau || !out || n_floats < 4 * s->n_wall) {
+    set_error("Closure output needs a learned-closure solver and 4 floats per wall cell.");
+    return PW_EINVAL;
+  }
+  hipError_t e = s->n_wall ? hipMemcpyAsync(out, s->closure_tau, s->n_wall * sizeof(float4), hipMemcpyDeviceToHost, s->stream)
+                           : hipSuccess;
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  return e == hipSuccess ? PW_OK : device_error(e, "closure output readback");
+}
+
+pw_status pw_solver_wall_forces(pw_solver* s, double* out, size_t n_doubles) {
+  if (pw_status st = check(s)) return st;
+  if (!out || n_doubles < 3 * s->n_wall) {
+    set_error("Wall force output needs 3 doubles per wall cell.");
+    return PW_EINVAL;
+  }
+  if (s->n_wall == 0) return PW_OK;
+  double* dev = nullptr;
+  hipError_t e = hipMallocAsync((void**)&dev, s->n_wall * 3 * sizeof(double), s->stream);
+  if (e != hipSuccess) return device_error(e, "wall force buffer");
+  switch (s->params.storage) {
+    case PW_STORAGE_F16: launch_wall_forces_s<StoreF16>(s, dev); break;
+    case PW_STORAGE_BF16: launch_wall_forces_s<StoreBF16>(s, dev); break;
+    default: launch_wall_forces_s<StoreF32>(s, dev); break;
+  }
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dev, s->n_wall * 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  (void)hipFreeAsync(dev, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  return e == hipSuccess ? PW_OK : device_error(e, "wall forces");
+}
+
+pw_status pw_solver_set_closure(pw_solver* s, const pw_mlp* net, const float* norm, size_t n_norm) {
+  if (pw_status st = check(s)) return st;
+  if (s->params.wall_model != PW_WALL_LEARNED || !s->closure_net) {
+    set_error("The solver was not created with the learned wall model.");
+    return PW_EINVAL;
+  }
+  if (!net || !norm || n_norm != 2 * closure::kFeatures + 3) {
+    set_error("A closure needs a network and 27 normalisation floats (12 means, 12 scales, output "
+              "scale, limit, velocity unit).");
+    return PW_EINVAL;
+  }
+  for (size_t k = 0; k < n_norm; ++k)
+    if (!std::isfinite(norm[k])) {
+      set_error("Closure normalisation must be finite.");
+      return PW_EINVAL;
+    }
+  if (!(norm[2 * closure::kFeatures + 2] > 0.0f) || !(norm[2 * closure::kFeatures + 1] > 0.0f)) {
+    set_error("Closure velocity unit and limit must be positive.");
+    return PW_EINVAL;
+  }
+  hipError_t e = hipMemcpy(s->closure_net, mlp_packed(net), mlp_packed_bytes(), hipMemcpyDeviceToDevice);
+  if (e != hipSuccess) return device_error(e, "closure network copy");
+  std::memcpy(s->closure_norm, norm, n_norm * sizeof(float));
+  s->closure_on = 1;
+  return PW_OK;
+}
+
+}  // extern "C"

@@ -326,6 +326,11 @@ struct pw_mlp {
   uint64_t adam_step = 0;
 };
 
+namespace pw {
+const void* mlp_packed(const pw_mlp* m) { return m ? m->packed : nullptr; }
+size_t mlp_packed_bytes() { return sizeof(Packed); }
+}  // namespace pw
+
 namespace {
 
 void free_mlp(pw_mlp* m) {

@@ -138,10 +138,12 @@ extern template void launch_step_s<StoreBF16>(pw_solver*, const void*, void*, bo
 namespace {
 
 // One step from the current set into the other; the step count seeds the stores' rounding.
-void launch_step(pw_solver* s, bool macro, const StepParams& p) {
+void launch_step(pw_solver* s, bool macro, const StepParams& params) {
   const void* in = s->pops[s->current];
   void* out = s->pops[s->current ^ 1];
   const uint32_t seed = (uint32_t)s->steps;
+  StepParams p = params;
+  launch_closure(s, p);
   switch (s->params.storage) {
     case PW_STORAGE_F16: launch_step_s<StoreF16>(s, in, out, macro, p, seed); break;
     case PW_STORAGE_BF16: launch_step_s<StoreBF16>(s, in, out, macro, p, seed); break;
@@ -222,6 +224,10 @@ void free_all(pw_solver* s) {
   for (void* p : s->pops) if (p) hipFree(p);
   if (s->geometry) hipFree(s->geometry);
   if (s->boundary) hipFree(s->boundary);
+  if (s->wall_cells) hipFree(s->wall_cells);
+  if (s->closure_slot) hipFree(s->closure_slot);
+  if (s->closure_tau) hipFree(s->closure_tau);
+  if (s->closure_net) hipFree(s->closure_net);
   if (s->macro) hipFree(s->macro);
   if (s->partials) hipFree(s->partials);
   if (s->force_dev) hipFree(s->force_dev);
@@ -268,8 +274,8 @@ pw_status validate_tunnel(const pw_layout& L, const uint32_t* geometry, size_t g
     set_error("Unknown operator, wall, storage or wall-model option.");
     return PW_EINVAL;
   }
-  if (params.wall_model == PW_WALL_LEARNED) {
-    set_error("The learned wall closure is not available in this build.");
+  if (params.wall_model == PW_WALL_LEARNED && !params.les) {
+    set_error("The learned wall closure replaces the LES wall model and needs LES.");
     return PW_EINVAL;
   }
   if (!pack_params(L, params, params.u_in, full)) return PW_EINVAL;
@@ -338,6 +344,11 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   s->n_boundary = n_boundary;
   s->first_fluid = first_fluid;
   s->n_partials = (n_boundary + kForceBlock - 1) / kForceBlock;
+  std::vector<uint32_t> wall_list;
+  for (size_t cell = 0; cell < N; ++cell)
+    if ((geometry[cell] & kTypeMask) == kFluid && (geometry[cell] >> 3) != 0) wall_list.push_back((uint32_t)cell);
+  s->n_wall = wall_list.size();
+  const bool learned = params->wall_model == PW_WALL_LEARNED;
   hipGetDevice(&s->device);
   auto fail = [&](pw_status st) {
     free_all(s);
@@ -352,6 +363,17 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   if ((e = hipMalloc((void**)&s->geometry, geometry_words * 4)) != hipSuccess) return fail(device_error(e, "geometry allocation"));
   if ((e = hipMalloc((void**)&s->macro, N * sizeof(float4))) != hipSuccess) return fail(device_error(e, "macro allocation"));
   if ((e = hipMalloc((void**)&s->boundary, (n_boundary ? n_boundary : 1) * 4)) != hipSuccess) return fail(device_error(e, "boundary allocation"));
+  if ((e = hipMalloc((void**)&s->wall_cells, (s->n_wall ? s->n_wall : 1) * 4)) != hipSuccess) return fail(device_error(e, "wall cell allocation"));
+  if (learned) {
+    std::vector<uint32_t> slot(N, 0xffffffffu);
+    for (size_t k = 0; k < wall_list.size(); ++k) slot[wall_list[k]] = (uint32_t)k;
+    if ((e = hipMalloc((void**)&s->closure_slot, N * 4)) != hipSuccess ||
+        (e = hipMalloc((void**)&s->closure_tau, (s->n_wall ? s->n_wall : 1) * sizeof(float4))) != hipSuccess ||
+        (e = hipMalloc(&s->closure_net, mlp_packed_bytes())) != hipSuccess)
+      return fail(device_error(e, "closure allocation"));
+    if ((e = hipMemcpy(s->closure_slot, slot.data(), N * 4, hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(device_error(e, "closure upload"));
+  }
   if ((e = hipMalloc((void**)&s->partials, (s->n_partials ? s->n_partials : 1) * 3 * sizeof(double))) != hipSuccess) return fail(device_error(e, "force partial allocation"));
   if ((e = hipMalloc((void**)&s->force_dev, 4 * sizeof(double))) != hipSuccess) return fail(device_error(e, "force allocation"));
   if ((e = hipHostMalloc((void**)&s->force_host, 4 * sizeof(double), hipHostMallocDefault)) != hipSuccess) return fail(device_error(e, "pinned force slot"));
@@ -359,6 +381,7 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   if ((e = hipEventCreate(&s->t0)) != hipSuccess || (e = hipEventCreate(&s->t1)) != hipSuccess) return fail(device_error(e, "event"));
   if ((e = hipMemcpyAsync(s->geometry, geometry, geometry_words * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "geometry upload"));
   if (n_boundary && (e = hipMemcpyAsync(s->boundary, boundary_cells, n_boundary * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "boundary upload"));
+  if (s->n_wall && (e = hipMemcpyAsync(s->wall_cells, wall_list.data(), s->n_wall * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "wall cell upload"));
   launch_init_both(s, full);
   const char* forced = std::getenv("PW_STEP_KERNEL");
   if (forced && std::strcmp(forced, "cell") == 0) s->kernel = PW_KERNEL_CELL;

@@ -45,6 +45,10 @@ struct StepParams {
   float inv_tau0;  // 1 / tau0 in f32: the relaxation rate without LES
   float wm0;       // TRT odd rate 1 / (lambda / (tau0 - 1/2) + 1/2) in f32
   FastDiv div_nx, div_ny;
+  // Learned wall closure (null: the wall law): shear stress per wall cell (xyz, w > 0 when
+  // valid) and the wall-cell index of every cell.
+  const float4* closure_tau;
+  const uint32_t* closure_slot;
 };
 
 void set_error(const std::string& message);
@@ -56,6 +60,18 @@ pw_status validate_tunnel(const pw_layout& layout, const uint32_t* geometry, siz
                           const uint32_t* boundary_cells, size_t n_boundary, const pw_params& params,
                           StepParams* full);
 
+}  // namespace pw
+
+struct pw_solver;
+struct pw_mlp;
+
+namespace pw {
+// Learned wall closure (closure.hip): the network's device copy for the matrix cores, and
+// the per-step closure launch, which reads the state the next step reads and points
+// p.closure_tau / p.closure_slot at its output when the solver has a network attached.
+const void* mlp_packed(const pw_mlp* m);
+size_t mlp_packed_bytes();
+void launch_closure(pw_solver* s, StepParams& p);
 }  // namespace pw
 
 struct pw_solver {
@@ -85,6 +101,13 @@ struct pw_solver {
   int ramp_settled = 0;
   long long first_fluid = -1;
   int destroyed = 0;
+  uint32_t* wall_cells = nullptr;  // fluid cells with wall links, in cell order
+  size_t n_wall = 0;
+  uint32_t* closure_slot = nullptr;  // wall-cell index of every cell (learned model only)
+  float4* closure_tau = nullptr;     // closure output per wall cell
+  void* closure_net = nullptr;       // device copy of the network's matrix-core parameters
+  float closure_norm[32] = {};       // closure::Norm
+  int closure_on = 0;
   uint32_t kernel = 1;  // pw_kernel actually used by step()
   uint32_t block = 256;  // threads per step block
 };

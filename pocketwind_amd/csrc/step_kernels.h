@@ -187,7 +187,7 @@ __host__ __device__ __forceinline__ float friction_velocity(float nu, float spee
 // flux along the slip is the wall-law shear; falls back to the bulk collision.
 template <int OP>
 __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float* g, const CellState& c,
-                                                     uint32_t flag) {
+                                                     uint32_t flag, uint32_t n) {
   float nx = 0, ny = 0, nz = 0, exposed = 0;
 #pragma unroll
   for (int i = 1; i < kQ; ++i)
@@ -213,8 +213,18 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
     return;
   }
   const float tx = sx / speed, ty = sy / speed, tz = sz / speed;
-  const float ut = friction_velocity(p.nu, speed);
-  const float shear = 6.0f * exposed * c.rho * ut * ut;
+  // The wall shear stress along the slip: the learned closure's where it gave a valid value
+  // for this cell, else the wall law's rho u_tau^2.
+  float stress = -1.0f;
+  if (p.closure_tau) {
+    const float4 tau = p.closure_tau[p.closure_slot[n]];
+    if (tau.w > 0.0f) stress = fmaxf(tau.x * tx + tau.y * ty + tau.z * tz, 0.0f);
+  }
+  if (stress < 0.0f) {
+    const float ut = friction_velocity(p.nu, speed);
+    stress = c.rho * ut * ut;
+  }
+  const float shear = 6.0f * exposed * stress;
   float flux = 0;
   Stress b{0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -374,11 +384,11 @@ __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ o
 // Moments, equilibrium and collision of one gathered fluid cell (stream_collide.wgsl:114-133);
 // g becomes the post-collision state; returns (rho, u).
 template <int OP, bool LES>
-__host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, uint32_t flag) {
+__host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, uint32_t flag, uint32_t n) {
   CellState c;
   cell_state(g, c);
   if (LES && (flag >> 3) != 0)
-    collide_wall<OP>(p, g, c, flag);
+    collide_wall<OP>(p, g, c, flag, n);
   else
     collide<OP, LES>(p, g, c);
   return make_float4(c.rho, c.ux, c.uy, c.uz);
@@ -452,7 +462,7 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
         g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i)
                                  : S::load(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
     }
-    m = relax<OP, LES>(p, g, flag);
+    m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
     m = eq_state<S>(in, p, n, x, g);
   } else {
@@ -593,7 +603,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
       for (int i = 1; i < kQ; ++i)
         if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i);
     }
-    m = relax<OP, LES>(p, g, flag);
+    m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
     m = eq_state<S>(in, p, n, x, g);
   } else {

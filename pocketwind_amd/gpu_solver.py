@@ -201,6 +201,36 @@ class GpuSolver:
     def block_size(self) -> int:
         return int(N.lib().pw_solver_block_size(self._live()))
 
+    # Learned wall closure (no reference counterpart; see include/pw.h).
+    def wall_cells(self) -> np.ndarray:
+        n = int(N.lib().pw_solver_wall_cell_count(self._live()))
+        out = np.empty(n, np.uint32)
+        N.check(N.lib().pw_solver_wall_cells(self._live(), out.ctypes.data, n))
+        return out
+
+    def closure_features(self, u_ref: float) -> np.ndarray:
+        """Per wall cell: 12 closure features, then the wall law's stress vector (15 floats)."""
+        n = int(N.lib().pw_solver_wall_cell_count(self._live()))
+        out = np.empty((n, 15), np.float32)
+        N.check(N.lib().pw_solver_closure_features(self._live(), C.c_float(u_ref), out.ctypes.data, out.size))
+        return out
+
+    def wall_forces(self) -> np.ndarray:
+        n = int(N.lib().pw_solver_wall_cell_count(self._live()))
+        out = np.empty((n, 3), np.float64)
+        N.check(N.lib().pw_solver_wall_forces(self._live(), out.ctypes.data, out.size))
+        return out
+
+    def set_closure(self, net, norm) -> None:
+        norm = np.ascontiguousarray(norm, np.float32)
+        N.check(N.lib().pw_solver_set_closure(self._live(), net._h, norm.ctypes.data, norm.size))
+
+    def closure_output(self) -> np.ndarray:
+        n = int(N.lib().pw_solver_wall_cell_count(self._live()))
+        out = np.empty((n, 4), np.float32)
+        N.check(N.lib().pw_solver_closure_output(self._live(), out.ctypes.data, out.size))
+        return out
+
     def synchronize(self) -> None:
         N.check(N.lib().pw_solver_synchronize(self._live()))
 
