@@ -17,9 +17,16 @@ Algorithmic bytes per cell update follow the reference's BYTES_PER_CELL_STEP
 (src/app/bench-page.ts:44-45): 19 populations read + 19 written + one 4-byte flag, 156 B at
 f32 and 80 B at f16/bf16 storage, over every cell of the lattice.
 
+Headline: f16 deviation storage with the learned wall closure (the network shipped in
+pocketwind_amd/closure_c2.npz, trained by `python -m pocketwind_amd.closure` on coarse/fine
+pairs of other bodies at this resolution), against plain f32 storage with the wall law. Its
+step is two launches (closure kernel over the wall cells, then the step kernel); the roofline
+is the step kernel's, timed in the f16 run without the closure.
+
 Cd: six flow-throughs (6 ceil(nx / u_in) steps) with an FP64 force sample every 8 steps,
 mean over the last two (src/validation/cases.ts:294-320), Cd = fx / (1/2 u^2 A) with A the
-voxel frontal area; reported for each storage precision on the same grid.
+voxel frontal area; reported for each configuration, and for plain f32 with the wall law on
+the 2x finer grid (512x256x256, L = 128, the same Re), the metric's Cd reference.
 
 The CPU baseline is the FP64 C restatement of the reference CPU solver (the oracle) stepping
 the same body grid single-threaded, like the reference's JS path, for a bounded number of
@@ -59,23 +66,38 @@ def empty_grid(layout):
     return VoxelGrid(*layout, geometry)
 
 
+CLOSURE = os.path.join(ROOT, "pocketwind_amd", "closure_c2.npz")
+CLOSURE_EVERY = 8  # steps between closure evaluations (the wall stress is reused in between)
+
+
 def body_grid(layout=LAYOUT, length=BODY_LENGTH):
+    """The body with its wall-link fractions (the closure's q feature; the walls stay halfway
+    bounce-back, which does not read them)."""
     from pocketwind_amd import bodies
     from pocketwind_amd.voxelize import voxelize
     body, _ = bodies.place_mesh(bodies.bluff_body(), None, length, layout, False)
-    return voxelize(body, None, layout, False, link_fractions=False)
+    return voxelize(body, None, layout, False, link_fractions=True)
 
 
-def params_for(precision: str):
+def params_for(precision: str, learned: bool = False, tau0: float = TAU0):
     from pocketwind_amd.gpu_solver import SolverParams
-    return SolverParams(TAU0, U_IN, True, operator="reg", precision=precision)
+    return SolverParams(tau0, U_IN, True, operator="reg", precision=precision,
+                        wall_model="learned" if learned else "law")
 
 
-def time_solver(grid, params, steps, warmup, repeats=3):
+def make_solver(grid, params, closure=None):
     from pocketwind_amd.gpu_solver import GpuSolver
     solver = GpuSolver.create(grid.layout, grid, params)
     if not isinstance(solver, GpuSolver):
         raise MemoryError(solver.message)
+    if closure is not None:
+        solver.set_closure(*closure)
+        solver.set_closure_interval(CLOSURE_EVERY)
+    return solver
+
+
+def time_solver(grid, params, steps, warmup, repeats=3, closure=None):
+    solver = make_solver(grid, params, closure)
     try:
         solver.step(warmup, False)
         solver.synchronize()
@@ -110,10 +132,9 @@ def measured_traffic(precision: str, variant: str):
     return table.get(f"{precision}/{variant.split('/')[0]}")
 
 
-def body_cd(grid, precision: str):
+def body_cd(grid, params, closure=None):
     from pocketwind_amd import drag
-    from pocketwind_amd.gpu_solver import GpuSolver
-    solver = GpuSolver.create(grid.layout, grid, params_for(precision))
+    solver = make_solver(grid, params, closure)
     try:
         flow = drag.flow_through_steps(grid.nx, U_IN)
         samples = solver.step_sampled(6 * flow, 8)
@@ -182,6 +203,11 @@ def main():
         print(f"# copy probe: {copy_bandwidth(512 << 20, 4):.0f} GB/s (512 MiB, non-temporal float4)",
               file=sys.stderr, flush=True)
         return 0
+    from pocketwind_amd import closure as closure_mod
+    net, norm, meta = closure_mod.load(CLOSURE)
+    learned = time_solver(grid, params_for("f16", learned=True), args.steps, args.warmup, closure=(net, norm))
+    print(f"# body f16 + learned closure: {learned['mlups']:.1f} MLUPS, {learned['launch_us']:.2f} us/step "
+          f"(closure + step kernel), {learned['variant']}, finite={learned['finite']}", file=sys.stderr, flush=True)
     empty = {}
     for precision in ("f16", "f32"):
         r = time_solver(empty_grid(LAYOUT), params_for(precision), args.steps, args.warmup)
@@ -189,46 +215,57 @@ def main():
         print(f"# empty {precision}: {r['mlups']:.1f} MLUPS, {r['launch_us']:.2f} us/launch, "
               f"{r['gbps']:.0f} GB/s, {r['variant']}", file=sys.stderr, flush=True)
     copy_gbps = copy_bandwidth(512 << 20, 40)
-    head = body["f16"]
+    step16 = body["f16"]
     line = {
         "metric": METRIC,
-        "value": round(head["mlups"], 1),
+        "value": round(learned["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(head["launch_us"] / 1e3, 5),
+        "ms_per_step": round(learned["launch_us"] / 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 arithmetic, f16 storage",
         "data": "synthetic",
         "config": {"workload": "bluff body L=64 (1.0x0.37x0.28 L, rounded nose, 25 deg slant) in a "
-                               "256x128x128 D3Q19 tunnel, REG + Smagorinsky Cs 0.1 + wall law, Re 320 "
-                               "(tau0 0.53), u_in 0.05, device-voxelised",
-                   "precision": "f16 deviation storage", "wall_closure": "wall law",
+                               "256x128x128 D3Q19 tunnel, REG + Smagorinsky Cs 0.1, halfway bounce-back, "
+                               "Re 320 (tau0 0.53), u_in 0.05, device-voxelised",
+                   "precision": "f16 deviation storage", "wall_closure": f"learned (bf16 MFMA MLP, evaluated every {CLOSURE_EVERY} steps), wall-law fallback",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(head["gbps"], 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(head["gbps"] / HBM_PEAK_GBPS, 4),
-                     "traffic": measured_traffic("f16", head["variant"])},
+        "roofline": {"bound": "hbm", "achieved": round(step16["gbps"], 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(step16["gbps"] / HBM_PEAK_GBPS, 4),
+                     "traffic": measured_traffic("f16", step16["variant"]),
+                     "kernel": "stream_collide_kernel, f16 storage", "kernel_us": round(step16["launch_us"], 2)},
         "f32": {"value": round(body["f32"]["mlups"], 1), "ms_per_step": round(body["f32"]["launch_us"] / 1e3, 5),
                 "achieved_gbps": round(body["f32"]["gbps"], 1),
-                "frac": round(body["f32"]["gbps"] / HBM_PEAK_GBPS, 4)},
-        "bf16": {"value": round(body["bf16"]["mlups"], 1), "achieved_gbps": round(body["bf16"]["gbps"], 1)},
-        "speedup_f16_vs_f32": round(head["mlups"] / body["f32"]["mlups"], 3),
+                "frac": round(body["f32"]["gbps"] / HBM_PEAK_GBPS, 4), "wall_closure": "wall law"},
+        "f16_wall_law": {"value": round(step16["mlups"], 1), "achieved_gbps": round(step16["gbps"], 1)},
+        "bf16_wall_law": {"value": round(body["bf16"]["mlups"], 1), "achieved_gbps": round(body["bf16"]["gbps"], 1)},
+        "speedup_vs_f32": round(learned["mlups"] / body["f32"]["mlups"], 3),
         "empty_grid": {p: {"value": round(r["mlups"], 1), "achieved_gbps": round(r["gbps"], 1)}
                        for p, r in empty.items()},
         "variants": {p: r["variant"] for p, r in body.items()},
         "copy_gbps": round(copy_gbps, 1),
     }
     if not args.no_cd:
-        cds = {}
-        for precision in ("f16", "f32", "bf16"):
-            cds[precision] = body_cd(grid, precision)
-            print(f"# Cd {precision}: {cds[precision]:.6f}", file=sys.stderr, flush=True)
-        line["cd"] = {p: round(v, 6) for p, v in cds.items()}
-        line["cd"]["f16_vs_f32_pct"] = round(100 * (cds["f16"] / cds["f32"] - 1), 4)
-        line["cd"]["bf16_vs_f32_pct"] = round(100 * (cds["bf16"] / cds["f32"] - 1), 4)
+        cds = {"f16_learned": body_cd(grid, params_for("f16", learned=True), closure=(net, norm)),
+               "f16_law": body_cd(grid, params_for("f16")),
+               "f32_law": body_cd(grid, params_for("f32")),
+               "bf16_law": body_cd(grid, params_for("bf16"))}
+        for k, v in cds.items():
+            print(f"# Cd {k}: {v:.6f}", file=sys.stderr, flush=True)
+        fine_layout = tuple(2 * v for v in LAYOUT)
+        fine = body_grid(fine_layout, 2 * BODY_LENGTH)
+        cds["f32_law_2x_grid"] = body_cd(fine, params_for("f32", tau0=3 * (2 * (TAU0 - 0.5) / 3) + 0.5))
+        print(f"# Cd f32_law_2x_grid: {cds['f32_law_2x_grid']:.6f}", file=sys.stderr, flush=True)
+        ref = cds["f32_law_2x_grid"]
+        line["cd"] = {k: round(v, 6) for k, v in cds.items()}
+        for k in ("f16_learned", "f16_law", "f32_law", "bf16_law"):
+            line["cd"][f"{k}_vs_2x_grid_pct"] = round(100 * (cds[k] / ref - 1), 3)
+        line["cd"]["f16_law_vs_f32_law_pct"] = round(100 * (cds["f16_law"] / cds["f32_law"] - 1), 4)
+    net.destroy()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(grid)
     print(json.dumps(line))

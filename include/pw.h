@@ -273,6 +273,9 @@ pw_status pw_solver_wall_forces(pw_solver *s, double *out, size_t n_doubles);
  * (normalised = (feature - mean) x scale), output scale (stress = output x scale), limit,
  * velocity unit u_ref (27 floats). */
 pw_status pw_solver_set_closure(pw_solver *s, const pw_mlp *net, const float *norm, size_t n_norm);
+/* Evaluate the closure on the steps whose count is a multiple of `every` (default 1) and
+ * reuse its stresses in between (the first step after attaching or reset always evaluates). */
+pw_status pw_solver_set_closure_interval(pw_solver *s, uint32_t every);
 /* The last closure evaluation: 4 floats per wall cell, stress xyz and 1 (used) or 0 (wall
  * law). */
 pw_status pw_solver_closure_output(pw_solver *s, float *out, size_t n_floats);

@@ -98,6 +98,7 @@ SIGNATURES = {
     "pw_solver_wall_forces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "pw_solver_set_closure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     "pw_solver_closure_output": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_solver_set_closure_interval": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pw_mlp_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "pw_mlp_destroy": (None, [C.c_void_p]),
     "pw_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

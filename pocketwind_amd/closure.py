@@ -238,3 +238,72 @@ def train_closure(features, targets, steps: int = 4000, batch: int = 512, lr: fl
 
 def predict(net, norm, features) -> np.ndarray:
     return net.forward(normalise(features, norm)) * norm[24]
+
+
+# --- Trained closure files ------------------------------------------------------------------
+
+def save(path, net, norm, meta=None):
+    import json
+    np.savez(path, params=net.params, norm=np.asarray(norm, np.float32),
+             meta=np.array(json.dumps(meta or {})))
+
+
+def load(path):
+    """(Mlp, norm, meta) from a file written by save()."""
+    import json
+    with np.load(path, allow_pickle=False) as d:
+        net = Mlp(d["params"])
+        return net, d["norm"].astype(np.float32), json.loads(str(d["meta"]))
+
+
+def main(argv=None):
+    """Train the closure on coarse/fine pairs of random bodies and save it.
+
+    python -m pocketwind_amd.closure --out closure.npz --coarse 256 128 128 --length 64 \\
+        --train 6 --held 2 --re 200 500
+    """
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--coarse", type=int, nargs=3, default=[256, 128, 128])
+    ap.add_argument("--length", type=int, default=64)
+    ap.add_argument("--train", type=int, default=6)
+    ap.add_argument("--held", type=int, default=2)
+    ap.add_argument("--re", type=float, nargs=2, default=[200.0, 500.0])
+    ap.add_argument("--flow-throughs", type=float, default=4.0)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--report", default=None)
+    args = ap.parse_args(argv)
+    t0 = time.time()
+    pairs = []
+    for k, seed in enumerate(list(range(args.train)) + [1000 + j for j in range(args.held)]):
+        p = coarse_fine_pair(seed, tuple(args.coarse), args.length, tuple(args.re), flow_throughs=args.flow_throughs)
+        pairs.append(p)
+        print(f"# pair {k} (seed {seed}, Re {p['re']:.0f}): {len(p['target'])} wall cells, "
+              f"{time.time() - t0:.0f} s", flush=True)
+    train, held = pairs[:args.train], pairs[args.train:]
+    x = np.concatenate([p["features"] for p in train])
+    t = np.concatenate([p["target"] for p in train])
+    net, norm, history = train_closure(x, t, steps=args.steps, batch=512, lr=2e-3, seed=1)
+    rms = lambda a: float(np.sqrt(np.mean(np.sum(a ** 2, axis=1))))
+    report = {"train_samples": int(len(t)), "final_loss": float(history[-100:].mean()),
+              "coarse": args.coarse, "length": args.length, "re": args.re}
+    if held:
+        xh = np.concatenate([p["features"] for p in held])
+        th = np.concatenate([p["target"] for p in held])
+        law = np.concatenate([p["law"] for p in held])
+        report.update(held_samples=int(len(th)), held_rms_target=rms(th),
+                      held_rms_error_network=rms(predict(net, norm, xh) - th), held_rms_error_wall_law=rms(law - th))
+    save(args.out, net, norm, report)
+    print(json.dumps(report))
+    if args.report:
+        json.dump(report, open(args.report, "w"), indent=1)
+    net.destroy()
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

@@ -59,7 +59,10 @@ pw_status check(pw_solver* s) {
 
 void launch_closure(pw_solver* s, StepParams& p) {
   if (!s->closure_on || s->n_wall == 0) return;
-  launch_closure_any(s, s->pops[s->current], s->closure_net, s->closure_tau, nullptr);
+  if (!s->closure_fresh || s->steps % s->closure_every == 0) {
+    launch_closure_any(s, s->pops[s->current], s->closure_net, s->closure_tau, nullptr);
+    s->closure_fresh = 1;
+  }
   p.closure_tau = s->closure_tau;
   p.closure_slot = s->closure_slot;
 }
@@ -163,6 +166,17 @@ pw_status pw_solver_set_closure(pw_solver* s, const pw_mlp* net, const float* no
   if (e != hipSuccess) return device_error(e, "closure network copy");
   std::memcpy(s->closure_norm, norm, n_norm * sizeof(float));
   s->closure_on = 1;
+  s->closure_fresh = 0;
+  return PW_OK;
+}
+
+pw_status pw_solver_set_closure_interval(pw_solver* s, uint32_t every) {
+  if (pw_status st = check(s)) return st;
+  if (every == 0) {
+    set_error("The closure interval must be at least one step.");
+    return PW_EINVAL;
+  }
+  s->closure_every = every;
   return PW_OK;
 }
 

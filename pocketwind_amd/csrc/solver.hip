@@ -448,6 +448,7 @@ pw_status pw_solver_reset(pw_solver* s) {
   PW_TRY(hipGetLastError(), "reset launch");
   s->current = 0;
   s->steps = 0;
+  s->closure_fresh = 0;
   s->ramp_settled = 0;
   s->force_encoded = 0;  // gpu-force.ts:208-210: an unread sample is discarded
   return PW_OK;

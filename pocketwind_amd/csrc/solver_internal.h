@@ -108,6 +108,8 @@ struct pw_solver {
   void* closure_net = nullptr;       // device copy of the network's matrix-core parameters
   float closure_norm[32] = {};       // closure::Norm
   int closure_on = 0;
+  uint32_t closure_every = 1;  // steps between closure evaluations
+  int closure_fresh = 0;       // closure_tau holds an evaluation of this run
   uint32_t kernel = 1;  // pw_kernel actually used by step()
   uint32_t block = 256;  // threads per step block
 };

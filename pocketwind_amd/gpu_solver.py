@@ -225,6 +225,9 @@ class GpuSolver:
         norm = np.ascontiguousarray(norm, np.float32)
         N.check(N.lib().pw_solver_set_closure(self._live(), net._h, norm.ctypes.data, norm.size))
 
+    def set_closure_interval(self, every: int) -> None:
+        N.check(N.lib().pw_solver_set_closure_interval(self._live(), int(every)))
+
     def closure_output(self) -> np.ndarray:
         n = int(N.lib().pw_solver_wall_cell_count(self._live()))
         out = np.empty((n, 4), np.float32)
