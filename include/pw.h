@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-#define PW_ABI_VERSION 2
+#define PW_ABI_VERSION 3
 
 /* Errors (src/lbm/gpu-resources.ts:3-27, gpu-force.ts:192-205). EINVAL is the reference's
  * RangeError, ENOMEM its AllocationFailure, EBUSY its 'busy' readback. */
@@ -129,6 +129,7 @@ typedef struct {
   const float *macro;   /* float4 per cell */
   const uint32_t *geometry;
   void *stream;         /* hipStream_t of the handle */
+  size_t plane_elements; /* population plane i starts at element i x plane_elements */
 } pw_device_state;
 pw_status pw_solver_device_state(pw_solver *s, pw_device_state *out);
 

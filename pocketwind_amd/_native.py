@@ -31,7 +31,7 @@ class ForceSample(C.Structure):
 
 class DeviceState(C.Structure):
     _fields_ = [("populations", C.c_void_p), ("macro", C.c_void_p), ("geometry", C.c_void_p),
-                ("stream", C.c_void_p)]
+                ("stream", C.c_void_p), ("plane_elements", C.c_size_t)]
 
 
 _u32p = C.POINTER(C.c_uint32)

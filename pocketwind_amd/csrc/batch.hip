@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(const Tunnel* __rest
     if (n >= tu.p.cells) continue;
     const uint32_t kind = tu.geo[n] & kTypeMask;
     const float ux = is_wall(kind) ? 0.0f : tu.p.u_in;
-    store_equilibrium<S>((typename S::T*)tu.pops[set], tu.p.cells, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
+    store_equilibrium<S>((typename S::T*)tu.pops[set], tu.p.plane, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
     tu.macro[n] = make_float4(1.0f, ux, 0.0f, 0.0f);
   }
 }
@@ -260,7 +260,7 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
     const size_t N = sps[t].cells;
     for (int k = 0; k < 2; ++k) {
       void* p = nullptr;
-      if ((e = hipMalloc(&p, (size_t)kQ * N * elem)) != hipSuccess) return fail(e, "batch populations");
+      if ((e = hipMalloc(&p, (size_t)kQ * sps[t].plane * elem)) != hipSuccess) return fail(e, "batch populations");
       b->allocations.push_back(p);
       tu.pops[k] = p;
     }
@@ -400,8 +400,10 @@ pw_status pw_batch_read_populations(pw_batch* b, uint32_t tunnel, void* out, siz
     set_error("Invalid tunnel index or output size.");
     return PW_EINVAL;
   }
-  hipError_t e = hipMemcpyAsync(out, b->host[tunnel].pops[b->parity], (size_t)kQ * b->host[tunnel].p.cells * elem,
-                                hipMemcpyDeviceToHost, b->stream);
+  const StepParams& p = b->host[tunnel].p;
+  const size_t row = (size_t)p.cells * elem, pitch = (size_t)p.plane * elem;
+  hipError_t e = hipMemcpy2DAsync(out, row, b->host[tunnel].pops[b->parity], pitch, row, kQ, hipMemcpyDeviceToHost,
+                                  b->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   return e == hipSuccess ? PW_OK : device_error(e, "batch population readback");
 }

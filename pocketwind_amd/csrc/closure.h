@@ -37,7 +37,7 @@ struct Norm {
 template <class S, bool BOUZIDI>
 __device__ __forceinline__ void gather_cell(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
                                             const StepParams& p, uint32_t n, float* g) {
-  const size_t N = p.cells;
+  const size_t P = p.plane;
   const uint32_t flag = geo[n];
   uint32_t x, y, z;
   cell_coords(n, p, x, y, z);
@@ -46,7 +46,7 @@ __device__ __forceinline__ void gather_cell(const typename S::T* __restrict__ in
 #pragma unroll
   for (int i = 1; i < kQ; ++i)
     g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i)
-                             : S::load(in + (size_t)i * N + nb.index(-cx(i), -cy(i), -cz(i)));
+                             : S::load(in + (size_t)i * P + nb.index(-cx(i), -cy(i), -cz(i)));
 }
 
 __device__ __forceinline__ void velocity_of(const float* g, float& rho, float& ux, float& uy, float& uz) {

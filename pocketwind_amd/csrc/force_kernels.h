@@ -24,14 +24,14 @@ __device__ __forceinline__ void cell_exchange(const typename S::T* __restrict__ 
                                               const uint32_t* __restrict__ geo, const StepParams& p,
                                               const Neighbours<>& nb, uint32_t n, uint32_t flag,
                                               double& fx, double& fy, double& fz) {
-  const size_t N = p.cells;
+  const size_t P = p.plane;
 #pragma unroll
   for (int i = 1; i < kQ; ++i) {
     if (!has_wall(flag, i)) continue;
     const uint32_t source = nb.index(-cx(i), -cy(i), -cz(i));
     if ((geo[source] & kTypeMask) != kSolid) continue;
     const int j = opp(i);
-    const double outgoing = S::load(state + (size_t)j * N + n);
+    const double outgoing = S::load(state + (size_t)j * P + n);
     const double incoming = wall_value<S, BOUZIDI, double>(state, geo, p, nb, n, flag, i);
     const double e = outgoing + incoming;
     fx += cx(j) * e;

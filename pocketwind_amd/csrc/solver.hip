@@ -109,6 +109,7 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
   out->lambda = 3.0f / 16.0f;
   out->inv_tau0 = 1.0f / out->tau0;
   out->wm0 = 1.0f / (out->lambda / (out->tau0 - 0.5f) + 0.5f);
+  out->plane = plane_elements((uint32_t)cells, prm.storage == PW_STORAGE_F32 ? 4u : 2u);
   out->div_nx = make_fast_div(L.nx);
   out->div_ny = make_fast_div(L.ny);
   return true;
@@ -357,7 +358,7 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   };
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) return fail(device_error(e, "hipStreamCreate"));
-  const size_t set_bytes = (size_t)kQ * N * s->elem_bytes;
+  const size_t set_bytes = (size_t)kQ * full.plane * s->elem_bytes;
   for (int k = 0; k < 2; ++k)
     if ((e = hipMalloc(&s->pops[k], set_bytes)) != hipSuccess) return fail(device_error(e, "population allocation"));
   if ((e = hipMalloc((void**)&s->geometry, geometry_words * 4)) != hipSuccess) return fail(device_error(e, "geometry allocation"));
@@ -483,7 +484,10 @@ pw_status pw_solver_read_populations(pw_solver* s, void* out, size_t out_bytes) 
     set_error("Population output is smaller than one state.");
     return PW_EINVAL;
   }
-  PW_TRY(hipMemcpyAsync(out, s->pops[s->current], bytes, hipMemcpyDeviceToHost, s->stream), "population readback");
+  // Planes are padded on the device; the caller gets them back to back (the reference layout).
+  const size_t row = s->cells * s->elem_bytes, pitch = (size_t)s->sp.plane * s->elem_bytes;
+  PW_TRY(hipMemcpy2DAsync(out, row, s->pops[s->current], pitch, row, kQ, hipMemcpyDeviceToHost, s->stream),
+         "population readback");
   PW_TRY(hipStreamSynchronize(s->stream), "population readback");
   return PW_OK;
 }
@@ -495,7 +499,9 @@ pw_status pw_solver_write_populations(pw_solver* s, const void* in, size_t in_by
     set_error("Population input must hold exactly one state.");
     return PW_EINVAL;
   }
-  PW_TRY(hipMemcpyAsync(s->pops[s->current], in, bytes, hipMemcpyHostToDevice, s->stream), "population upload");
+  const size_t row = s->cells * s->elem_bytes, pitch = (size_t)s->sp.plane * s->elem_bytes;
+  PW_TRY(hipMemcpy2DAsync(s->pops[s->current], pitch, in, row, row, kQ, hipMemcpyHostToDevice, s->stream),
+         "population upload");
   PW_TRY(hipStreamSynchronize(s->stream), "population upload");
   return PW_OK;
 }
@@ -517,6 +523,7 @@ pw_status pw_debug_inject_nan(pw_solver* s) {
 pw_status pw_solver_device_state(pw_solver* s, pw_device_state* out) {
   if (pw_status st = check_handle(s)) return st;
   out->populations = s->pops[s->current];
+  out->plane_elements = s->sp.plane;
   out->macro = (const float*)s->macro;
   out->geometry = s->geometry;
   out->stream = (void*)s->stream;
@@ -528,7 +535,7 @@ pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
   if (kernel == PW_KERNEL_AUTO) {
     PW_TRY(hipStreamSynchronize(s->stream), "tune");
     // Tuning steps advance the state: keep a copy of the latest set and restore it.
-    const size_t bytes = pw_solver_population_bytes(s);
+    const size_t bytes = (size_t)kQ * s->sp.plane * s->elem_bytes;
     void* keep = nullptr;
     PW_TRY(hipMalloc(&keep, bytes), "tune scratch");
     const int cur = s->current;

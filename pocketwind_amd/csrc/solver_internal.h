@@ -23,6 +23,14 @@ __host__ __device__ __forceinline__ uint32_t fast_div(uint32_t n, const FastDiv&
   return (t + ((n - t) >> 1)) >> f.s;
 }
 
+// Padding between population planes. With 38 streams in flight per step, plane strides that
+// are large powers of two put every stream on the same HBM channels at once; these pads (the
+// fastest of a sweep of a 19-plane stream copy on MI355X) break the alignment.
+inline uint32_t plane_elements(uint32_t cells, uint32_t elem_bytes) {
+  const uint32_t pad_bytes = elem_bytes == 4 ? 4352u : 1024u;
+  return cells + pad_bytes / elem_bytes;
+}
+
 inline FastDiv make_fast_div(uint32_t d) {
   if (d <= 1) return FastDiv{1, 0, 0};
   uint32_t l = 0;
@@ -37,6 +45,7 @@ inline FastDiv make_fast_div(uint32_t d) {
 struct StepParams {
   uint32_t nx, ny, nz;
   uint32_t cells;
+  uint32_t plane;  // elements from one population plane to the next (cells plus padding)
   float u_in;      // inlet speed this step (ramped)
   float nu;        // f32 viscosity, (tau0 - 1/2) / 3 rounded as the reference packs it
   float tau0;      // 3 nu + 1/2 evaluated in f32, as relaxation_time does

@@ -318,9 +318,9 @@ template <class S, bool BOUZIDI, class R = float, int SHIFT>
 __host__ __device__ __forceinline__ R wall_value(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
                                                  const StepParams& p, const Neighbours<SHIFT>& nb, uint32_t n,
                                                  uint32_t flag, int i) {
-  const size_t N = p.cells;
+  const size_t N = p.cells, P = p.plane;
   const int j = opp(i);
-  const R reflected = S::load(in + (size_t)j * N + n);
+  const R reflected = S::load(in + (size_t)j * P + n);
   if (!BOUZIDI) return reflected;
   const uint32_t base = geo[N + n];
   if (base == kNoQ) return reflected;
@@ -330,19 +330,19 @@ __host__ __device__ __forceinline__ R wall_value(const typename S::T* __restrict
   if (q < R(0.5)) {
     const uint32_t second = nb.index(cx(i), cy(i), cz(i));
     if ((geo[second] & kTypeMask) != kFluid) return reflected;
-    return R(2) * q * reflected + (R(1) - R(2) * q) * R(S::load(in + (size_t)j * N + second));
+    return R(2) * q * reflected + (R(1) - R(2) * q) * R(S::load(in + (size_t)j * P + second));
   }
-  return (reflected + (R(2) * q - R(1)) * R(S::load(in + (size_t)i * N + n))) / (R(2) * q);
+  return (reflected + (R(2) * q - R(1)) * R(S::load(in + (size_t)i * P + n))) / (R(2) * q);
 }
 
-// Stores of one cell's 19 values; `self` is the cell's byte offset in a plane and h its
-// random word for this step.
+// Stores of one cell's 19 values; P is the plane stride, `self` the cell's byte offset in a
+// plane and h its random word for this step.
 template <class S, bool NT = false>
-__host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__ out, size_t N, uint32_t self,
+__host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__ out, size_t P, uint32_t self,
                                                      const float* g, uint32_t h) {
 #pragma unroll
   for (int i = 0; i < kQ; ++i)
-    st<NT>(byte_offset(out + (size_t)i * N, self), S::encode(g[i], direction_random(h, i)));
+    st<NT>(byte_offset(out + (size_t)i * P, self), S::encode(g[i], direction_random(h, i)));
 }
 
 // common.wgsl:145-150 with rho - 1 = 0, in pair form: eq_i = w (4.5 cu^2 - 1.5 u^2) + 3 w cu.
@@ -361,11 +361,11 @@ __host__ __device__ __forceinline__ void equilibrium_values(float ux, float uy, 
 }
 
 template <class S, bool NT = false>
-__host__ __device__ __forceinline__ void store_equilibrium(typename S::T* __restrict__ out, size_t N, uint32_t n,
+__host__ __device__ __forceinline__ void store_equilibrium(typename S::T* __restrict__ out, size_t P, uint32_t n,
                                                            float ux, float uy, float uz, uint32_t h) {
   float g[kQ];
   equilibrium_values(ux, uy, uz, g);
-  store_state<S, NT>(out, N, n << S::kShift, g, h);
+  store_state<S, NT>(out, P, n << S::kShift, g, h);
 }
 
 // init.wgsl:1-8: walls at rest, every other cell at the inflow equilibrium.
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ o
   if (n >= p.cells) return;
   const uint32_t kind = geo[n] & kTypeMask;
   const float ux = is_wall(kind) ? 0.0f : p.u_in;
-  store_equilibrium<S>(out, p.cells, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
+  store_equilibrium<S>(out, p.plane, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
   macro[n] = make_float4(1.0f, ux, 0.0f, 0.0f);
 }
 
@@ -402,11 +402,11 @@ __host__ __device__ __forceinline__ float4 eq_state(const typename S::T* __restr
                                                     uint32_t n, uint32_t x, float* g) {
   float ux = p.u_in, uy = 0.0f, uz = 0.0f;
   if (x == p.nx - 1 && x > 0) {
-    const size_t N = p.cells;
+    const size_t P = p.plane;
     float mass = 1.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
-      const float v = S::load(in + (size_t)i * N + (n - 1));
+      const float v = S::load(in + (size_t)i * P + (n - 1));
       mass += v;
       mx += cx(i) * v;
       my += cy(i) * v;
@@ -441,7 +441,7 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
                                                              const uint32_t* __restrict__ geo,
                                                              float4* __restrict__ macro,
                                                              const StepParams& p, uint32_t n, uint32_t seed) {
-  const size_t N = p.cells;
+  const size_t P = p.plane;
   const uint32_t flag = geo[n];
   const uint32_t kind = flag & kTypeMask;
   uint32_t x, y, z;
@@ -455,12 +455,12 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
     if ((flag >> 3) == 0) {
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
-        g[i] = load_pop<S, POL == kNtAll>(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
+        g[i] = load_pop<S, POL == kNtAll>(byte_offset(in + (size_t)i * P, nb.at(-cx(i), -cy(i), -cz(i))));
     } else {
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
         g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i)
-                                 : S::load(byte_offset(in + (size_t)i * N, nb.at(-cx(i), -cy(i), -cz(i))));
+                                 : S::load(byte_offset(in + (size_t)i * P, nb.at(-cx(i), -cy(i), -cz(i))));
     }
     m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
@@ -469,7 +469,7 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
     rest_state(g);
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
-  store_state<S, POL != kCached>(out, N, self, g, cell_random(n, seed));
+  store_state<S, POL != kCached>(out, P, self, g, cell_random(n, seed));
   if (MACRO && !is_wall(kind)) macro[n] = m;
 }
 
@@ -504,7 +504,7 @@ template <class S, bool NT>
 __device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in, const StepParams& p,
                                            const RowWave& w, uint32_t lane, float* g) {
   constexpr int SH = S::kShift;
-  const size_t N = p.cells;
+  const size_t P = p.plane;
   const uint32_t row = p.nx, plane = p.nx * p.ny;
   const uint32_t y = w.y, z = w.z, x0 = w.x0;
   const uint32_t yo[3] = {((y == 0 ? p.ny - 1 : y - 1) * row) << SH, (y * row) << SH,
@@ -516,7 +516,7 @@ __device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in,
   const uint32_t xs = lane << SH;
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
-    const typename S::T* src = byte_offset(in + (size_t)i * N, yo[1 - cy(i)] + zo[1 - cz(i)]);
+    const typename S::T* src = byte_offset(in + (size_t)i * P, yo[1 - cy(i)] + zo[1 - cz(i)]);
     const uint32_t own = load_bits<S, NT>(byte_offset(src, (x0 << SH) + xs));
     if (cx(i) == 0) {
       g[i] = decode_bits<S>(own);
@@ -533,8 +533,8 @@ __device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in,
 // the compiler addresses each store as scalar base + lane offset rather than chaining 64-bit
 // vector adds from one plane to the next.
 template <class T>
-__device__ __forceinline__ __attribute__((address_space(1))) T* plane_base(T* out, size_t N, int i, uint32_t n0) {
-  T* p = out + (size_t)i * N + n0;
+__device__ __forceinline__ __attribute__((address_space(1))) T* plane_base(T* out, size_t P, int i, uint32_t n0) {
+  T* p = out + (size_t)i * P + n0;
   asm("" : "+s"(p));
   return (__attribute__((address_space(1))) T*)p;
 }
@@ -542,29 +542,29 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* plane_base(T* ou
 // Stores of a row wavefront: plane i of cell n0 + lane. 16-bit formats round two directions
 // into one register and store its halves.
 template <class S, bool NT>
-__device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_t N, uint32_t n0, uint32_t lane,
+__device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_t P, uint32_t n0, uint32_t lane,
                                           const float* g, uint32_t h) {
   using T = typename S::T;
   if constexpr (sizeof(T) == 4) {
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
       const float v = S::encode(g[i], direction_random(h, i));
-      auto* b = plane_base(out, N, i, n0);
+      auto* b = plane_base(out, P, i, n0);
       if constexpr (NT) __builtin_nontemporal_store(v, b + lane);
       else b[lane] = v;
     }
   } else {
     uint16_t* o = reinterpret_cast<uint16_t*>(out);
     const uint16_t v0 = __builtin_bit_cast(uint16_t, S::encode(g[0], direction_random(h, 0)));
-    auto* b0 = plane_base(o, N, 0, n0);
+    auto* b0 = plane_base(o, P, 0, n0);
     if constexpr (NT) __builtin_nontemporal_store(v0, b0 + lane);
     else b0[lane] = v0;
 #pragma unroll
     for (int q = 0; q < kPairs; ++q) {
       const int i = 2 * q + 1;
       const uint32_t v = S::encode_pair(g[i], direction_random(h, i), g[i + 1], direction_random(h, i + 1));
-      auto* bi = plane_base(o, N, i, n0);
-      auto* bj = plane_base(o, N, i + 1, n0);
+      auto* bi = plane_base(o, P, i, n0);
+      auto* bj = plane_base(o, P, i + 1, n0);
       if constexpr (NT) {
         __builtin_nontemporal_store((uint16_t)(v & 0xffffu), bi + lane);
         __builtin_nontemporal_store((uint16_t)(v >> 16), bj + lane);
@@ -610,7 +610,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     rest_state(g);
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
-  store_row<S, POL != kCached>(out, p.cells, w.n0, lane, g, cell_random(n, seed));
+  store_row<S, POL != kCached>(out, p.plane, w.n0, lane, g, cell_random(n, seed));
   if (MACRO && !is_wall(kind)) macro[n] = m;
 }
 
@@ -620,7 +620,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
 constexpr uint32_t kXcds = 8;
 
 template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL>
-__global__ __launch_bounds__(256) void stream_collide_kernel(const typename S::T* __restrict__ in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void stream_collide_kernel(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
                                                              float4* __restrict__ macro, StepParams p,
