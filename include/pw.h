@@ -122,6 +122,15 @@ size_t pw_solver_population_bytes(const pw_solver *s);
  * of the first fluid cell of the latest state. */
 pw_status pw_debug_inject_nan(pw_solver *s);
 
+/* Instability guard (src/lbm/nan-guard.ts:151-196, shaders/nan_guard.wgsl:4-17), fused into
+ * the step kernel instead of a separate pass: advances `count` steps in batches that end on
+ * multiples of PW_GUARD_INTERVAL; every batch writes the macros on its last step, and on a
+ * batch ending at a multiple that step also flags any fluid cell whose density is not finite
+ * or lies outside [0.5, 1.5]. A flagged check resets the solver (pw_solver_reset) and
+ * *unstable_step receives the step of the check; otherwise it is 0. */
+#define PW_GUARD_INTERVAL 200
+pw_status pw_solver_advance_guarded(pw_solver *s, uint32_t count, uint64_t *unstable_step);
+
 /* Device views for co-resident kernels (particles, interop). Pointers stay valid until
  * destroy; `populations` is the latest state and changes after every step. */
 typedef struct {

@@ -268,7 +268,7 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
     float4* macro = nullptr;
     if ((e = dev_alloc(b, &geo, geometry_words[t])) != hipSuccess) return fail(e, "batch geometry");
     if ((e = dev_alloc(b, &bnd, n_boundary[t])) != hipSuccess) return fail(e, "batch boundary");
-    if ((e = dev_alloc(b, &macro, N)) != hipSuccess) return fail(e, "batch macros");
+    if ((e = dev_alloc(b, &macro, N + 1)) != hipSuccess) return fail(e, "batch macros");  // + the guard word
     if ((e = hipMemcpyAsync(geo, geometries[t], geometry_words[t] * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return fail(e, "batch upload");
     if (n_boundary[t] && (e = hipMemcpyAsync(bnd, boundary_cells[t], n_boundary[t] * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return fail(e, "batch upload");
     tu.geo = geo;

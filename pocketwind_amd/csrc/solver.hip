@@ -27,6 +27,9 @@ pw_status device_error(hipError_t e, const char* what) {
   return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PW_ENOMEM : PW_EDEVICE;
 }
 
+// Clears the guard word (a kernel rather than a memset, so a captured step stays valid).
+__global__ void guard_clear_kernel(uint32_t* flag) { *flag = 0u; }
+
 __global__ __launch_bounds__(kForceBlock) void force_reduce_kernel(const double* __restrict__ partials,
                                                                    uint32_t n_partials,
                                                                    double* __restrict__ result) {
@@ -233,6 +236,7 @@ void free_all(pw_solver* s) {
   if (s->partials) hipFree(s->partials);
   if (s->force_dev) hipFree(s->force_dev);
   if (s->force_host) hipHostFree(s->force_host);
+  if (s->guard_host) hipHostFree(s->guard_host);
   if (s->force_event) hipEventDestroy(s->force_event);
   if (s->t0) hipEventDestroy(s->t0);
   if (s->t1) hipEventDestroy(s->t1);
@@ -362,7 +366,7 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   for (int k = 0; k < 2; ++k)
     if ((e = hipMalloc(&s->pops[k], set_bytes)) != hipSuccess) return fail(device_error(e, "population allocation"));
   if ((e = hipMalloc((void**)&s->geometry, geometry_words * 4)) != hipSuccess) return fail(device_error(e, "geometry allocation"));
-  if ((e = hipMalloc((void**)&s->macro, N * sizeof(float4))) != hipSuccess) return fail(device_error(e, "macro allocation"));
+  if ((e = hipMalloc((void**)&s->macro, (N + 1) * sizeof(float4))) != hipSuccess) return fail(device_error(e, "macro allocation"));
   if ((e = hipMalloc((void**)&s->boundary, (n_boundary ? n_boundary : 1) * 4)) != hipSuccess) return fail(device_error(e, "boundary allocation"));
   if ((e = hipMalloc((void**)&s->wall_cells, (s->n_wall ? s->n_wall : 1) * 4)) != hipSuccess) return fail(device_error(e, "wall cell allocation"));
   if (learned) {
@@ -378,6 +382,8 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   if ((e = hipMalloc((void**)&s->partials, (s->n_partials ? s->n_partials : 1) * 3 * sizeof(double))) != hipSuccess) return fail(device_error(e, "force partial allocation"));
   if ((e = hipMalloc((void**)&s->force_dev, 4 * sizeof(double))) != hipSuccess) return fail(device_error(e, "force allocation"));
   if ((e = hipHostMalloc((void**)&s->force_host, 4 * sizeof(double), hipHostMallocDefault)) != hipSuccess) return fail(device_error(e, "pinned force slot"));
+  s->guard_flag = reinterpret_cast<uint32_t*>(s->macro + N);
+  if ((e = hipHostMalloc((void**)&s->guard_host, sizeof(uint32_t), hipHostMallocDefault)) != hipSuccess) return fail(device_error(e, "pinned guard slot"));
   if ((e = hipEventCreateWithFlags(&s->force_event, hipEventDisableTiming)) != hipSuccess) return fail(device_error(e, "event"));
   if ((e = hipEventCreate(&s->t0)) != hipSuccess || (e = hipEventCreate(&s->t1)) != hipSuccess) return fail(device_error(e, "event"));
   if ((e = hipMemcpyAsync(s->geometry, geometry, geometry_words * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "geometry upload"));
@@ -397,6 +403,44 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
 }
 
 pw_status pw_solver_step(pw_solver* s, uint32_t count, int write_macro_on_last) {
+  return step_run(s, count, write_macro_on_last);
+}
+
+pw_status pw_solver_advance_guarded(pw_solver* s, uint32_t count, uint64_t* unstable_step) {
+  if (pw_status st = check_handle(s)) return st;
+  if (!unstable_step) {
+    set_error("Null output.");
+    return PW_EINVAL;
+  }
+  *unstable_step = 0;
+  // nan-guard.ts:151-196: batches end on multiples of the interval; each batch writes the
+  // macros on its last step, and a batch that ends on a multiple is checked.
+  uint32_t remaining = count;
+  while (remaining > 0) {
+    const uint32_t until = PW_GUARD_INTERVAL - (uint32_t)(s->steps % PW_GUARD_INTERVAL);
+    const uint32_t steps = remaining < until ? remaining : until;
+    const bool check = (s->steps + steps) % PW_GUARD_INTERVAL == 0;
+    if (check) hipLaunchKernelGGL(guard_clear_kernel, dim3(1), dim3(1), 0, s->stream, s->guard_flag);
+    if (pw_status st = step_run(s, steps, 1)) return st;
+    remaining -= steps;
+    if (!check) continue;
+    PW_TRY(hipMemcpyAsync(s->guard_host, s->guard_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream), "guard readback");
+    PW_TRY(hipStreamSynchronize(s->stream), "guard readback");
+    if (*s->guard_host != 0) {
+      const uint64_t at = s->steps;
+      if (pw_status st = pw_solver_reset(s)) return st;
+      PW_TRY(hipStreamSynchronize(s->stream), "reset");
+      *unstable_step = at;
+      return PW_OK;
+    }
+  }
+  return PW_OK;
+}
+
+}  // extern "C"
+
+namespace pw {
+pw_status step_run(pw_solver* s, uint32_t count, int write_macro_on_last) {
   if (pw_status st = check_handle(s)) return st;
   for (uint32_t k = 0; k < count; ++k) {
     const bool macro = write_macro_on_last && k + 1 == count;
@@ -412,6 +456,9 @@ pw_status pw_solver_step(pw_solver* s, uint32_t count, int write_macro_on_last) 
   if (count) PW_TRY(hipGetLastError(), "step launch");
   return PW_OK;
 }
+}  // namespace pw
+
+extern "C" {
 
 pw_status pw_solver_encode_force(pw_solver* s) {
   if (pw_status st = check_handle(s)) return st;

@@ -81,6 +81,8 @@ namespace pw {
 const void* mlp_packed(const pw_mlp* m);
 size_t mlp_packed_bytes();
 void launch_closure(pw_solver* s, StepParams& p);
+// `count` steps, the last writing the macros (and the guard word after them) when asked.
+pw_status step_run(pw_solver* s, uint32_t count, int write_macro_on_last);
 }  // namespace pw
 
 struct pw_solver {
@@ -119,6 +121,8 @@ struct pw_solver {
   int closure_on = 0;
   uint32_t closure_every = 1;  // steps between closure evaluations
   int closure_fresh = 0;       // closure_tau holds an evaluation of this run
+  uint32_t* guard_flag = nullptr;       // the word after the macros (not a separate allocation)
+  uint32_t* guard_host = nullptr;       // pinned readback
   uint32_t kernel = 1;  // pw_kernel actually used by step()
   uint32_t block = 256;  // threads per step block
 };

@@ -430,6 +430,15 @@ __host__ __device__ __forceinline__ void rest_state(float* g) {
   for (int i = 0; i < kQ; ++i) g[i] = 0.0f;
 }
 
+// nan_guard.wgsl:4-17 fused into every macro-writing step: a density that is not finite or
+// outside [0.5, 1.5] (NaN fails both comparisons) sets the guard word, the first word after
+// the cells' macros (macro buffers hold cells + 1 entries). Every writer stores the same 1, so
+// a plain store suffices; the word lives beside the macros so that the step needs no further
+// kernel argument.
+__host__ __device__ __forceinline__ void guard_density(float4* macro, const StepParams& p, float rho) {
+  if (!(rho >= 0.5f && rho <= 1.5f)) *reinterpret_cast<uint32_t*>(macro + p.cells) = 1u;
+}
+
 // Cache policy of the population streams: cached, non-temporal loads and stores, or cached
 // loads with non-temporal stores. Flags and wall-path loads are always cached.
 enum { kCached = 0, kNtAll = 1, kNtStores = 2 };
@@ -470,7 +479,10 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
   store_state<S, POL != kCached>(out, P, self, g, cell_random(n, seed));
-  if (MACRO && !is_wall(kind)) macro[n] = m;
+  if (MACRO && !is_wall(kind)) {
+    macro[n] = m;
+    guard_density(macro, p, m.x);
+  }
 }
 
 // Raw storage bits of one element, zero-extended to 32 bits, and their decoding.
@@ -611,7 +623,10 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
   store_row<S, POL != kCached>(out, p.plane, w.n0, lane, g, cell_random(n, seed));
-  if (MACRO && !is_wall(kind)) macro[n] = m;
+  if (MACRO && !is_wall(kind)) {
+    macro[n] = m;
+    guard_density(macro, p, m.x);
+  }
 }
 
 // Blocks are renumbered so that each of the 8 XCDs (which receive blocks round-robin) steps

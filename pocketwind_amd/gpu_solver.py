@@ -13,8 +13,9 @@ Device work is queued on the solver's own HIP stream. Readbacks wait for it.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
-from typing import Literal, Union
+from typing import Literal, Optional, Union
 
 import numpy as np
 
@@ -27,6 +28,19 @@ _OPERATORS = {"trt": 0, "reg": 1}
 _WALLS = {"bb": 0, "bouzidi": 1}
 _STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
 _WALL_MODELS = {"law": 0, "learned": 1}
+
+NAN_GUARD_INTERVAL = 200  # nan-guard.ts:6
+
+
+def is_unstable_density(rho: float) -> bool:
+    """nan-guard.ts: non-finite or outside [0.5, 1.5] (both endpoints are stable)."""
+    return not math.isfinite(rho) or rho < 0.5 or rho > 1.5
+
+
+def unstable_message(step: int) -> dict:
+    """The 'unstable' event detail (nan-guard.ts resetUnstableRun, app/messages.ts)."""
+    return {"step": step, "what": f"The run went unstable at step {step} and was reset",
+            "next": "Lower the speed, or choose a higher tier under Advanced"}
 
 
 @dataclass(frozen=True)
@@ -179,6 +193,30 @@ class GpuSolver:
             return
         N.check(N.lib().pw_debug_inject_nan(self._live()))
         self._injected_nan = True
+
+    def advance_guarded(self, count: int, inject_nan: Optional[int] = None):
+        """NanGuard.advance (nan-guard.ts:151-196) with the density check fused into the step:
+        steps `count` times in batches ending on multiples of NAN_GUARD_INTERVAL; a check that
+        finds a density outside [0.5, 1.5] or not finite resets the solver and returns the
+        instability message {step, what, next}; otherwise None. `inject_nan` plays the runner's
+        inject_nan parameter: NaN goes into the first fluid cell after that step."""
+        if not isinstance(count, int) or count < 0:
+            raise ValueError("Step count must be a non-negative safe integer.")
+        remaining = count
+        while remaining > 0:
+            until = NAN_GUARD_INTERVAL - self.step_count % NAN_GUARD_INTERVAL
+            until_inject = (inject_nan - self.step_count) if (inject_nan and inject_nan > self.step_count) \
+                else remaining
+            steps = min(remaining, until, until_inject)
+            at = C.c_uint64()
+            N.check(N.lib().pw_solver_advance_guarded(self._live(), steps, C.byref(at)))
+            remaining -= steps
+            if at.value:
+                self._injected_nan = False
+                return unstable_message(int(at.value))
+            if inject_nan:
+                self.inject_nan_for_runner(inject_nan)
+        return None
 
     def device_state(self) -> N.DeviceState:
         out = N.DeviceState()
