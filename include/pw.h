@@ -131,6 +131,15 @@ pw_status pw_debug_inject_nan(pw_solver *s);
 #define PW_GUARD_INTERVAL 200
 pw_status pw_solver_advance_guarded(pw_solver *s, uint32_t count, uint64_t *unstable_step);
 
+/* The binary blocks of the legacy VTK field file (src/io/vtk.ts:27-80) for the macro buffer,
+ * with velocities scaled as latticeToField does (src/io/field-export.ts:22-34): rho_be gets
+ * 4*N bytes, big-endian float32 densities; velocity_be gets 12*N bytes, big-endian float32
+ * (ux, uy, uz) * velocity_scale per cell, each product of the float32 value and the double
+ * scale rounded once to float32. velocity_scale = speedMps / U_LU must be positive and
+ * finite. The host writes the text lines around the blocks. */
+pw_status pw_solver_field_vtk_blocks(pw_solver *s, double velocity_scale, void *rho_be, size_t rho_bytes,
+                                     void *velocity_be, size_t velocity_bytes);
+
 /* Device views for co-resident kernels (particles, interop). Pointers stay valid until
  * destroy; `populations` is the latest state and changes after every step. */
 typedef struct {

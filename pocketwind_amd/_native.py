@@ -65,6 +65,8 @@ SIGNATURES = {
     "pw_solver_device_state": (C.c_int, [C.c_void_p, C.POINTER(DeviceState)]),
     "pw_solver_set_kernel": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pw_solver_advance_guarded": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "pw_solver_field_vtk_blocks": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_size_t, C.c_void_p,
+                                             C.c_size_t]),
     "pw_solver_block_size": (C.c_uint32, [C.c_void_p]),
     "pw_solver_kernel": (C.c_uint32, [C.c_void_p]),
     "pw_solver_synchronize": (C.c_int, [C.c_void_p]),
