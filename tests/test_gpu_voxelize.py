@@ -76,3 +76,15 @@ def test_rejects_invalid_input():
     bad[4] = np.inf
     with pytest.raises(ValueError):
         voxelize(bad, None, (64, 64, 64), False)
+
+
+def test_imported_3mf_ellipsoid_placed_and_voxelised_bit_exact():
+    """A mesh from the 3MF importer (327680 triangles), placed as a body, free and on the floor,
+    and voxelised with link fractions on the device, against the oracle."""
+    from archives import ellipsoid_archive
+    from pocketwind_amd.mesh_io import parse_model
+    model = parse_model(ellipsoid_archive(), "ellipsoid.3mf")
+    layout = (128, 64, 64)
+    for floor in (False, True):
+        body, _ = bodies.place_mesh(model.mesh.positions, None, 40, layout, floor)
+        assert_same(voxelize(body, None, layout, floor), O.voxelize(body, None, layout, floor))
