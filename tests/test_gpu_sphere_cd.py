@@ -47,11 +47,15 @@ def sphere_cd(tier: str, operator: str, precision: str = "f32"):
         solver.destroy()
 
 
+MEASURED = {}  # (tier, operator) -> Cd of this run, for the convergence check
+
+
 @pytest.mark.parametrize("operator", ["trt", "reg"])
 @pytest.mark.parametrize("tier", ["S", "M", "L"])
 def test_sphere_cd_within_half_percent_of_reference(tier, operator):
     row = ROWS[f"sphere-re-100-{tier}-{operator}"]
     cd, grid, samples = sphere_cd(tier, operator)
+    MEASURED[(tier, operator)] = cd
     assert grid.frontal_cells == round(row["voxelAreaM2"] / row["cellSizeM"] ** 2)
     assert row["steps"] == 6 * drag.flow_through_steps(TIERS[tier][0])
     err = cd / row["cd"] - 1
@@ -65,3 +69,23 @@ def test_sphere_cd_f16_storage_close_to_f32():
     cd16, _, _ = sphere_cd("S", "trt", "f16")
     print(f"sphere S trt: Cd f32 {cd32:.8f} f16 {cd16:.8f} ({100 * (cd16 / cd32 - 1):+.4f} %)")
     assert abs(cd16 / cd32 - 1) < 0.01
+
+
+@pytest.mark.parametrize("operator", ["trt", "reg"])
+def test_grid_convergence_of_the_sphere_series_matches_the_reference(operator):
+    """The convergence section (results.ts:208-260, h = 1 / nx over S, M, L) from this solver's
+    Cd values: monotone like the reference's, with the extrapolated Cd within 0.5 % of the
+    reference's and the observed order within 0.25."""
+    from pocketwind_amd.validation import grid_convergence
+    samples = []
+    for tier in ("S", "M", "L"):
+        cd = MEASURED.get((tier, operator))
+        if cd is None:
+            cd = sphere_cd(tier, operator)[0]
+        samples.append({"h": 1 / TIERS[tier][0], "value": cd})
+    mine = grid_convergence(samples)
+    ref = next(r for r in GOLDEN["convergence"] if r["case"] == "sphere-re-100" and r["operator"] == operator)
+    print(f"sphere {operator}: convergence {mine} reference {ref}")
+    assert mine["kind"] == ref["kind"] == "monotone"
+    assert abs(mine["extrapolated"] / ref["extrapolated"] - 1) < 0.005
+    assert abs(mine["order"] - ref["order"]) < 0.25

@@ -1,0 +1,36 @@
+"""The validation harness end to end on the device: the laminar sphere cases at tier S run on
+this solver (forces in newtons every 8 steps, measured frontal area and blockage), turned into
+result rows by the reference's rules, checked against the reference's stored rows (same voxel
+area and blockage, Cd within 0.5 %, same pass verdict and reasons), then written as a foreign
+results document and scored."""
+from __future__ import annotations
+
+import json
+import os
+
+import pytest
+
+from pocketwind_amd import validation as V
+
+pytestmark = pytest.mark.gpu
+
+REF = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_results.json")))
+STORED = {r["id"]: r for r in REF["laminar"]}
+
+
+def test_laminar_sphere_rows_match_the_reference_rows():
+    rows = []
+    for operator in ("trt", "reg"):
+        spec = V.laminar_sphere_case("S", operator)
+        run = V.run_laminar_sphere(spec)
+        row = V.row_from_run(spec, run, V.RunContext("test", "2026-01-01T00:00:00.000Z"))
+        ref = STORED[spec.id]
+        print(f"{spec.id}: Cd {row['cd']:.10f} (reference {ref['cd']:.10f}), relStd {row['relStdPct']:.4f} %, "
+              f"drift {row['driftPct']:.4f} %, pass {row['pass']} {row['reasons']}")
+        assert (row["voxelAreaM2"], row["blockage"]) == (ref["voxelAreaM2"], ref["blockage"])
+        assert abs(row["cd"] / ref["cd"] - 1) < 0.005
+        assert (row["pass"], row["reasons"]) == (ref["pass"], ref["reasons"])
+        assert len(row["forceHistory"]) == spec.steps // 8 and row["steps"] == spec.steps
+        rows.append(row)
+    out, err, status = V.score_results(V.foreign_results("pocketwind_amd", rows))
+    assert status == 0 and out[-1] == "unmatched: 0" and len(out) == 3
