@@ -29,7 +29,7 @@ def test_laminar_sphere_rows_match_the_reference_rows():
               f"drift {row['driftPct']:.4f} %, pass {row['pass']} {row['reasons']}")
         assert (row["voxelAreaM2"], row["blockage"]) == (ref["voxelAreaM2"], ref["blockage"])
         assert abs(row["cd"] / ref["cd"] - 1) < 0.005
-        # The S rows sit near the 10 % band edge (reference errors 10.03 % and 10.28 %), so the
+        # The S TRT row sits at the 10 % band edge (reference error 10.03 %), so the
         # verdict follows this run's own error.
         expected = [] if abs(row["errorPct"]) <= 10 else [f"reference error {V.to_fixed(row['errorPct'], 1)}% exceeds 10%"]
         assert (row["pass"], row["reasons"]) == (not expected, expected)
