@@ -634,8 +634,15 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
 // wavefronts share stay in one XCD's L2. The grid holds 8 per_xcd blocks.
 constexpr uint32_t kXcds = 8;
 
+// Waves per SIMD the step kernel is compiled for (a register budget of 512 / waves VGPRs).
+// Seven leaves REG + LES a few spills and measured fastest; TRT + LES keeps the populations,
+// their pair sums and the equilibrium live through the wall function and spills heavily
+// below about 106 registers, so it runs at four.
+template <int OP, bool LES>
+constexpr int kStepWaves = (LES && OP == kOpTrt) ? 4 : 7;
+
 template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void stream_collide_kernel(const typename S::T* __restrict__ in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<OP, LES>))) void stream_collide_kernel(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
                                                              float4* __restrict__ macro, StepParams p,

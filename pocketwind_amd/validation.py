@@ -77,7 +77,7 @@ SAFETY_FACTOR = 1.25
 
 
 def _sign(x: float) -> float:
-    return (x > 0) - (x < 0)
+    return float(x > 0) - float(x < 0)
 
 
 def _observed_order(ratio: float, r21: float, r32: float, s: float) -> Optional[float]:
@@ -98,7 +98,7 @@ def _observed_order(ratio: float, r21: float, r32: float, s: float) -> Optional[
 
 def grid_convergence(rows: Sequence[Dict[str, float]]) -> Dict[str, Union[str, float]]:
     """gridConvergence: rows of {h, value}; only the three finest levels are used."""
-    ordered = sorted(rows, key=lambda r: r["h"])
+    ordered = sorted(({"h": float(r["h"]), "value": float(r["value"])} for r in rows), key=lambda r: r["h"])
     if len(ordered) < 3:
         return {"kind": "insufficient"}
     fine, mid, coarse = ordered[:3]
