@@ -155,6 +155,22 @@ void launch_step(pw_solver* s, bool macro, const StepParams& params) {
   }
 }
 
+template <class S>
+void launch_edge(pw_solver* s, void* set, const StepParams& p) {
+  hipLaunchKernelGGL(edge_moments_kernel<S>, dim3(blocks_for((size_t)p.ny * p.nz)), dim3(kBlock), 0, s->stream,
+                     (typename S::T*)set, p);
+}
+
+// Recompute the upstream-outlet moments stored with a set from its populations (after the
+// populations were written other than by a step).
+void launch_edge_any(pw_solver* s, void* set, const StepParams& p) {
+  switch (s->params.storage) {
+    case PW_STORAGE_F16: launch_edge<StoreF16>(s, set, p); break;
+    case PW_STORAGE_BF16: launch_edge<StoreBF16>(s, set, p); break;
+    default: launch_edge<StoreF32>(s, set, p); break;
+  }
+}
+
 void launch_init_both(pw_solver* s, const StepParams& p) {
   for (int k = 0; k < 2; ++k) {
     switch (s->params.storage) {
@@ -162,7 +178,13 @@ void launch_init_both(pw_solver* s, const StepParams& p) {
       case PW_STORAGE_BF16: launch_init<StoreBF16>(s, s->pops[k], p); break;
       default: launch_init<StoreF32>(s, s->pops[k], p); break;
     }
+    launch_edge_any(s, s->pops[k], p);
   }
+}
+
+// Bytes of one population set: 19 planes, then one float4 of upstream-outlet moments per row.
+size_t set_bytes(const pw_solver* s) {
+  return edge_offset_bytes(s->sp.plane, s->elem_bytes) + (size_t)s->sp.ny * s->sp.nz * sizeof(float4);
 }
 
 template <class S>
@@ -362,9 +384,9 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   };
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) return fail(device_error(e, "hipStreamCreate"));
-  const size_t set_bytes = (size_t)kQ * full.plane * s->elem_bytes;
+  const size_t bytes_per_set = edge_offset_bytes(full.plane, s->elem_bytes) + (size_t)full.ny * full.nz * sizeof(float4);
   for (int k = 0; k < 2; ++k)
-    if ((e = hipMalloc(&s->pops[k], set_bytes)) != hipSuccess) return fail(device_error(e, "population allocation"));
+    if ((e = hipMalloc(&s->pops[k], bytes_per_set)) != hipSuccess) return fail(device_error(e, "population allocation"));
   if ((e = hipMalloc((void**)&s->geometry, geometry_words * 4)) != hipSuccess) return fail(device_error(e, "geometry allocation"));
   if ((e = hipMalloc((void**)&s->macro, (N + 1) * sizeof(float4))) != hipSuccess) return fail(device_error(e, "macro allocation"));
   if ((e = hipMalloc((void**)&s->boundary, (n_boundary ? n_boundary : 1) * 4)) != hipSuccess) return fail(device_error(e, "boundary allocation"));
@@ -549,6 +571,8 @@ pw_status pw_solver_write_populations(pw_solver* s, const void* in, size_t in_by
   const size_t row = s->cells * s->elem_bytes, pitch = (size_t)s->sp.plane * s->elem_bytes;
   PW_TRY(hipMemcpy2DAsync(s->pops[s->current], pitch, in, row, row, kQ, hipMemcpyHostToDevice, s->stream),
          "population upload");
+  launch_edge_any(s, s->pops[s->current], s->sp);
+  PW_TRY(hipGetLastError(), "upstream moments");
   PW_TRY(hipStreamSynchronize(s->stream), "population upload");
   return PW_OK;
 }
@@ -557,6 +581,7 @@ pw_status pw_debug_inject_nan(pw_solver* s) {
   if (pw_status st = check_handle(s)) return st;
   if (s->first_fluid < 0) return PW_OK;
   const size_t off = (size_t)s->first_fluid * s->elem_bytes;
+  PW_TRY(hipStreamSynchronize(s->stream), "inject");
   if (s->elem_bytes == 4) {
     const uint32_t nan_bits = 0x7fc00000u;
     PW_TRY(hipMemcpy((char*)s->pops[s->current] + off, &nan_bits, 4, hipMemcpyHostToDevice), "inject");
@@ -564,6 +589,8 @@ pw_status pw_debug_inject_nan(pw_solver* s) {
     const uint16_t nan_bits = s->params.storage == PW_STORAGE_F16 ? 0x7e00u : 0x7fc0u;
     PW_TRY(hipMemcpy((char*)s->pops[s->current] + off, &nan_bits, 2, hipMemcpyHostToDevice), "inject");
   }
+  launch_edge_any(s, s->pops[s->current], s->sp);
+  PW_TRY(hipGetLastError(), "upstream moments");
   return PW_OK;
 }
 
@@ -582,7 +609,7 @@ pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
   if (kernel == PW_KERNEL_AUTO) {
     PW_TRY(hipStreamSynchronize(s->stream), "tune");
     // Tuning steps advance the state: keep a copy of the latest set and restore it.
-    const size_t bytes = (size_t)kQ * s->sp.plane * s->elem_bytes;
+    const size_t bytes = set_bytes(s);
     void* keep = nullptr;
     PW_TRY(hipMalloc(&keep, bytes), "tune scratch");
     const int cur = s->current;

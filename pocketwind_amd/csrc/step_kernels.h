@@ -346,7 +346,10 @@ __host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__
 }
 
 // common.wgsl:145-150 with rho - 1 = 0, in pair form: eq_i = w (4.5 cu^2 - 1.5 u^2) + 3 w cu.
+// Evaluated without contraction into FMAs, so that the result does not depend on where it is
+// inlined (the EQ cells of the row and cell paths, and of batched tunnels, agree bit for bit).
 __host__ __device__ __forceinline__ void equilibrium_values(float ux, float uy, float uz, float* g) {
+#pragma clang fp contract(off)
   const float h15 = 1.5f * (ux * ux + uy * uy + uz * uz);
   g[0] = wf(0) * (0.0f - h15);  // +0 at rest, as the stored rest state of walls
 #pragma unroll
@@ -368,6 +371,15 @@ __host__ __device__ __forceinline__ void store_equilibrium(typename S::T* __rest
   store_state<S, NT>(out, P, n << S::kShift, g, h);
 }
 
+// Walls keep the rest state (all deviations zero; stream_collide.wgsl:96 skips them). They
+// store it with their wavefront anyway: a wavefront whose lanes all store writes whole cache
+// lines, while a skipped lane leaves a partly written line that HBM must read, merge and
+// write back.
+__host__ __device__ __forceinline__ void rest_state(float* g) {
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) g[i] = 0.0f;
+}
+
 // init.wgsl:1-8: walls at rest, every other cell at the inflow equilibrium.
 template <class S>
 __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ out,
@@ -376,9 +388,14 @@ __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ o
   const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= p.cells) return;
   const uint32_t kind = geo[n] & kTypeMask;
-  const float ux = is_wall(kind) ? 0.0f : p.u_in;
-  store_equilibrium<S>(out, p.plane, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
-  macro[n] = make_float4(1.0f, ux, 0.0f, 0.0f);
+  const bool wall = is_wall(kind);
+  float g[kQ];
+  // Walls store exact zeros (the rest state) rather than an equilibrium at zero velocity,
+  // whose direction-0 term could come out as -0 depending on how the compiler folds it.
+  if (wall) rest_state(g);
+  else equilibrium_values(p.u_in, 0.0f, 0.0f, g);
+  store_state<S, false>(out, p.plane, n << S::kShift, g, cell_random(n, kInitSeed));
+  macro[n] = make_float4(1.0f, wall ? 0.0f : p.u_in, 0.0f, 0.0f);
 }
 
 // Moments, equilibrium and collision of one gathered fluid cell (stream_collide.wgsl:114-133);
@@ -394,41 +411,90 @@ __host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, 
   return make_float4(c.rho, c.ux, c.uy, c.uz);
 }
 
-// stream_collide.wgsl:83-107: an EQ cell takes the equilibrium at unit density and the
-// inlet speed or, on the outlet plane x = nx - 1, the upstream cell's velocity (pressure
-// outlet). Writes the post-step values into g and returns (1, u).
+// The outlet plane x = nx - 1 takes its velocity from the stored state of the cell upstream,
+// x = nx - 2 (stream_collide.wgsl:102-107): (1 + sum g, sum c g) of its 19 stored values, in
+// direction order.
+struct RowMoments {
+  float mass, mx, my, mz;
+};
+
+__host__ __device__ __forceinline__ void add_moment(RowMoments& m, int i, float v) {
+  m.mass += v;
+  m.mx += cx(i) * v;
+  m.my += cy(i) * v;
+  m.mz += cz(i) * v;
+}
+
 template <class S>
-__host__ __device__ __forceinline__ float4 eq_state(const typename S::T* __restrict__ in, const StepParams& p,
-                                                    uint32_t n, uint32_t x, float* g) {
+__host__ __device__ __forceinline__ RowMoments stored_moments(const typename S::T* __restrict__ in, size_t P,
+                                                              uint32_t n) {
+  RowMoments m{1.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) add_moment(m, i, S::load(in + (size_t)i * P + n));
+  return m;
+}
+
+// The step that stores a row's x = nx - 2 state also stores those moments, one float4 per row
+// (y + ny z) in the set's allocation after its 19 planes; the outlet reads that one value
+// instead of gathering 19 at the end of its wavefront, whose latency nothing would hide.
+__host__ __device__ __forceinline__ size_t edge_offset_bytes(size_t plane, size_t elem_bytes) {
+  return (kQ * plane * elem_bytes + 255) & ~(size_t)255;
+}
+
+template <class T>
+__host__ __device__ __forceinline__ float4* edge_rows(T* set, const StepParams& p) {
+  return reinterpret_cast<float4*>(reinterpret_cast<char*>(set) + edge_offset_bytes(p.plane, sizeof(T)));
+}
+
+template <class T>
+__host__ __device__ __forceinline__ const float4* edge_rows(const T* set, const StepParams& p) {
+  return reinterpret_cast<const float4*>(reinterpret_cast<const char*>(set) + edge_offset_bytes(p.plane, sizeof(T)));
+}
+
+__host__ __device__ __forceinline__ RowMoments row_moments(const float4& e) { return RowMoments{e.x, e.y, e.z, e.w}; }
+
+// Moments of the values a store of g with the random word h writes.
+template <class S>
+__host__ __device__ __forceinline__ RowMoments encoded_moments(const float* g, uint32_t h) {
+  RowMoments m{1.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) add_moment(m, i, S::decode(S::encode(g[i], direction_random(h, i))));
+  return m;
+}
+
+// stream_collide.wgsl:83-107: an EQ cell takes the equilibrium at unit density and the
+// inlet speed or, on the outlet plane x = nx - 1 (nx > 1), the velocity of the upstream
+// moments `up`. Writes the post-step values into g and returns (1, u).
+__host__ __device__ __forceinline__ float4 eq_from(const StepParams& p, uint32_t x, const RowMoments& up, float* g) {
   float ux = p.u_in, uy = 0.0f, uz = 0.0f;
   if (x == p.nx - 1 && x > 0) {
-    const size_t P = p.plane;
-    float mass = 1.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
-#pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-      const float v = S::load(in + (size_t)i * P + (n - 1));
-      mass += v;
-      mx += cx(i) * v;
-      my += cy(i) * v;
-      mz += cz(i) * v;
-    }
-    const float inv = fast_rcp(mass);
-    ux = mx * inv;
-    uy = my * inv;
-    uz = mz * inv;
+    const float inv = fast_rcp(up.mass);
+    ux = up.mx * inv;
+    uy = up.my * inv;
+    uz = up.mz * inv;
   }
   equilibrium_values(ux, uy, uz, g);
   return make_float4(1.0f, ux, uy, uz);
 }
 
-// Walls keep the rest state (all deviations zero; stream_collide.wgsl:96 skips them). They
-// store it with their wavefront anyway: a wavefront whose lanes all store writes whole cache
-// lines, while a skipped lane leaves a partly written line that HBM must read, merge and
-// write back.
-__host__ __device__ __forceinline__ void rest_state(float* g) {
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) g[i] = 0.0f;
+// The same with the upstream state gathered from the input set.
+template <class S>
+__host__ __device__ __forceinline__ float4 eq_state(const typename S::T* __restrict__ in, const StepParams& p,
+                                                    uint32_t n, uint32_t x, float* g) {
+  RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
+  if (x == p.nx - 1 && x > 0) up = stored_moments<S>(in, p.plane, n - 1);
+  return eq_from(p, x, up, g);
 }
+
+// The upstream moments of every row from a set's stored state (after init or an upload).
+template <class S>
+__global__ __launch_bounds__(256) void edge_moments_kernel(typename S::T* __restrict__ set, StepParams p) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= p.ny * p.nz || p.nx < 2) return;
+  const RowMoments m = stored_moments<S>(set, p.plane, r * p.nx + (p.nx - 2));
+  edge_rows(set, p)[r] = make_float4(m.mass, m.mx, m.my, m.mz);
+}
+
 
 // nan_guard.wgsl:4-17 fused into every macro-writing step: a density that is not finite or
 // outside [0.5, 1.5] (NaN fails both comparisons) sets the guard word, the first word after
@@ -444,7 +510,10 @@ __host__ __device__ __forceinline__ void guard_density(float4* macro, const Step
 enum { kCached = 0, kNtAll = 1, kNtStores = 2 };
 
 // stream_collide.wgsl:95-136, one cell per lane; `seed` selects the step's rounding words.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL = kCached>
+// EDGE: the outlet reads the upstream moments stored with the input set and the row's
+// x = nx - 2 cell stores its own with the output set (the solver's sets carry them); without
+// it the outlet gathers the upstream state (batched tunnels).
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL = kCached, bool EDGE = false>
 __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
@@ -473,12 +542,25 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
     }
     m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
-    m = eq_state<S>(in, p, n, x, g);
+    if constexpr (EDGE) {
+      RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
+      if (x == p.nx - 1 && x > 0) up = row_moments(edge_rows(in, p)[y + p.ny * z]);
+      m = eq_from(p, x, up, g);
+    } else {
+      m = eq_state<S>(in, p, n, x, g);
+    }
   } else {
     rest_state(g);
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
-  store_state<S, POL != kCached>(out, P, self, g, cell_random(n, seed));
+  const uint32_t h = cell_random(n, seed);
+  store_state<S, POL != kCached>(out, P, self, g, h);
+  if constexpr (EDGE) {
+    if (x + 2 == p.nx) {
+      const RowMoments e = encoded_moments<S>(g, h);
+      edge_rows(out, p)[y + p.ny * z] = make_float4(e.mass, e.mx, e.my, e.mz);
+    }
+  }
   if (MACRO && !is_wall(kind)) {
     macro[n] = m;
     guard_density(macro, p, m.x);
@@ -553,9 +635,10 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* plane_base(T* ou
 
 // Stores of a row wavefront: plane i of cell n0 + lane. 16-bit formats round two directions
 // into one register and store its halves.
+// With `moments` (wavefront-uniform) the stored values' moments accumulate into m.
 template <class S, bool NT>
 __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_t P, uint32_t n0, uint32_t lane,
-                                          const float* g, uint32_t h) {
+                                          const float* g, uint32_t h, bool moments, RowMoments& m) {
   using T = typename S::T;
   if constexpr (sizeof(T) == 4) {
 #pragma unroll
@@ -564,6 +647,7 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_
       auto* b = plane_base(out, P, i, n0);
       if constexpr (NT) __builtin_nontemporal_store(v, b + lane);
       else b[lane] = v;
+      if (moments) add_moment(m, i, v);
     }
   } else {
     uint16_t* o = reinterpret_cast<uint16_t*>(out);
@@ -571,10 +655,15 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_
     auto* b0 = plane_base(o, P, 0, n0);
     if constexpr (NT) __builtin_nontemporal_store(v0, b0 + lane);
     else b0[lane] = v0;
+    if (moments) add_moment(m, 0, decode_bits<S>(v0));
 #pragma unroll
     for (int q = 0; q < kPairs; ++q) {
       const int i = 2 * q + 1;
       const uint32_t v = S::encode_pair(g[i], direction_random(h, i), g[i + 1], direction_random(h, i + 1));
+      if (moments) {
+        add_moment(m, i, decode_bits<S>(v & 0xffffu));
+        add_moment(m, i + 1, decode_bits<S>(v >> 16));
+      }
       auto* bi = plane_base(o, P, i, n0);
       auto* bj = plane_base(o, P, i + 1, n0);
       if constexpr (NT) {
@@ -604,6 +693,11 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
   RowWave w;
   w.n0 = __builtin_amdgcn_readfirstlane(n - lane);
   cell_coords(w.n0, p, w.x0, w.y, w.z);
+  // The wavefront holding the outlet (lane 63) and its upstream cell (lane 62) of a row.
+  const bool outlet_wave = w.x0 + 64 == p.nx;
+  const uint32_t row = w.y + p.ny * w.z;
+  RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
+  if (outlet_wave) up = row_moments(edge_rows(in, p)[row]);
   float g[kQ];
   gather_row<S, POL == kNtAll>(in, p, w, lane, g);
   const uint32_t x = w.x0 + lane;
@@ -617,12 +711,14 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     }
     m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
-    m = eq_state<S>(in, p, n, x, g);
+    m = eq_from(p, x, up, g);
   } else {
     rest_state(g);
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
-  store_row<S, POL != kCached>(out, p.plane, w.n0, lane, g, cell_random(n, seed));
+  RowMoments stored{1.0f, 0.0f, 0.0f, 0.0f};
+  store_row<S, POL != kCached>(out, p.plane, w.n0, lane, g, cell_random(n, seed), outlet_wave, stored);
+  if (outlet_wave && lane == 62) edge_rows(out, p)[row] = make_float4(stored.mass, stored.mx, stored.my, stored.mz);
   if (MACRO && !is_wall(kind)) {
     macro[n] = m;
     guard_density(macro, p, m.x);
@@ -654,7 +750,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<
   if (p.nx % 64 == 0)
     stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL>(in, out, geo, macro, p, n, seed);
   else
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL>(in, out, geo, macro, p, n, seed);
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL, true>(in, out, geo, macro, p, n, seed);
 }
 
 }  // namespace pw
