@@ -420,9 +420,9 @@ struct RowMoments {
 
 __host__ __device__ __forceinline__ void add_moment(RowMoments& m, int i, float v) {
   m.mass += v;
-  m.mx += cx(i) * v;
-  m.my += cy(i) * v;
-  m.mz += cz(i) * v;
+  if (cx(i)) m.mx += cx(i) * v;
+  if (cy(i)) m.my += cy(i) * v;
+  if (cz(i)) m.mz += cz(i) * v;
 }
 
 template <class S>
@@ -693,8 +693,9 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
   RowWave w;
   w.n0 = __builtin_amdgcn_readfirstlane(n - lane);
   cell_coords(w.n0, p, w.x0, w.y, w.z);
-  // The wavefront holding the outlet (lane 63) and its upstream cell (lane 62) of a row.
-  const bool outlet_wave = w.x0 + 64 == p.nx;
+  // The wavefront holding an EQ outlet cell (lane 63) and its upstream cell (lane 62).
+  const bool outlet_wave =
+      w.x0 + 64 == p.nx && (__builtin_amdgcn_readlane(kind, 63) == kEq);
   const uint32_t row = w.y + p.ny * w.z;
   RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
   if (outlet_wave) up = row_moments(edge_rows(in, p)[row]);
