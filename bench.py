@@ -109,7 +109,7 @@ def time_solver(grid, params, steps, warmup, repeats=3, closure=None):
             runs.append((ms, wall))
         solver.step(1, True)
         finite = bool(np.all(np.isfinite(solver.read_macro())))
-        variant = f"{solver.kernel}/{solver.block_size}"
+        variant = f"{solver.kernel}/{solver.block_size}/xcd-chunks-{solver.xcd_chunks}"
     finally:
         solver.destroy()
     runs.sort()

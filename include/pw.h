@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-#define PW_ABI_VERSION 3
+#define PW_ABI_VERSION 4
 
 /* Errors (src/lbm/gpu-resources.ts:3-27, gpu-force.ts:192-205). EINVAL is the reference's
  * RangeError, ENOMEM its AllocationFailure, EBUSY its 'busy' readback. */
@@ -141,13 +141,16 @@ pw_status pw_solver_field_vtk_blocks(pw_solver *s, double velocity_scale, void *
                                      void *velocity_be, size_t velocity_bytes);
 
 /* Device views for co-resident kernels (particles, interop). Pointers stay valid until
- * destroy; `populations` is the latest state and changes after every step. */
+ * destroy; `populations` is the latest state and changes after every step. On the device each
+ * x-row keeps its 19 directions as runs of R = run_elements cells (64 when nx is a multiple of
+ * 64, else nx): element (i, x, y, z) lies at i R + x + 18 R floor(x / R) + 19 nx (y + ny z).
+ * The readbacks above return the reference's planes instead. */
 typedef struct {
   const void *populations;
   const float *macro;   /* float4 per cell */
   const uint32_t *geometry;
   void *stream;         /* hipStream_t of the handle */
-  size_t plane_elements; /* population plane i starts at element i x plane_elements */
+  size_t run_elements;  /* R */
 } pw_device_state;
 pw_status pw_solver_device_state(pw_solver *s, pw_device_state *out);
 
@@ -167,6 +170,9 @@ pw_status pw_solver_set_kernel(pw_solver *s, uint32_t kernel);
 uint32_t pw_solver_kernel(const pw_solver *s);
 /* Threads per block of the step launches. */
 uint32_t pw_solver_block_size(const pw_solver *s);
+/* Chunks of the lattice each of the 8 XCDs steps (1: one contiguous eighth each; 16: chunks
+ * spread over the lattice), chosen with the variant by AUTO. */
+uint32_t pw_solver_xcd_chunks(const pw_solver *s);
 
 /* The validation stepper stepWithForces (src/app/case-run.ts:154-169) without a host round
  * trip per sample: queue `count` steps and, after each step whose count is a multiple of

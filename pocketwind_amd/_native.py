@@ -31,7 +31,7 @@ class ForceSample(C.Structure):
 
 class DeviceState(C.Structure):
     _fields_ = [("populations", C.c_void_p), ("macro", C.c_void_p), ("geometry", C.c_void_p),
-                ("stream", C.c_void_p), ("plane_elements", C.c_size_t)]
+                ("stream", C.c_void_p), ("run_elements", C.c_size_t)]
 
 
 _u32p = C.POINTER(C.c_uint32)
@@ -68,6 +68,7 @@ SIGNATURES = {
     "pw_solver_field_vtk_blocks": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_size_t, C.c_void_p,
                                              C.c_size_t]),
     "pw_solver_block_size": (C.c_uint32, [C.c_void_p]),
+    "pw_solver_xcd_chunks": (C.c_uint32, [C.c_void_p]),
     "pw_solver_kernel": (C.c_uint32, [C.c_void_p]),
     "pw_solver_synchronize": (C.c_int, [C.c_void_p]),
     "pw_solver_step_sampled": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t]),

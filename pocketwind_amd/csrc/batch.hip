@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(const Tunnel* __rest
     if (n >= tu.p.cells) continue;
     const uint32_t kind = tu.geo[n] & kTypeMask;
     const float ux = is_wall(kind) ? 0.0f : tu.p.u_in;
-    store_equilibrium<S>((typename S::T*)tu.pops[set], tu.p.plane, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
+    store_equilibrium<S>((typename S::T*)tu.pops[set], tu.p, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
     tu.macro[n] = make_float4(1.0f, ux, 0.0f, 0.0f);
   }
 }
@@ -56,10 +56,11 @@ __global__ __launch_bounds__(kForceBlock) void batch_force_kernel(const Tunnel* 
     const StepParams& p = tu.p;
     uint32_t x, y, z;
     cell_coords(n, p, x, y, z);
-    const Neighbours<> nb(x, y, z, p);
+    const Neighbours nb(x, y, z, p);
+    const uint32_t self = pop_x(p, x) + pop_row(p, y + p.ny * z);
     double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
-    cell_exchange<S, BOUZIDI>((const typename S::T*)tu.pops[0], tu.geo, p, nb, n, flag, ax, ay, az);
-    cell_exchange<S, BOUZIDI>((const typename S::T*)tu.pops[1], tu.geo, p, nb, n, flag, bx, by, bz);
+    cell_exchange<S, BOUZIDI>((const typename S::T*)tu.pops[0], tu.geo, p, nb, n, self, flag, ax, ay, az);
+    cell_exchange<S, BOUZIDI>((const typename S::T*)tu.pops[1], tu.geo, p, nb, n, self, flag, bx, by, bz);
     fx = 0.5 * (ax + bx);
     fy = 0.5 * (ay + by);
     fz = 0.5 * (az + bz);
@@ -260,7 +261,7 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
     const size_t N = sps[t].cells;
     for (int k = 0; k < 2; ++k) {
       void* p = nullptr;
-      if ((e = hipMalloc(&p, (size_t)kQ * sps[t].plane * elem)) != hipSuccess) return fail(e, "batch populations");
+      if ((e = hipMalloc(&p, (size_t)kQ * sps[t].cells * elem)) != hipSuccess) return fail(e, "batch populations");
       b->allocations.push_back(p);
       tu.pops[k] = p;
     }
@@ -401,10 +402,7 @@ pw_status pw_batch_read_populations(pw_batch* b, uint32_t tunnel, void* out, siz
     return PW_EINVAL;
   }
   const StepParams& p = b->host[tunnel].p;
-  const size_t row = (size_t)p.cells * elem, pitch = (size_t)p.plane * elem;
-  hipError_t e = hipMemcpy2DAsync(out, row, b->host[tunnel].pops[b->parity], pitch, row, kQ, hipMemcpyDeviceToHost,
-                                  b->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  const hipError_t e = set_to_host_planes(b->host[tunnel].pops[b->parity], out, p, elem, b->stream);
   return e == hipSuccess ? PW_OK : device_error(e, "batch population readback");
 }
 

@@ -37,16 +37,18 @@ struct Norm {
 template <class S, bool BOUZIDI>
 __device__ __forceinline__ void gather_cell(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
                                             const StepParams& p, uint32_t n, float* g) {
-  const size_t P = p.plane;
+  const size_t P = p.run;
   const uint32_t flag = geo[n];
   uint32_t x, y, z;
   cell_coords(n, p, x, y, z);
-  const Neighbours<> nb(x, y, z, p);
-  g[0] = S::load(in + n);
+  const Neighbours nb(x, y, z, p);
+  const PopNeighbours<> pnb(x, y, z, p);
+  const uint32_t self = pnb.at(0, 0, 0);
+  g[0] = S::load(in + self);
 #pragma unroll
   for (int i = 1; i < kQ; ++i)
-    g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i)
-                             : S::load(in + (size_t)i * P + nb.index(-cx(i), -cy(i), -cz(i)));
+    g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, self, flag, i)
+                             : S::load(in + (size_t)i * P + pnb.at(-cx(i), -cy(i), -cz(i)));
 }
 
 __device__ __forceinline__ void velocity_of(const float* g, float& rho, float& ux, float& uy, float& uz) {
@@ -95,7 +97,7 @@ __device__ __forceinline__ void wall_features(const typename S::T* __restrict__ 
   if (dx | dy | dz) {
     uint32_t x, y, z;
     cell_coords(n, p, x, y, z);
-    const uint32_t m = Neighbours<>(x, y, z, p).index(dx, dy, dz);
+    const uint32_t m = Neighbours(x, y, z, p).index(dx, dy, dz);
     if ((geo[m] & kTypeMask) == kFluid) {
       float h[kQ], r2;
       gather_cell<S, BOUZIDI>(in, geo, p, m, h);
@@ -196,10 +198,11 @@ __global__ __launch_bounds__(256) void wall_force_kernel(const typename S::T* __
   const uint32_t flag = geo[n];
   uint32_t x, y, z;
   cell_coords(n, p, x, y, z);
-  const Neighbours<> nb(x, y, z, p);
+  const Neighbours nb(x, y, z, p);
+  const uint32_t self = pop_x(p, x) + pop_row(p, y + p.ny * z);
   double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
-  cell_exchange<S, BOUZIDI>(state_a, geo, p, nb, n, flag, ax, ay, az);
-  cell_exchange<S, BOUZIDI>(state_b, geo, p, nb, n, flag, bx, by, bz);
+  cell_exchange<S, BOUZIDI>(state_a, geo, p, nb, n, self, flag, ax, ay, az);
+  cell_exchange<S, BOUZIDI>(state_b, geo, p, nb, n, self, flag, bx, by, bz);
   out[3 * (size_t)k] = 0.5 * (ax + bx);
   out[3 * (size_t)k + 1] = 0.5 * (ay + by);
   out[3 * (size_t)k + 2] = 0.5 * (az + bz);

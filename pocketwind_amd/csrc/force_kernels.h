@@ -22,17 +22,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <class S, bool BOUZIDI>
 __device__ __forceinline__ void cell_exchange(const typename S::T* __restrict__ state,
                                               const uint32_t* __restrict__ geo, const StepParams& p,
-                                              const Neighbours<>& nb, uint32_t n, uint32_t flag,
+                                              const Neighbours& nb, uint32_t n, uint32_t self, uint32_t flag,
                                               double& fx, double& fy, double& fz) {
-  const size_t P = p.plane;
+  const size_t P = p.run;
 #pragma unroll
   for (int i = 1; i < kQ; ++i) {
     if (!has_wall(flag, i)) continue;
     const uint32_t source = nb.index(-cx(i), -cy(i), -cz(i));
     if ((geo[source] & kTypeMask) != kSolid) continue;
     const int j = opp(i);
-    const double outgoing = S::load(state + (size_t)j * P + n);
-    const double incoming = wall_value<S, BOUZIDI, double>(state, geo, p, nb, n, flag, i);
+    const double outgoing = S::load(state + (size_t)j * P + self);
+    const double incoming = wall_value<S, BOUZIDI, double>(state, geo, p, nb, n, self, flag, i);
     const double e = outgoing + incoming;
     fx += cx(j) * e;
     fy += cy(j) * e;
@@ -52,10 +52,11 @@ __global__ __launch_bounds__(kForceBlock) void force_partials_kernel(
     const uint32_t flag = geo[n];
     uint32_t x, y, z;
     cell_coords(n, p, x, y, z);
-    const Neighbours<> nb(x, y, z, p);
+    const Neighbours nb(x, y, z, p);
+    const uint32_t self = pop_x(p, x) + pop_row(p, y + p.ny * z);
     double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
-    cell_exchange<S, BOUZIDI>(state_a, geo, p, nb, n, flag, ax, ay, az);
-    cell_exchange<S, BOUZIDI>(state_b, geo, p, nb, n, flag, bx, by, bz);
+    cell_exchange<S, BOUZIDI>(state_a, geo, p, nb, n, self, flag, ax, ay, az);
+    cell_exchange<S, BOUZIDI>(state_b, geo, p, nb, n, self, flag, bx, by, bz);
     fx = 0.5 * (ax + bx);
     fy = 0.5 * (ay + by);
     fz = 0.5 * (az + bz);

@@ -23,13 +23,14 @@ __host__ __device__ __forceinline__ uint32_t fast_div(uint32_t n, const FastDiv&
   return (t + ((n - t) >> 1)) >> f.s;
 }
 
-// Padding between population planes. With 38 streams in flight per step, plane strides that
-// are large powers of two put every stream on the same HBM channels at once; these pads (the
-// fastest of a sweep of a 19-plane stream copy on MI355X) break the alignment.
-inline uint32_t plane_elements(uint32_t cells, uint32_t elem_bytes) {
-  const uint32_t pad_bytes = elem_bytes == 4 ? 4352u : 1024u;
-  return cells + pad_bytes / elem_bytes;
-}
+// Population layout. Each x-row holds its 19 directions as runs of R cells: R = 64 when nx is
+// a multiple of 64 (a run is one wavefront's cells), else R = nx. Element (i, x, y, z) of a
+// set lies at
+//   i R + x + 18 R floor(x / R) + 19 nx (y + ny z),
+// so one direction of a wavefront is one aligned run and a step writes each row's 19 runs
+// contiguously, instead of streaming 19 planes that lie far apart (a row-wavefront pull copy
+// measured about 16 % faster this way on MI355X). A set is exactly 19 N elements.
+inline uint32_t pop_run(uint32_t nx) { return nx % 64 == 0 ? 64u : nx; }
 
 inline FastDiv make_fast_div(uint32_t d) {
   if (d <= 1) return FastDiv{1, 0, 0};
@@ -45,7 +46,8 @@ inline FastDiv make_fast_div(uint32_t d) {
 struct StepParams {
   uint32_t nx, ny, nz;
   uint32_t cells;
-  uint32_t plane;  // elements from one population plane to the next (cells plus padding)
+  uint32_t run;     // cells per direction run (64, or nx when nx is not a multiple of 64)
+  uint32_t x_tile;  // 18 R when R = 64, else 0: element of x = x + x_tile floor(x / 64)
   float u_in;      // inlet speed this step (ramped)
   float nu;        // f32 viscosity, (tau0 - 1/2) / 3 rounded as the reference packs it
   float tau0;      // 3 nu + 1/2 evaluated in f32, as relaxation_time does
@@ -125,4 +127,5 @@ struct pw_solver {
   uint32_t* guard_host = nullptr;       // pinned readback
   uint32_t kernel = 1;  // pw_kernel actually used by step()
   uint32_t block = 256;  // threads per step block
+  uint32_t xcd_chunks = 1;  // chunks of the lattice per XCD in the step launch (1 or 16)
 };

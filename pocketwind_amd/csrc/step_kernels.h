@@ -277,18 +277,44 @@ __host__ __device__ __forceinline__ void cell_coords(uint32_t n, const StepParam
   y = r - z * p.ny;
 }
 
-// Wrapped neighbour offsets of a cell, scaled by 2^SHIFT (element bytes for the streaming
-// loads, 1 for flat indices): the offset of (x + dx, y + dy, z + dz) is
-// xo[dx+1] + yo[dy+1] + zo[dz+1] with every axis periodic (common.wgsl:64-68). Population
-// planes are below 4 GiB (the cell count is checked at create), so a byte offset fits a u32.
-template <int SHIFT = 0>
+// Wrapped flat indices of a cell's neighbours (cell arrays such as the geometry): the index of
+// (x + dx, y + dy, z + dz) is xo[dx+1] + yo[dy+1] + zo[dz+1] with every axis periodic
+// (common.wgsl:64-68).
 struct Neighbours {
   uint32_t xo[3], yo[3], zo[3];
   __host__ __device__ __forceinline__ Neighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p) {
     const uint32_t row = p.nx, plane = p.nx * p.ny;
-    xo[0] = (x == 0 ? p.nx - 1 : x - 1) << SHIFT;
-    xo[1] = x << SHIFT;
-    xo[2] = (x + 1 == p.nx ? 0 : x + 1) << SHIFT;
+    xo[0] = x == 0 ? p.nx - 1 : x - 1;
+    xo[1] = x;
+    xo[2] = x + 1 == p.nx ? 0 : x + 1;
+    yo[0] = (y == 0 ? p.ny - 1 : y - 1) * row;
+    yo[1] = y * row;
+    yo[2] = (y + 1 == p.ny ? 0 : y + 1) * row;
+    zo[0] = (z == 0 ? p.nz - 1 : z - 1) * plane;
+    zo[1] = z * plane;
+    zo[2] = (z + 1 == p.nz ? 0 : z + 1) * plane;
+  }
+  __host__ __device__ __forceinline__ uint32_t index(int dx, int dy, int dz) const {
+    return xo[dx + 1] + yo[dy + 1] + zo[dz + 1];
+  }
+};
+
+// Population element offsets (direction 0) of the layout described with StepParams::run:
+// x + x_tile floor(x / 64) within a row, 19 nx per row.
+__host__ __device__ __forceinline__ uint32_t pop_x(const StepParams& p, uint32_t x) { return x + p.x_tile * (x >> 6); }
+__host__ __device__ __forceinline__ uint32_t pop_row(const StepParams& p, uint32_t r) { return r * (kQ * p.nx); }
+
+// The same for wrapped neighbours, scaled by 2^SHIFT (element bytes for byte addressing). The
+// layout is additive in x and in the row, and a set is below 4 GiB (checked at create), so a
+// byte offset fits a u32.
+template <int SHIFT = 0>
+struct PopNeighbours {
+  uint32_t xo[3], yo[3], zo[3];
+  __host__ __device__ __forceinline__ PopNeighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p) {
+    const uint32_t row = kQ * p.nx, plane = row * p.ny;
+    xo[0] = pop_x(p, x == 0 ? p.nx - 1 : x - 1) << SHIFT;
+    xo[1] = pop_x(p, x) << SHIFT;
+    xo[2] = pop_x(p, x + 1 == p.nx ? 0 : x + 1) << SHIFT;
     yo[0] = ((y == 0 ? p.ny - 1 : y - 1) * row) << SHIFT;
     yo[1] = (y * row) << SHIFT;
     yo[2] = ((y + 1 == p.ny ? 0 : y + 1) * row) << SHIFT;
@@ -299,8 +325,6 @@ struct Neighbours {
   __host__ __device__ __forceinline__ uint32_t at(int dx, int dy, int dz) const {
     return xo[dx + 1] + yo[dy + 1] + zo[dz + 1];
   }
-  // Flat cell index of a neighbour.
-  __host__ __device__ __forceinline__ uint32_t index(int dx, int dy, int dz) const { return at(dx, dy, dz) >> SHIFT; }
 };
 
 template <class T>
@@ -313,14 +337,15 @@ __host__ __device__ __forceinline__ T* byte_offset(T* base, uint32_t bytes) {
 }
 
 // walls.wgsl:6-21: halfway bounce-back or linear Bouzidi for incoming direction i, evaluated
-// in R (f32 inside the step, f64 inside the drag reduction).
-template <class S, bool BOUZIDI, class R = float, int SHIFT>
+// in R (f32 inside the step, f64 inside the drag reduction). n is the cell's flat index, self
+// its population element offset and nb its flat neighbours.
+template <class S, bool BOUZIDI, class R = float>
 __host__ __device__ __forceinline__ R wall_value(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
-                                                 const StepParams& p, const Neighbours<SHIFT>& nb, uint32_t n,
+                                                 const StepParams& p, const Neighbours& nb, uint32_t n, uint32_t self,
                                                  uint32_t flag, int i) {
-  const size_t N = p.cells, P = p.plane;
+  const size_t N = p.cells, P = p.run;
   const int j = opp(i);
-  const R reflected = S::load(in + (size_t)j * P + n);
+  const R reflected = S::load(in + (size_t)j * P + self);
   if (!BOUZIDI) return reflected;
   const uint32_t base = geo[N + n];
   if (base == kNoQ) return reflected;
@@ -330,13 +355,21 @@ __host__ __device__ __forceinline__ R wall_value(const typename S::T* __restrict
   if (q < R(0.5)) {
     const uint32_t second = nb.index(cx(i), cy(i), cz(i));
     if ((geo[second] & kTypeMask) != kFluid) return reflected;
-    return R(2) * q * reflected + (R(1) - R(2) * q) * R(S::load(in + (size_t)j * P + second));
+    const uint32_t r = fast_div(second, p.div_nx);
+    const uint32_t second_elem = pop_x(p, second - r * p.nx) + pop_row(p, r);
+    return R(2) * q * reflected + (R(1) - R(2) * q) * R(S::load(in + (size_t)j * P + second_elem));
   }
-  return (reflected + (R(2) * q - R(1)) * R(S::load(in + (size_t)i * P + n))) / (R(2) * q);
+  return (reflected + (R(2) * q - R(1)) * R(S::load(in + (size_t)i * P + self))) / (R(2) * q);
 }
 
-// Stores of one cell's 19 values; P is the plane stride, `self` the cell's byte offset in a
-// plane and h its random word for this step.
+// Population element offset of a flat cell index.
+__host__ __device__ __forceinline__ uint32_t pop_cell(const StepParams& p, uint32_t n) {
+  const uint32_t r = fast_div(n, p.div_nx);
+  return pop_x(p, n - r * p.nx) + pop_row(p, r);
+}
+
+// Stores of one cell's 19 values; P is the direction stride (StepParams::run), `self` the
+// cell's byte offset within direction 0 and h its random word for this step.
 template <class S, bool NT = false>
 __host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__ out, size_t P, uint32_t self,
                                                      const float* g, uint32_t h) {
@@ -364,11 +397,75 @@ __host__ __device__ __forceinline__ void equilibrium_values(float ux, float uy, 
 }
 
 template <class S, bool NT = false>
-__host__ __device__ __forceinline__ void store_equilibrium(typename S::T* __restrict__ out, size_t P, uint32_t n,
-                                                           float ux, float uy, float uz, uint32_t h) {
+__host__ __device__ __forceinline__ void store_equilibrium(typename S::T* __restrict__ out, const StepParams& p,
+                                                           uint32_t n, float ux, float uy, float uz, uint32_t h) {
   float g[kQ];
   equilibrium_values(ux, uy, uz, g);
-  store_state<S, NT>(out, P, n << S::kShift, g, h);
+  store_state<S, NT>(out, p.run, pop_cell(p, n) << S::kShift, g, h);
+}
+
+// Between a set in the population layout and the reference's planes (19 planes of N cells,
+// direction-major; docs/numerics.md "Population storage"). T holds the raw storage bits.
+template <class T>
+__global__ __launch_bounds__(256) void to_planes_kernel(const T* __restrict__ set, T* __restrict__ planes,
+                                                        StepParams p) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.cells) return;
+  const uint32_t e = pop_cell(p, n);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) planes[(size_t)i * p.cells + n] = set[(size_t)i * p.run + e];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void from_planes_kernel(const T* __restrict__ planes, T* __restrict__ set,
+                                                          StepParams p) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.cells) return;
+  const uint32_t e = pop_cell(p, n);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) set[(size_t)i * p.run + e] = planes[(size_t)i * p.cells + n];
+}
+
+// Copy a set out to (or in from) host planes through a device staging buffer, on `stream`.
+inline hipError_t set_to_host_planes(const void* set, void* host, const StepParams& p, uint32_t elem, hipStream_t stream) {
+  const size_t bytes = (size_t)kQ * p.cells * elem;
+  void* stage = nullptr;
+  hipError_t e = hipMallocAsync(&stage, bytes, stream);
+  if (e != hipSuccess) return e;
+  const unsigned blocks = (unsigned)((p.cells + 255) / 256);
+  if (elem == 4)
+    hipLaunchKernelGGL(to_planes_kernel<uint32_t>, dim3(blocks), dim3(256), 0, stream, (const uint32_t*)set,
+                       (uint32_t*)stage, p);
+  else
+    hipLaunchKernelGGL(to_planes_kernel<uint16_t>, dim3(blocks), dim3(256), 0, stream, (const uint16_t*)set,
+                       (uint16_t*)stage, p);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(host, stage, bytes, hipMemcpyDeviceToHost, stream);
+  (void)hipFreeAsync(stage, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  return e;
+}
+
+inline hipError_t set_from_host_planes(void* set, const void* host, const StepParams& p, uint32_t elem,
+                                       hipStream_t stream) {
+  const size_t bytes = (size_t)kQ * p.cells * elem;
+  void* stage = nullptr;
+  hipError_t e = hipMallocAsync(&stage, bytes, stream);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, stream);
+  const unsigned blocks = (unsigned)((p.cells + 255) / 256);
+  if (e == hipSuccess) {
+    if (elem == 4)
+      hipLaunchKernelGGL(from_planes_kernel<uint32_t>, dim3(blocks), dim3(256), 0, stream, (const uint32_t*)stage,
+                         (uint32_t*)set, p);
+    else
+      hipLaunchKernelGGL(from_planes_kernel<uint16_t>, dim3(blocks), dim3(256), 0, stream, (const uint16_t*)stage,
+                         (uint16_t*)set, p);
+    e = hipGetLastError();
+  }
+  (void)hipFreeAsync(stage, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  return e;
 }
 
 // Walls keep the rest state (all deviations zero; stream_collide.wgsl:96 skips them). They
@@ -394,7 +491,7 @@ __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ o
   // whose direction-0 term could come out as -0 depending on how the compiler folds it.
   if (wall) rest_state(g);
   else equilibrium_values(p.u_in, 0.0f, 0.0f, g);
-  store_state<S, false>(out, p.plane, n << S::kShift, g, cell_random(n, kInitSeed));
+  store_state<S, false>(out, p.run, pop_cell(p, n) << S::kShift, g, cell_random(n, kInitSeed));
   macro[n] = make_float4(1.0f, wall ? 0.0f : p.u_in, 0.0f, 0.0f);
 }
 
@@ -426,29 +523,30 @@ __host__ __device__ __forceinline__ void add_moment(RowMoments& m, int i, float 
 }
 
 template <class S>
-__host__ __device__ __forceinline__ RowMoments stored_moments(const typename S::T* __restrict__ in, size_t P,
-                                                              uint32_t n) {
+__host__ __device__ __forceinline__ RowMoments stored_moments(const typename S::T* __restrict__ in,
+                                                              const StepParams& p, uint32_t n) {
   RowMoments m{1.0f, 0.0f, 0.0f, 0.0f};
+  const uint32_t e = pop_cell(p, n);
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) add_moment(m, i, S::load(in + (size_t)i * P + n));
+  for (int i = 0; i < kQ; ++i) add_moment(m, i, S::load(in + (size_t)i * p.run + e));
   return m;
 }
 
 // The step that stores a row's x = nx - 2 state also stores those moments, one float4 per row
-// (y + ny z) in the set's allocation after its 19 planes; the outlet reads that one value
+// (y + ny z) in the set's allocation after its 19 N elements; the outlet reads that one value
 // instead of gathering 19 at the end of its wavefront, whose latency nothing would hide.
-__host__ __device__ __forceinline__ size_t edge_offset_bytes(size_t plane, size_t elem_bytes) {
-  return (kQ * plane * elem_bytes + 255) & ~(size_t)255;
+__host__ __device__ __forceinline__ size_t edge_offset_bytes(size_t cells, size_t elem_bytes) {
+  return (kQ * cells * elem_bytes + 255) & ~(size_t)255;
 }
 
 template <class T>
 __host__ __device__ __forceinline__ float4* edge_rows(T* set, const StepParams& p) {
-  return reinterpret_cast<float4*>(reinterpret_cast<char*>(set) + edge_offset_bytes(p.plane, sizeof(T)));
+  return reinterpret_cast<float4*>(reinterpret_cast<char*>(set) + edge_offset_bytes(p.cells, sizeof(T)));
 }
 
 template <class T>
 __host__ __device__ __forceinline__ const float4* edge_rows(const T* set, const StepParams& p) {
-  return reinterpret_cast<const float4*>(reinterpret_cast<const char*>(set) + edge_offset_bytes(p.plane, sizeof(T)));
+  return reinterpret_cast<const float4*>(reinterpret_cast<const char*>(set) + edge_offset_bytes(p.cells, sizeof(T)));
 }
 
 __host__ __device__ __forceinline__ RowMoments row_moments(const float4& e) { return RowMoments{e.x, e.y, e.z, e.w}; }
@@ -482,7 +580,7 @@ template <class S>
 __host__ __device__ __forceinline__ float4 eq_state(const typename S::T* __restrict__ in, const StepParams& p,
                                                     uint32_t n, uint32_t x, float* g) {
   RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
-  if (x == p.nx - 1 && x > 0) up = stored_moments<S>(in, p.plane, n - 1);
+  if (x == p.nx - 1 && x > 0) up = stored_moments<S>(in, p, n - 1);
   return eq_from(p, x, up, g);
 }
 
@@ -491,7 +589,7 @@ template <class S>
 __global__ __launch_bounds__(256) void edge_moments_kernel(typename S::T* __restrict__ set, StepParams p) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= p.ny * p.nz || p.nx < 2) return;
-  const RowMoments m = stored_moments<S>(set, p.plane, r * p.nx + (p.nx - 2));
+  const RowMoments m = stored_moments<S>(set, p, r * p.nx + (p.nx - 2));
   edge_rows(set, p)[r] = make_float4(m.mass, m.mx, m.my, m.mz);
 }
 
@@ -519,26 +617,27 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
                                                              const uint32_t* __restrict__ geo,
                                                              float4* __restrict__ macro,
                                                              const StepParams& p, uint32_t n, uint32_t seed) {
-  const size_t P = p.plane;
+  const size_t P = p.run;
   const uint32_t flag = geo[n];
   const uint32_t kind = flag & kTypeMask;
   uint32_t x, y, z;
   cell_coords(n, p, x, y, z);
-  const uint32_t self = n << S::kShift;
+  const PopNeighbours<S::kShift> pnb(x, y, z, p);
+  const uint32_t self = pnb.at(0, 0, 0);
   float g[kQ];
   float4 m;
   if (kind == kFluid) {
-    const Neighbours<S::kShift> nb(x, y, z, p);
     g[0] = load_pop<S, POL == kNtAll>(byte_offset(in, self));
     if ((flag >> 3) == 0) {
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
-        g[i] = load_pop<S, POL == kNtAll>(byte_offset(in + (size_t)i * P, nb.at(-cx(i), -cy(i), -cz(i))));
+        g[i] = load_pop<S, POL == kNtAll>(byte_offset(in + (size_t)i * P, pnb.at(-cx(i), -cy(i), -cz(i))));
     } else {
+      const Neighbours nb(x, y, z, p);
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
-        g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i)
-                                 : S::load(byte_offset(in + (size_t)i * P, nb.at(-cx(i), -cy(i), -cz(i))));
+        g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, self >> S::kShift, flag, i)
+                                 : S::load(byte_offset(in + (size_t)i * P, pnb.at(-cx(i), -cy(i), -cz(i))));
     }
     m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
@@ -587,6 +686,7 @@ __device__ __forceinline__ float decode_bits(uint32_t b) {
 // byte offset alone.
 struct RowWave {
   uint32_t n0, x0, y, z;
+  uint32_t e0;  // population element offset of n0 (its run starts there: x0 is a multiple of 64)
 };
 
 // Gather of a row wavefront. Each plane is read with aligned, unit-stride loads of the pull's
@@ -598,20 +698,20 @@ template <class S, bool NT>
 __device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in, const StepParams& p,
                                            const RowWave& w, uint32_t lane, float* g) {
   constexpr int SH = S::kShift;
-  const size_t P = p.plane;
-  const uint32_t row = p.nx, plane = p.nx * p.ny;
+  const size_t P = p.run;  // 64: one wavefront's run
+  const uint32_t row = kQ * p.nx, plane = row * p.ny;
   const uint32_t y = w.y, z = w.z, x0 = w.x0;
   const uint32_t yo[3] = {((y == 0 ? p.ny - 1 : y - 1) * row) << SH, (y * row) << SH,
                           ((y + 1 == p.ny ? 0 : y + 1) * row) << SH};
   const uint32_t zo[3] = {((z == 0 ? p.nz - 1 : z - 1) * plane) << SH, (z * plane) << SH,
                           ((z + 1 == p.nz ? 0 : z + 1) * plane) << SH};
-  const uint32_t left = (x0 == 0 ? p.nx - 1 : x0 - 1) << SH;     // source of lane 0 when c_x = +1
-  const uint32_t right = (x0 + 64 == p.nx ? 0 : x0 + 64) << SH;  // source of lane 63 when c_x = -1
-  const uint32_t xs = lane << SH;
+  const uint32_t left = pop_x(p, x0 == 0 ? p.nx - 1 : x0 - 1) << SH;     // source of lane 0 when c_x = +1
+  const uint32_t right = pop_x(p, x0 + 64 == p.nx ? 0 : x0 + 64) << SH;  // source of lane 63 when c_x = -1
+  const uint32_t xs = (pop_x(p, x0) + lane) << SH;
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
     const typename S::T* src = byte_offset(in + (size_t)i * P, yo[1 - cy(i)] + zo[1 - cz(i)]);
-    const uint32_t own = load_bits<S, NT>(byte_offset(src, (x0 << SH) + xs));
+    const uint32_t own = load_bits<S, NT>(byte_offset(src, xs));
     if (cx(i) == 0) {
       g[i] = decode_bits<S>(own);
     } else {
@@ -627,8 +727,8 @@ __device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in,
 // the compiler addresses each store as scalar base + lane offset rather than chaining 64-bit
 // vector adds from one plane to the next.
 template <class T>
-__device__ __forceinline__ __attribute__((address_space(1))) T* plane_base(T* out, size_t P, int i, uint32_t n0) {
-  T* p = out + (size_t)i * P + n0;
+__device__ __forceinline__ __attribute__((address_space(1))) T* plane_base(T* out, size_t P, int i, uint32_t e0) {
+  T* p = out + (size_t)i * P + e0;
   asm("" : "+s"(p));
   return (__attribute__((address_space(1))) T*)p;
 }
@@ -637,14 +737,14 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* plane_base(T* ou
 // into one register and store its halves.
 // With `moments` (wavefront-uniform) the stored values' moments accumulate into m.
 template <class S, bool NT>
-__device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_t P, uint32_t n0, uint32_t lane,
+__device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_t P, uint32_t e0, uint32_t lane,
                                           const float* g, uint32_t h, bool moments, RowMoments& m) {
   using T = typename S::T;
   if constexpr (sizeof(T) == 4) {
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
       const float v = S::encode(g[i], direction_random(h, i));
-      auto* b = plane_base(out, P, i, n0);
+      auto* b = plane_base(out, P, i, e0);
       if constexpr (NT) __builtin_nontemporal_store(v, b + lane);
       else b[lane] = v;
       if (moments) add_moment(m, i, v);
@@ -652,7 +752,7 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_
   } else {
     uint16_t* o = reinterpret_cast<uint16_t*>(out);
     const uint16_t v0 = __builtin_bit_cast(uint16_t, S::encode(g[0], direction_random(h, 0)));
-    auto* b0 = plane_base(o, P, 0, n0);
+    auto* b0 = plane_base(o, P, 0, e0);
     if constexpr (NT) __builtin_nontemporal_store(v0, b0 + lane);
     else b0[lane] = v0;
     if (moments) add_moment(m, 0, decode_bits<S>(v0));
@@ -664,8 +764,8 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_
         add_moment(m, i, decode_bits<S>(v & 0xffffu));
         add_moment(m, i + 1, decode_bits<S>(v >> 16));
       }
-      auto* bi = plane_base(o, P, i, n0);
-      auto* bj = plane_base(o, P, i + 1, n0);
+      auto* bi = plane_base(o, P, i, e0);
+      auto* bj = plane_base(o, P, i + 1, e0);
       if constexpr (NT) {
         __builtin_nontemporal_store((uint16_t)(v & 0xffffu), bi + lane);
         __builtin_nontemporal_store((uint16_t)(v >> 16), bj + lane);
@@ -693,6 +793,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
   RowWave w;
   w.n0 = __builtin_amdgcn_readfirstlane(n - lane);
   cell_coords(w.n0, p, w.x0, w.y, w.z);
+  w.e0 = pop_x(p, w.x0) + pop_row(p, w.y + p.ny * w.z);
   // The wavefront holding an EQ outlet cell (lane 63) and its upstream cell (lane 62).
   const bool outlet_wave =
       w.x0 + 64 == p.nx && (__builtin_amdgcn_readlane(kind, 63) == kEq);
@@ -705,10 +806,10 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
   float4 m;
   if (kind == kFluid) {
     if ((flag >> 3) != 0) {
-      const Neighbours<> nb(x, w.y, w.z, p);
+      const Neighbours nb(x, w.y, w.z, p);
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
-        if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, flag, i);
+        if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, w.e0 + lane, flag, i);
     }
     m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
@@ -718,7 +819,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
   RowMoments stored{1.0f, 0.0f, 0.0f, 0.0f};
-  store_row<S, POL != kCached>(out, p.plane, w.n0, lane, g, cell_random(n, seed), outlet_wave, stored);
+  store_row<S, POL != kCached>(out, p.run, w.e0, lane, g, cell_random(n, seed), outlet_wave, stored);
   if (outlet_wave && lane == 62) edge_rows(out, p)[row] = make_float4(stored.mass, stored.mx, stored.my, stored.mz);
   if (MACRO && !is_wall(kind)) {
     macro[n] = m;
@@ -727,8 +828,10 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
 }
 
 // Blocks are renumbered so that each of the 8 XCDs (which receive blocks round-robin) steps
-// one contiguous eighth of the lattice: the cache lines that neighbouring rows and
-// wavefronts share stay in one XCD's L2. The grid holds 8 per_xcd blocks.
+// contiguous chunks of `chunk` blocks: XCD k takes chunks k, k + 8, k + 16, ... The cache lines
+// that neighbouring rows and wavefronts share stay in one XCD's L2 within a chunk. One chunk per
+// XCD (an eighth of the lattice each) suits uniform lattices; smaller chunks spread a body's
+// wall cells, and their slower wavefronts, over all XCDs. The grid holds a multiple of 8 chunks.
 constexpr uint32_t kXcds = 8;
 
 // Waves per SIMD the step kernel is compiled for (a register budget of 512 / waves VGPRs).
@@ -743,9 +846,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
                                                              float4* __restrict__ macro, StepParams p,
-                                                             uint32_t seed, uint32_t per_xcd) {
+                                                             uint32_t seed, uint32_t chunk) {
   const uint32_t b = blockIdx.x;
-  const uint32_t logical = (b % kXcds) * per_xcd + b / kXcds;
+  const uint32_t local = b / kXcds;
+  const uint32_t logical = ((local / chunk) * kXcds + b % kXcds) * chunk + local % chunk;
   const uint32_t n = logical * blockDim.x + threadIdx.x;
   if (n >= p.cells) return;  // whole wavefronts when nx is a multiple of 64
   if (p.nx % 64 == 0)

@@ -25,9 +25,13 @@ void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const St
   auto k = macro ? pick(std::true_type{}) : pick(std::false_type{});
   const uint32_t block = s->block;
   const uint64_t blocks = (s->cells + block - 1) / block;
-  const uint32_t per_xcd = (uint32_t)((blocks + kXcds - 1) / kXcds);
+  // Each XCD takes s->xcd_chunks chunks; the grid is rounded up to whole chunks (the extra
+  // blocks fall beyond the lattice and return at once).
+  const uint32_t parts = s->xcd_chunks;
+  const uint32_t per_xcd = (uint32_t)(((blocks + kXcds - 1) / kXcds + parts - 1) / parts * parts);
+  const uint32_t chunk = per_xcd / parts;
   hipLaunchKernelGGL(k, dim3(kXcds * per_xcd), dim3(block), 0, s->stream, (const typename S::T*)in,
-                     (typename S::T*)out, s->geometry, s->macro, p, seed, per_xcd);
+                     (typename S::T*)out, s->geometry, s->macro, p, seed, chunk);
 }
 
 template <class S>

@@ -239,6 +239,10 @@ class GpuSolver:
     def block_size(self) -> int:
         return int(N.lib().pw_solver_block_size(self._live()))
 
+    @property
+    def xcd_chunks(self) -> int:
+        return int(N.lib().pw_solver_xcd_chunks(self._live()))
+
     # Learned wall closure (no reference counterpart; see include/pw.h).
     def wall_cells(self) -> np.ndarray:
         n = int(N.lib().pw_solver_wall_cell_count(self._live()))
