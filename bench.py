@@ -122,14 +122,16 @@ def time_solver(grid, params, steps, warmup, repeats=3, closure=None):
 
 
 def measured_traffic(precision: str, variant: str):
-    """HBM bytes per launch that rocprof counters gave for this storage and step variant
-    (profiles/step_traffic.json), or None when that variant was not profiled."""
+    """HBM bytes per launch that rocprof counters gave for this storage, cache policy and XCD
+    chunking (profiles/step_traffic.json), or None when that variant was not profiled. The
+    block size is left out of the key: it changes the traffic by well under a percent."""
     path = os.path.join(ROOT, "profiles", "step_traffic.json")
     try:
         table = json.load(open(path))["bytes_per_launch"]
     except (OSError, ValueError, KeyError):
         return None
-    return table.get(f"{precision}/{variant.split('/')[0]}")
+    kernel, _, chunking = variant.split("/")
+    return table.get(f"{precision}/{kernel}/{chunking}")
 
 
 def body_cd(grid, params, closure=None):

@@ -216,6 +216,27 @@ void launch_force(pw_solver* s, double* dest) {
 // Time each step variant at each block size on the handle's own buffers and keep the
 // fastest. The caller re-initialises (or restores) the populations afterwards, so tuning
 // leaves no trace in the state.
+// Parse a PW_STEP_KERNEL value into the solver's variant; false when absent or malformed.
+bool forced_variant(pw_solver* s, const char* text) {
+  if (!text) return false;
+  const std::string v(text);
+  const size_t a = v.find('/'), b = a == std::string::npos ? a : v.find('/', a + 1);
+  const std::string name = v.substr(0, a);
+  uint32_t kernel = 0;
+  if (name == "cell") kernel = PW_KERNEL_CELL;
+  else if (name == "cell-nt") kernel = PW_KERNEL_CELL_NT;
+  else if (name == "cell-nt-stores") kernel = PW_KERNEL_CELL_NT_STORES;
+  else return false;
+  uint32_t block = s->block, chunks = s->xcd_chunks;
+  if (a != std::string::npos) block = (uint32_t)std::strtoul(v.c_str() + a + 1, nullptr, 10);
+  if (b != std::string::npos) chunks = (uint32_t)std::strtoul(v.c_str() + b + 1, nullptr, 10);
+  if ((block != 64 && block != 128 && block != 256) || (chunks != 1 && chunks != 16)) return false;
+  s->kernel = kernel;
+  s->block = block;
+  s->xcd_chunks = chunks;
+  return true;
+}
+
 hipError_t autotune(pw_solver* s) {
   const uint32_t kernels[3] = {PW_KERNEL_CELL, PW_KERNEL_CELL_NT, PW_KERNEL_CELL_NT_STORES};
   const uint32_t blocks[3] = {64, 128, 256};
@@ -430,11 +451,10 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   if (n_boundary && (e = hipMemcpyAsync(s->boundary, boundary_cells, n_boundary * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "boundary upload"));
   if (s->n_wall && (e = hipMemcpyAsync(s->wall_cells, wall_list.data(), s->n_wall * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "wall cell upload"));
   launch_init_both(s, full);
-  const char* forced = std::getenv("PW_STEP_KERNEL");
-  if (forced && std::strcmp(forced, "cell") == 0) s->kernel = PW_KERNEL_CELL;
-  else if (forced && std::strcmp(forced, "cell-nt") == 0) s->kernel = PW_KERNEL_CELL_NT;
-  else if (forced && std::strcmp(forced, "cell-nt-stores") == 0) s->kernel = PW_KERNEL_CELL_NT_STORES;
-  else if ((e = autotune(s)) != hipSuccess) return fail(device_error(e, "step autotune"));
+  // PW_STEP_KERNEL=<cell|cell-nt|cell-nt-stores>[/<threads per block>[/<chunks per XCD>]]
+  // fixes the step variant instead of tuning it (profiling runs, reproducible timings).
+  if (!forced_variant(s, std::getenv("PW_STEP_KERNEL")) && (e = autotune(s)) != hipSuccess)
+    return fail(device_error(e, "step autotune"));
   launch_init_both(s, sp);
   if ((e = hipGetLastError()) != hipSuccess) return fail(device_error(e, "init launch"));
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return fail(device_error(e, "create"));
