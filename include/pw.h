@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-#define PW_ABI_VERSION 4
+#define PW_ABI_VERSION 5
 
 /* Errors (src/lbm/gpu-resources.ts:3-27, gpu-force.ts:192-205). EINVAL is the reference's
  * RangeError, ENOMEM its AllocationFailure, EBUSY its 'busy' readback. */
@@ -44,6 +44,18 @@ typedef enum { PW_STORAGE_F32 = 0, PW_STORAGE_F16 = 1, PW_STORAGE_BF16 = 2 } pw_
  * (src/lbm/shaders/stream_collide.wgsl:8-80), or the learned MLP closure loaded with
  * pw_solver_set_wall_closure, which falls back to the wall law outside its training range. */
 typedef enum { PW_WALL_LAW = 0, PW_WALL_LEARNED = 1 } pw_wall_model;
+/* Side boundaries y = 0, ny - 1 and z = 0, nz - 1: periodic as in the reference
+ * (src/lbm/shaders/common.wgsl:62-68), or free-slip: halfway specular reflection, a pull that
+ * would cross a side reads the mirrored direction of the source cell unshifted in that axis.
+ * Free-slip sides need every wall cell at least two cells from the four sides (so wall links,
+ * interpolated-wall second nodes and wall-function gathers never cross one). */
+typedef enum { PW_SIDES_PERIODIC = 0, PW_SIDES_FREE_SLIP = 1 } pw_sides;
+/* Outlet plane x = nx - 1: the reference's equilibrium (pressure) outlet on EQ cells
+ * (src/lbm/shaders/stream_collide.wgsl:82-107), or zero-gradient: its EQ cells are stepped as
+ * fluid cells whose pulls from x = nx take x = nx - 1 (the unknown populations are copied from
+ * the upstream neighbour after streaming); pulls from wall cells there bounce back halfway.
+ * Needs nx >= 2. The learned wall closure and batched tunnels take the defaults only. */
+typedef enum { PW_OUTLET_EQUILIBRIUM = 0, PW_OUTLET_ZERO_GRADIENT = 1 } pw_outlet;
 
 typedef struct {
   uint32_t nx, ny, nz;
@@ -59,6 +71,8 @@ typedef struct {
   uint32_t storage;    /* pw_storage */
   uint32_t ramp_steps; /* linear inlet ramp length; 0 = full speed from the start */
   uint32_t wall_model; /* pw_wall_model */
+  uint32_t sides;      /* pw_sides; no reference counterpart (the reference is periodic) */
+  uint32_t outlet;     /* pw_outlet; no reference counterpart (the reference's is equilibrium) */
 } pw_params;
 
 /* ForceSample (src/lbm/gpu-force.ts:11); components accumulated and returned in FP64. */

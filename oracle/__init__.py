@@ -84,6 +84,12 @@ def _declare(L):
                                     C.c_int, C.c_int, _u32, C.c_size_t, _d, _d, _d,
                                     C.c_char_p, C.c_size_t]
     L.pwo_solver_create.restype = C.c_void_p
+    L.pwo_solver_create_bc.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, C.c_int, _u32, C.c_size_t, _d, _d,
+                                       _d, C.c_char_p, C.c_size_t]
+    L.pwo_solver_create_bc.restype = C.c_void_p
+    L.pwo_solver_source.argtypes = [C.c_void_p, C.c_size_t, C.c_int]
+    L.pwo_solver_source.restype = C.c_int64
     L.pwo_solver_destroy.argtypes = [C.c_void_p]
     L.pwo_solver_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.pwo_solver_current.argtypes = [C.c_void_p]
@@ -183,7 +189,8 @@ class Solver:
     """FP64 CpuSolver (cpu-solver.ts:69-323). ``populations`` views the latest state."""
 
     def __init__(self, layout, tau0, operator=TRT, les=False, walls=BB, flags=None,
-                 body_force=None, equilibrium_state=None, initial=None, threads=1):
+                 body_force=None, equilibrium_state=None, initial=None, threads=1,
+                 sides="periodic", outlet="equilibrium"):
         self.nx, self.ny, self.nz = (int(v) for v in layout)
         self.cells = self.nx * self.ny * self.nz
         self.threads = threads
@@ -195,10 +202,12 @@ class Solver:
             eq = np.array([rho, *u], dtype=np.float64)
         init = None if initial is None else np.ascontiguousarray(initial, np.float64).reshape(-1)
         err = C.create_string_buffer(256)
-        self._h = lib().pwo_solver_create(
+        bc_sides = {"periodic": 0, "free-slip": 1}[sides]
+        bc_outlet = {"equilibrium": 0, "zero-gradient": 1}[outlet]
+        self._h = lib().pwo_solver_create_bc(
             self.nx, self.ny, self.nz, float(tau0), int(operator), int(les), int(walls),
-            _ptr(fl, _u32), 0 if fl is None else fl.size, _ptr(bf, _d), _ptr(eq, _d),
-            _ptr(init, _d), err, 256)
+            bc_sides, bc_outlet, _ptr(fl, _u32), 0 if fl is None else fl.size, _ptr(bf, _d),
+            _ptr(eq, _d), _ptr(init, _d), err, 256)
         if not self._h:
             raise ValueError(err.value.decode())
 
@@ -210,6 +219,10 @@ class Solver:
 
     def step(self, count: int = 1):
         lib().pwo_solver_step(self._h, int(count), int(self.threads))
+
+    def source(self, cell: int, i: int) -> int:
+        """Flat state index fluid cell ``cell`` pulls direction i from (-1 for other cells)."""
+        return int(lib().pwo_solver_source(self._h, int(cell), int(i)))
 
     def _view(self, p):
         return np.ctypeslib.as_array(p, shape=(19 * self.cells,))

@@ -204,6 +204,7 @@ struct pwo_solver {
   size_t cells;
   double tau0;
   int op, les, walls;
+  int sides, outlet;
   uint8_t *types;
   int64_t *sources; /* 19*N: flat index into a population state */
   wall_link *links;
@@ -224,12 +225,35 @@ static void coords(const pwo_solver *s, size_t cell, long *x, long *y, long *z) 
   *z = (long)(cell / ((size_t)s->nx * s->ny));
 }
 
-/* cpu-solver.ts:179-183: every axis wraps. */
-static size_t wrapped(const pwo_solver *s, size_t cell, int cx, int cy, int cz) {
+/* cpu-solver.ts:179-183: every axis wraps. With a zero-gradient outlet (not in the reference)
+ * x = nx is held at nx - 1; with free-slip sides (not in the reference) y and z outside the
+ * grid are held at the cell's own coordinate, and *mirror_y / *mirror_z (when given) report
+ * that the pull's direction takes its specular image in that axis. */
+static long along(long c, int d, long n, int hold, int *held) {
+  const long v = c + d;
+  if (v >= 0 && v < n) return v;
+  if (held) *held = hold;
+  return hold ? c : (v + n) % n;
+}
+static size_t neighbour(const pwo_solver *s, size_t cell, int cx, int cy, int cz, int *mirror_y,
+                        int *mirror_z) {
   long x, y, z;
   coords(s, cell, &x, &y, &z);
   const long nx = s->nx, ny = s->ny, nz = s->nz;
-  return (size_t)(((x + cx + nx) % nx) + nx * (((y + cy + ny) % ny) + ny * ((z + cz + nz) % nz)));
+  const long sx = (s->outlet == PWO_OUTLET_ZERO_GRADIENT && x + cx == nx) ? x : (x + cx + nx) % nx;
+  const int slip = s->sides == PWO_SIDES_FREE_SLIP;
+  const long sy = along(y, cy, ny, slip, mirror_y), sz = along(z, cz, nz, slip, mirror_z);
+  return (size_t)(sx + nx * (sy + ny * sz));
+}
+static size_t wrapped(const pwo_solver *s, size_t cell, int cx, int cy, int cz) {
+  return neighbour(s, cell, cx, cy, cz, NULL, NULL);
+}
+
+/* Index of the velocity (x, y, z). */
+static int velocity_index(int x, int y, int z) {
+  for (int i = 0; i < 19; ++i)
+    if (PWO_C[i][0] == x && PWO_C[i][1] == y && PWO_C[i][2] == z) return i;
+  return -1;
 }
 
 static int is_wall_type(int t) { return t >= PWO_SOLID && t <= PWO_FLOOR; }
@@ -242,6 +266,21 @@ pwo_solver *pwo_solver_create(uint32_t nx, uint32_t ny, uint32_t nz, double tau0
                               int les, int walls, const uint32_t *flags, size_t flags_len,
                               const double *body_force, const double *eq_state,
                               const double *initial, char *err, size_t err_len) {
+  return pwo_solver_create_bc(nx, ny, nz, tau0, op, les, walls, PWO_SIDES_PERIODIC,
+                              PWO_OUTLET_EQUILIBRIUM, flags, flags_len, body_force, eq_state,
+                              initial, err, err_len);
+}
+
+pwo_solver *pwo_solver_create_bc(uint32_t nx, uint32_t ny, uint32_t nz, double tau0, int op,
+                                 int les, int walls, int sides, int outlet,
+                                 const uint32_t *flags, size_t flags_len,
+                                 const double *body_force, const double *eq_state,
+                                 const double *initial, char *err, size_t err_len) {
+  if (sides < PWO_SIDES_PERIODIC || sides > PWO_SIDES_FREE_SLIP || outlet < PWO_OUTLET_EQUILIBRIUM ||
+      outlet > PWO_OUTLET_ZERO_GRADIENT || (outlet == PWO_OUTLET_ZERO_GRADIENT && nx < 2)) {
+    fail(err, err_len, "Unknown side or outlet boundary option.");
+    return NULL;
+  }
   if (nx < 1 || ny < 1 || nz < 1) {
     fail(err, err_len, "Grid dimensions must be positive safe integers.");
     return NULL;
@@ -284,6 +323,8 @@ pwo_solver *pwo_solver_create(uint32_t nx, uint32_t ny, uint32_t nz, double tau0
   s->op = op;
   s->les = les;
   s->walls = walls;
+  s->sides = sides;
+  s->outlet = outlet;
   s->types = calloc(cells, 1);
   s->sources = malloc(sizeof(int64_t) * cells * 19);
   s->first_link = calloc(cells + 1, sizeof(size_t));
@@ -303,7 +344,9 @@ pwo_solver *pwo_solver_create(uint32_t nx, uint32_t ny, uint32_t nz, double tau0
       pwo_solver_destroy(s);
       return NULL;
     }
-    s->types[cell] = (uint8_t)t;
+    /* The zero-gradient outlet steps the EQ cells of x = nx - 1 as fluid cells. */
+    const int opened = outlet == PWO_OUTLET_ZERO_GRADIENT && t == PWO_EQ && cell % nx == nx - 1;
+    s->types[cell] = (uint8_t)(opened ? PWO_FLUID : t);
     const double *st = initial ? initial + 4 * cell : eqs;
     if (!valid_state(st)) {
       fail(err, err_len, "Initial density and velocity must be finite, density positive.");
@@ -312,7 +355,8 @@ pwo_solver *pwo_solver_create(uint32_t nx, uint32_t ny, uint32_t nz, double tau0
     }
     if (is_wall_type((int)t)) continue;
     pwo_equilibrium(geq, st[0], st + 1);
-    for (int i = 0; i < 19; ++i) s->cur[(size_t)i * cells + cell] = t == PWO_EQ ? s->eq_cell[i] : geq[i];
+    for (int i = 0; i < 19; ++i)
+      s->cur[(size_t)i * cells + cell] = s->types[cell] == PWO_EQ ? s->eq_cell[i] : geq[i];
   }
   memcpy(s->nxt, s->cur, sizeof(double) * cells * 19);
 
@@ -324,12 +368,15 @@ pwo_solver *pwo_solver_create(uint32_t nx, uint32_t ny, uint32_t nz, double tau0
     if (s->types[cell] != PWO_FLUID) continue;
     int rank = 0;
     for (int i = 0; i < 19; ++i) {
-      const size_t nb = wrapped(s, cell, -PWO_C[i][0], -PWO_C[i][1], -PWO_C[i][2]);
+      int my = 0, mz = 0;
+      const size_t nb = neighbour(s, cell, -PWO_C[i][0], -PWO_C[i][1], -PWO_C[i][2], &my, &mz);
+      const int plane = velocity_index(PWO_C[i][0], my ? -PWO_C[i][1] : PWO_C[i][1],
+                                       mz ? -PWO_C[i][2] : PWO_C[i][2]);
       const int t = s->types[nb];
       const int wall = is_wall_type(t);
       s->sources[(size_t)i * cells + cell] =
           wall ? (int64_t)pwo_opp(i) * (int64_t)cells + (int64_t)cell
-               : (int64_t)i * (int64_t)cells + (int64_t)nb;
+               : (int64_t)plane * (int64_t)cells + (int64_t)nb;
       if (!wall) continue;
       double q = 0.5;
       if (walls == PWO_BOUZIDI) {
@@ -383,6 +430,9 @@ void pwo_solver_destroy(pwo_solver *s) {
 double *pwo_solver_current(pwo_solver *s) { return s->cur; }
 double *pwo_solver_previous(pwo_solver *s) { return s->nxt; }
 size_t pwo_solver_link_count(const pwo_solver *s) { return s->n_links; }
+int64_t pwo_solver_source(const pwo_solver *s, size_t cell, int i) {
+  return s->types[cell] == PWO_FLUID ? s->sources[(size_t)i * s->cells + cell] : -1;
+}
 
 /* cpu-solver.ts:200-208: linear Bouzidi from a post-collision state. */
 static double wall_value(const pwo_solver *s, const wall_link *l, const double *state) {

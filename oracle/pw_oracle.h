@@ -55,6 +55,20 @@ pwo_solver *pwo_solver_create(uint32_t nx, uint32_t ny, uint32_t nz, double tau0
                               int les, int walls, const uint32_t *flags, size_t flags_len,
                               const double *body_force, const double *eq_state,
                               const double *initial, char *err, size_t err_len);
+/* Boundary options with no reference counterpart (the reference is periodic in y and z with an
+ * equilibrium outlet). Free-slip sides: a pull that would cross y = 0, ny - 1 or z = 0, nz - 1
+ * reads the specular image (the direction with that component negated) at the source cell held
+ * in that axis. Zero-gradient outlet: EQ cells of x = nx - 1 are fluid cells whose pulls from
+ * x = nx take x = nx - 1. pwo_solver_create is this with PERIODIC and EQUILIBRIUM. */
+enum { PWO_SIDES_PERIODIC = 0, PWO_SIDES_FREE_SLIP = 1 };
+enum { PWO_OUTLET_EQUILIBRIUM = 0, PWO_OUTLET_ZERO_GRADIENT = 1 };
+pwo_solver *pwo_solver_create_bc(uint32_t nx, uint32_t ny, uint32_t nz, double tau0, int op,
+                                 int les, int walls, int sides, int outlet,
+                                 const uint32_t *flags, size_t flags_len,
+                                 const double *body_force, const double *eq_state,
+                                 const double *initial, char *err, size_t err_len);
+/* Flat index into a state that fluid cell `cell` pulls direction i from (-1 for other cells). */
+int64_t pwo_solver_source(const pwo_solver *s, size_t cell, int i);
 void pwo_solver_destroy(pwo_solver *s);
 void pwo_solver_step(pwo_solver *s, int count, int threads);
 double *pwo_solver_current(pwo_solver *s);  /* 19*N, direction-major */

@@ -209,8 +209,10 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   }
   *out = nullptr;
   if (shared->ramp_steps != 0 || shared->wall_model != PW_WALL_LAW || shared->storage > PW_STORAGE_BF16 ||
-      shared->op > PW_OP_REG || shared->walls > PW_WALLS_BOUZIDI) {
-    set_error("Batched tunnels take operator, walls, LES and storage options without a ramp.");
+      shared->op > PW_OP_REG || shared->walls > PW_WALLS_BOUZIDI || shared->sides != PW_SIDES_PERIODIC ||
+      shared->outlet != PW_OUTLET_EQUILIBRIUM) {
+    set_error("Batched tunnels take operator, walls, LES and storage options without a ramp, periodic "
+              "sides and the equilibrium outlet.");
     return PW_EINVAL;
   }
   // Validate every tunnel exactly as a single solver would, on the host.

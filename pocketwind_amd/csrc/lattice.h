@@ -30,6 +30,17 @@ __host__ __device__ constexpr float wf(int i) {
 }
 __host__ __device__ constexpr int opp(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
 
+// Index of the velocity (x, y, z) (every component in {-1, 0, 1}; 19 of the 27 exist).
+__host__ __device__ constexpr int dir_of(int x, int y, int z) {
+  for (int i = 0; i < 19; ++i)
+    if (cx(i) == x && cy(i) == y && cz(i) == z) return i;
+  return -1;
+}
+// Specular image of velocity i in a free-slip side: its y (MY) and/or z (MZ) component negated.
+__host__ __device__ constexpr int mirror_dir(int i, bool my, bool mz) {
+  return dir_of(cx(i), my ? -cy(i) : cy(i), mz ? -cz(i) : cz(i));
+}
+
 __host__ __device__ __forceinline__ float as_float(uint32_t u) { return __builtin_bit_cast(float, u); }
 __host__ __device__ __forceinline__ uint32_t as_uint(float f) { return __builtin_bit_cast(uint32_t, f); }
 

@@ -120,7 +120,48 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
   out->x_tile = out->run == 64 ? 18u * 64u : 0u;
   out->div_nx = make_fast_div(L.nx);
   out->div_ny = make_fast_div(L.ny);
+  out->bc = (prm.sides == PW_SIDES_FREE_SLIP ? kBcFreeSlip : 0u) |
+            (prm.outlet == PW_OUTLET_ZERO_GRADIENT ? kBcZeroGradient : 0u);
   return true;
+}
+
+// Free-slip sides: no wall cell within two cells of a side, so that the wall masks over wrapped
+// neighbours equal those over the step's sources and no wall gather crosses a side.
+bool walls_clear_of_sides(const pw_layout& L, const uint32_t* geo) {
+  for (uint32_t z = 0; z < L.nz; ++z)
+    for (uint32_t y = 0; y < L.ny; ++y) {
+      if (y >= 2 && y + 2 < L.ny && z >= 2 && z + 2 < L.nz) continue;
+      for (uint32_t x = 0; x < L.nx; ++x)
+        if (wall_kind(geo[x + (size_t)L.nx * (y + (size_t)L.ny * z)] & kTypeMask)) {
+          set_error("Free-slip sides need every wall cell at least two cells from the sides.");
+          return false;
+        }
+    }
+  return true;
+}
+
+// Zero-gradient outlet: the EQ cells of the plane x = nx - 1 become fluid cells with the pull
+// wall mask of their sources (x = nx for c_x = -1 held at nx - 1) and no wall-link fractions
+// (halfway bounce-back); every other word is kept.
+void open_outlet(const pw_layout& L, uint32_t sides, std::vector<uint32_t>& geo) {
+  const size_t N = (size_t)L.nx * L.ny * L.nz;
+  const bool slip = sides == PW_SIDES_FREE_SLIP;
+  const uint32_t x = L.nx - 1;
+  for (uint32_t z = 0; z < L.nz; ++z)
+    for (uint32_t y = 0; y < L.ny; ++y) {
+      const size_t cell = x + (size_t)L.nx * (y + (size_t)L.ny * z);
+      if ((geo[cell] & kTypeMask) != kEq) continue;
+      uint32_t mask = 0;
+      for (int i = 1; i < kQ; ++i) {
+        const long sx = cx(i) < 0 ? (long)x : (long)x - cx(i);
+        long sy = (long)y - cy(i), sz = (long)z - cz(i);
+        sy = sy < 0 ? (slip ? 0 : L.ny - 1) : (sy >= (long)L.ny ? (slip ? L.ny - 1 : 0) : sy);
+        sz = sz < 0 ? (slip ? 0 : L.nz - 1) : (sz >= (long)L.nz ? (slip ? L.nz - 1 : 0) : sz);
+        if (wall_kind(geo[sx + L.nx * (sy + (size_t)L.ny * sz)] & kTypeMask)) mask |= 1u << (3 + i);
+      }
+      geo[cell] = kFluid | mask;
+      geo[N + cell] = kNoQ;
+    }
 }
 
 // gpu-solver.ts:119-123
@@ -344,8 +385,22 @@ pw_status validate_tunnel(const pw_layout& L, const uint32_t* geometry, size_t g
     set_error("The learned wall closure replaces the LES wall model and needs LES.");
     return PW_EINVAL;
   }
+  if (params.sides > PW_SIDES_FREE_SLIP || params.outlet > PW_OUTLET_ZERO_GRADIENT) {
+    set_error("Unknown side or outlet boundary option.");
+    return PW_EINVAL;
+  }
+  if ((params.sides != PW_SIDES_PERIODIC || params.outlet != PW_OUTLET_EQUILIBRIUM) &&
+      params.wall_model == PW_WALL_LEARNED) {
+    set_error("The learned wall closure runs with periodic sides and the equilibrium outlet only.");
+    return PW_EINVAL;
+  }
+  if (params.outlet == PW_OUTLET_ZERO_GRADIENT && L.nx < 2) {
+    set_error("A zero-gradient outlet needs nx >= 2.");
+    return PW_EINVAL;
+  }
   if (!pack_params(L, params, params.u_in, full)) return PW_EINVAL;
   if (!validate_geometry(L, geometry, geometry_words)) return PW_EINVAL;
+  if (params.sides == PW_SIDES_FREE_SLIP && !walls_clear_of_sides(L, geometry)) return PW_EINVAL;
   const size_t N = (size_t)L.nx * L.ny * L.nz;
   // gpu-force.ts:76-81
   for (size_t k = 0; k < n_boundary; ++k) {
@@ -388,6 +443,12 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   StepParams sp = full;
   if (params->ramp_steps > 0) sp.u_in = 0.0f;
   const size_t N = (size_t)L.nx * L.ny * L.nz;
+  std::vector<uint32_t> opened;
+  if (params->outlet == PW_OUTLET_ZERO_GRADIENT) {
+    opened.assign(geometry, geometry + geometry_words);
+    open_outlet(L, params->sides, opened);
+    geometry = opened.data();
+  }
   long long first_fluid = -1;
   for (size_t c = 0; c < N && first_fluid < 0; ++c)
     if ((geometry[c] & kTypeMask) == kFluid) first_fluid = (long long)c;

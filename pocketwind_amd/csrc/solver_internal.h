@@ -40,6 +40,14 @@ inline FastDiv make_fast_div(uint32_t d) {
   return FastDiv{d, (uint32_t)m, l - 1};
 }
 
+// Boundary options beyond the reference's (which has periodic y/z sides and a pressure
+// outlet). Free-slip sides: a pull that would cross y = 0, ny - 1 or z = 0, nz - 1 takes the
+// specular image, i.e. the mirrored direction from the source cell unshifted in that axis
+// (halfway specular reflection). Zero-gradient outlet: the plane x = nx - 1 is stepped as fluid
+// and its pulls from x = nx take x = nx - 1 instead, which copies the unknown populations from
+// the upstream neighbour after streaming.
+enum : uint32_t { kBcFreeSlip = 1u, kBcZeroGradient = 2u };
+
 // Per-launch scalars of the step kernel (the reference's 48-byte uniform, struct Params in
 // src/lbm/shaders/common.wgsl:31-46, plus constants this kernel precomputes), passed by
 // value so a ramped step needs no buffer write.
@@ -56,6 +64,7 @@ struct StepParams {
   float inv_tau0;  // 1 / tau0 in f32: the relaxation rate without LES
   float wm0;       // TRT odd rate 1 / (lambda / (tau0 - 1/2) + 1/2) in f32
   FastDiv div_nx, div_ny;
+  uint32_t bc;  // kBcFreeSlip | kBcZeroGradient, or 0 for the reference's periodic y/z sides
   // Learned wall closure (null: the wall law): shear stress per wall cell (xyz, w > 0 when
   // valid) and the wall-cell index of every cell.
   const float4* closure_tau;

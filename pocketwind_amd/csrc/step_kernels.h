@@ -20,6 +20,8 @@
 // another order.
 #pragma once
 
+#include <type_traits>
+
 #include "lattice.h"
 #include "solver_internal.h"
 
@@ -277,22 +279,37 @@ __host__ __device__ __forceinline__ void cell_coords(uint32_t n, const StepParam
   y = r - z * p.ny;
 }
 
-// Wrapped flat indices of a cell's neighbours (cell arrays such as the geometry): the index of
-// (x + dx, y + dy, z + dz) is xo[dx+1] + yo[dy+1] + zo[dz+1] with every axis periodic
-// (common.wgsl:64-68).
+// Neighbour coordinate lo (c - 1) and hi (c + 1) of c on an axis of n cells: periodic
+// (common.wgsl:64-68), or held at c where `clamp_lo` / `clamp_hi` holds (free-slip sides, the
+// zero-gradient outlet).
+__host__ __device__ __forceinline__ uint32_t below(uint32_t c, uint32_t n, bool clamp) {
+  return c == 0 ? (clamp ? c : n - 1) : c - 1;
+}
+__host__ __device__ __forceinline__ uint32_t above(uint32_t c, uint32_t n, bool clamp) {
+  return c + 1 == n ? (clamp ? c : 0) : c + 1;
+}
+
+// Flat indices of a cell's neighbours (cell arrays such as the geometry): the index of
+// (x + dx, y + dy, z + dz) is xo[dx+1] + yo[dy+1] + zo[dz+1], every axis periodic except
+// where p.bc holds the source at the boundary.
 struct Neighbours {
   uint32_t xo[3], yo[3], zo[3];
-  __host__ __device__ __forceinline__ Neighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p) {
+  __host__ __device__ __forceinline__ Neighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p)
+      : Neighbours(x, y, z, p, p.bc) {}
+  // bc: the boundary options in force (p.bc, or 0 where the caller knows them to be off).
+  __host__ __device__ __forceinline__ Neighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p,
+                                                 uint32_t bc) {
     const uint32_t row = p.nx, plane = p.nx * p.ny;
-    xo[0] = x == 0 ? p.nx - 1 : x - 1;
+    const bool slip = bc & kBcFreeSlip;
+    xo[0] = below(x, p.nx, false);
     xo[1] = x;
-    xo[2] = x + 1 == p.nx ? 0 : x + 1;
-    yo[0] = (y == 0 ? p.ny - 1 : y - 1) * row;
+    xo[2] = above(x, p.nx, bc & kBcZeroGradient);
+    yo[0] = below(y, p.ny, slip) * row;
     yo[1] = y * row;
-    yo[2] = (y + 1 == p.ny ? 0 : y + 1) * row;
-    zo[0] = (z == 0 ? p.nz - 1 : z - 1) * plane;
+    yo[2] = above(y, p.ny, slip) * row;
+    zo[0] = below(z, p.nz, slip) * plane;
     zo[1] = z * plane;
-    zo[2] = (z + 1 == p.nz ? 0 : z + 1) * plane;
+    zo[2] = above(z, p.nz, slip) * plane;
   }
   __host__ __device__ __forceinline__ uint32_t index(int dx, int dy, int dz) const {
     return xo[dx + 1] + yo[dy + 1] + zo[dz + 1];
@@ -304,23 +321,27 @@ struct Neighbours {
 __host__ __device__ __forceinline__ uint32_t pop_x(const StepParams& p, uint32_t x) { return x + p.x_tile * (x >> 6); }
 __host__ __device__ __forceinline__ uint32_t pop_row(const StepParams& p, uint32_t r) { return r * (kQ * p.nx); }
 
-// The same for wrapped neighbours, scaled by 2^SHIFT (element bytes for byte addressing). The
-// layout is additive in x and in the row, and a set is below 4 GiB (checked at create), so a
-// byte offset fits a u32.
+// The same for population elements, scaled by 2^SHIFT (element bytes for byte addressing).
+// The layout is additive in x and in the row, and a set is below 4 GiB (checked at create), so
+// a byte offset fits a u32.
 template <int SHIFT = 0>
 struct PopNeighbours {
   uint32_t xo[3], yo[3], zo[3];
-  __host__ __device__ __forceinline__ PopNeighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p) {
+  __host__ __device__ __forceinline__ PopNeighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p)
+      : PopNeighbours(x, y, z, p, p.bc) {}
+  __host__ __device__ __forceinline__ PopNeighbours(uint32_t x, uint32_t y, uint32_t z, const StepParams& p,
+                                                    uint32_t bc) {
     const uint32_t row = kQ * p.nx, plane = row * p.ny;
-    xo[0] = pop_x(p, x == 0 ? p.nx - 1 : x - 1) << SHIFT;
+    const bool slip = bc & kBcFreeSlip;
+    xo[0] = pop_x(p, below(x, p.nx, false)) << SHIFT;
     xo[1] = pop_x(p, x) << SHIFT;
-    xo[2] = pop_x(p, x + 1 == p.nx ? 0 : x + 1) << SHIFT;
-    yo[0] = ((y == 0 ? p.ny - 1 : y - 1) * row) << SHIFT;
+    xo[2] = pop_x(p, above(x, p.nx, bc & kBcZeroGradient)) << SHIFT;
+    yo[0] = (below(y, p.ny, slip) * row) << SHIFT;
     yo[1] = (y * row) << SHIFT;
-    yo[2] = ((y + 1 == p.ny ? 0 : y + 1) * row) << SHIFT;
-    zo[0] = ((z == 0 ? p.nz - 1 : z - 1) * plane) << SHIFT;
+    yo[2] = (above(y, p.ny, slip) * row) << SHIFT;
+    zo[0] = (below(z, p.nz, slip) * plane) << SHIFT;
     zo[1] = (z * plane) << SHIFT;
-    zo[2] = ((z + 1 == p.nz ? 0 : z + 1) * plane) << SHIFT;
+    zo[2] = (above(z, p.nz, slip) * plane) << SHIFT;
   }
   __host__ __device__ __forceinline__ uint32_t at(int dx, int dy, int dz) const {
     return xo[dx + 1] + yo[dy + 1] + zo[dz + 1];
@@ -603,6 +624,48 @@ __host__ __device__ __forceinline__ void guard_density(float4* macro, const Step
   if (!(rho >= 0.5f && rho <= 1.5f)) *reinterpret_cast<uint32_t*>(macro + p.cells) = 1u;
 }
 
+// A cell in a free-slip side row (y = 0, ny - 1 or z = 0, nz - 1).
+__host__ __device__ __forceinline__ bool on_side(uint32_t y, uint32_t z, const StepParams& p) {
+  return y == 0 || y + 1 == p.ny || z == 0 || z + 1 == p.nz;
+}
+
+// Plane that direction i pulls from in a free-slip side row: the specular image of i in each
+// side the pull would cross (ylo: y = 0, and so on). The source offsets already hold that axis
+// at the cell, so the pull reads the mirrored direction of (x - c_x, y, z - c_z) at y = 0, etc.
+template <int I>
+__host__ __device__ __forceinline__ int slip_plane(bool ylo, bool yhi, bool zlo, bool zhi) {
+  const bool my = cy(I) > 0 ? ylo : (cy(I) < 0 ? yhi : false);
+  const bool mz = cz(I) > 0 ? zlo : (cz(I) < 0 ? zhi : false);
+  constexpr int y_image = mirror_dir(I, true, false), z_image = mirror_dir(I, false, true),
+                yz_image = mirror_dir(I, true, true);
+  return my ? (mz ? yz_image : y_image) : (mz ? z_image : I);
+}
+
+template <int I, int END, class F>
+__host__ __device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < END) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, END>(f);
+  }
+}
+
+// Free-slip side row, cell path: after the plain gather, the pulls that cross a side are read
+// again from their mirrored planes (the source offsets already hold the crossed axis at the
+// cell). Side rows hold no wall links (walls keep two cells from a free-slip side).
+template <class S, bool NT>
+__host__ __device__ __forceinline__ void slip_cell(const typename S::T* __restrict__ in, const StepParams& p,
+                                                   const PopNeighbours<S::kShift>& pnb, uint32_t y, uint32_t z,
+                                                   float* g) {
+  const bool ylo = y == 0, yhi = y + 1 == p.ny, zlo = z == 0, zhi = z + 1 == p.nz;
+  static_for<1, kQ>([&](auto c) {
+    constexpr int i = decltype(c)::value;
+    if constexpr (cy(i) != 0 || cz(i) != 0) {
+      const int j = slip_plane<i>(ylo, yhi, zlo, zhi);
+      if (j != i) g[i] = load_pop<S, NT>(byte_offset(in + (size_t)j * p.run, pnb.at(-cx(i), -cy(i), -cz(i))));
+    }
+  });
+}
+
 // Cache policy of the population streams: cached, non-temporal loads and stores, or cached
 // loads with non-temporal stores. Flags and wall-path loads are always cached.
 enum { kCached = 0, kNtAll = 1, kNtStores = 2 };
@@ -611,7 +674,7 @@ enum { kCached = 0, kNtAll = 1, kNtStores = 2 };
 // EDGE: the outlet reads the upstream moments stored with the input set and the row's
 // x = nx - 2 cell stores its own with the output set (the solver's sets carry them); without
 // it the outlet gathers the upstream state (batched tunnels).
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL = kCached, bool EDGE = false>
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL = kCached, bool EDGE = false, bool BC = false>
 __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
@@ -622,7 +685,8 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
   const uint32_t kind = flag & kTypeMask;
   uint32_t x, y, z;
   cell_coords(n, p, x, y, z);
-  const PopNeighbours<S::kShift> pnb(x, y, z, p);
+  const uint32_t bc = BC ? p.bc : 0u;
+  const PopNeighbours<S::kShift> pnb(x, y, z, p, bc);
   const uint32_t self = pnb.at(0, 0, 0);
   float g[kQ];
   float4 m;
@@ -633,12 +697,13 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
       for (int i = 1; i < kQ; ++i)
         g[i] = load_pop<S, POL == kNtAll>(byte_offset(in + (size_t)i * P, pnb.at(-cx(i), -cy(i), -cz(i))));
     } else {
-      const Neighbours nb(x, y, z, p);
+      const Neighbours nb(x, y, z, p, bc);
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
         g[i] = has_wall(flag, i) ? wall_value<S, BOUZIDI>(in, geo, p, nb, n, self >> S::kShift, flag, i)
                                  : S::load(byte_offset(in + (size_t)i * P, pnb.at(-cx(i), -cy(i), -cz(i))));
     }
+    if (BC && (p.bc & kBcFreeSlip) && on_side(y, z, p)) slip_cell<S, POL == kNtAll>(in, p, pnb, y, z, g);
     m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
     if constexpr (EDGE) {
@@ -694,33 +759,73 @@ struct RowWave {
 // its neighbour lane's value with a DPP wavefront shift (wave_shr:1 moves lane l - 1 to lane
 // l, wave_shl:1 lane l + 1 to lane l); the lane at the wavefront's edge takes the value one
 // load brings for the whole wavefront. All 64 lanes must be active.
+// Pull of direction I for a row wavefront from the source row `src` of its plane.
+template <class S, bool NT, int I>
+__device__ __forceinline__ float pull_row(const typename S::T* src, uint32_t xs, uint32_t left, uint32_t right) {
+  const uint32_t own = load_bits<S, NT>(byte_offset(src, xs));
+  if constexpr (cx(I) == 0) {
+    return decode_bits<S>(own);
+  } else {
+    const uint32_t edge = load_bits<S, NT>(byte_offset(src, cx(I) > 0 ? left : right));
+    const int v = cx(I) > 0 ? __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x138, 0xf, 0xf, false)
+                            : __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x130, 0xf, 0xf, false);
+    return decode_bits<S>((uint32_t)v);
+  }
+}
+
+// Sources of lane 0 for c_x = +1 and of lane 63 for c_x = -1 (the zero-gradient outlet holds
+// the last wavefront's at its own column), and the lane's own column, as byte offsets.
+template <int SH>
+__device__ __forceinline__ void row_columns(const StepParams& p, uint32_t bc, uint32_t x0, uint32_t lane,
+                                            uint32_t& left, uint32_t& right, uint32_t& xs) {
+  left = pop_x(p, below(x0, p.nx, false)) << SH;
+  right = pop_x(p, above(x0 + 63, p.nx, bc & kBcZeroGradient)) << SH;
+  xs = (pop_x(p, x0) + lane) << SH;
+}
+
 template <class S, bool NT>
-__device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in, const StepParams& p,
+__device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in, const StepParams& p, uint32_t bc,
                                            const RowWave& w, uint32_t lane, float* g) {
   constexpr int SH = S::kShift;
   const size_t P = p.run;  // 64: one wavefront's run
   const uint32_t row = kQ * p.nx, plane = row * p.ny;
-  const uint32_t y = w.y, z = w.z, x0 = w.x0;
+  const uint32_t y = w.y, z = w.z;
   const uint32_t yo[3] = {((y == 0 ? p.ny - 1 : y - 1) * row) << SH, (y * row) << SH,
                           ((y + 1 == p.ny ? 0 : y + 1) * row) << SH};
   const uint32_t zo[3] = {((z == 0 ? p.nz - 1 : z - 1) * plane) << SH, (z * plane) << SH,
                           ((z + 1 == p.nz ? 0 : z + 1) * plane) << SH};
-  const uint32_t left = pop_x(p, x0 == 0 ? p.nx - 1 : x0 - 1) << SH;     // source of lane 0 when c_x = +1
-  const uint32_t right = pop_x(p, x0 + 64 == p.nx ? 0 : x0 + 64) << SH;  // source of lane 63 when c_x = -1
-  const uint32_t xs = (pop_x(p, x0) + lane) << SH;
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    const typename S::T* src = byte_offset(in + (size_t)i * P, yo[1 - cy(i)] + zo[1 - cz(i)]);
-    const uint32_t own = load_bits<S, NT>(byte_offset(src, xs));
-    if (cx(i) == 0) {
-      g[i] = decode_bits<S>(own);
-    } else {
-      const uint32_t edge = load_bits<S, NT>(byte_offset(src, cx(i) > 0 ? left : right));
-      const int v = cx(i) > 0 ? __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x138, 0xf, 0xf, false)
-                              : __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x130, 0xf, 0xf, false);
-      g[i] = decode_bits<S>((uint32_t)v);
+  uint32_t left, right, xs;
+  row_columns<SH>(p, bc, w.x0, lane, left, right, xs);
+  static_for<0, kQ>([&](auto c) {
+    constexpr int i = decltype(c)::value;
+    g[i] = pull_row<S, NT, i>(byte_offset(in + (size_t)i * P, yo[1 - cy(i)] + zo[1 - cz(i)]), xs, left, right);
+  });
+}
+
+// Free-slip side row, row path: after the plain gather, the pulls that cross a side are read
+// again, from the mirrored plane of the row unshifted in the crossed axis (all wavefront-uniform).
+template <class S, bool NT>
+__device__ __forceinline__ void slip_row(const typename S::T* __restrict__ in, const StepParams& p,
+                                         const RowWave& w, uint32_t lane, float* g) {
+  constexpr int SH = S::kShift;
+  const uint32_t row = kQ * p.nx, plane = row * p.ny;
+  const uint32_t y = w.y, z = w.z;
+  const bool ylo = y == 0, yhi = y + 1 == p.ny, zlo = z == 0, zhi = z + 1 == p.nz;
+  uint32_t left, right, xs;
+  row_columns<SH>(p, p.bc, w.x0, lane, left, right, xs);
+  static_for<1, kQ>([&](auto c) {
+    constexpr int i = decltype(c)::value;
+    if constexpr (cy(i) != 0 || cz(i) != 0) {
+      const bool my = cy(i) > 0 ? ylo : (cy(i) < 0 ? yhi : false);
+      const bool mz = cz(i) > 0 ? zlo : (cz(i) < 0 ? zhi : false);
+      if (my || mz) {  // a pull that does not cross stays in range in that axis
+        const uint32_t sy = my ? y : y - cy(i), sz = mz ? z : z - cz(i);
+        const int j = slip_plane<i>(ylo, yhi, zlo, zhi);
+        g[i] = pull_row<S, NT, i>(byte_offset(in + (size_t)j * p.run, (sy * row + sz * plane) << SH), xs, left,
+                                  right);
+      }
     }
-  }
+  });
 }
 
 // Base of plane i at cell n0 as an opaque wavefront-uniform global pointer. Kept opaque so
@@ -780,7 +885,7 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_
 // stream_collide_cell for a row wavefront: the whole wavefront gathers (gather_row), then fluid
 // lanes replace wall-link directions and collide, EQ and wall lanes take their fixed states,
 // and every lane stores. A wavefront of walls only skips the step.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL>
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool BC>
 __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __restrict__ in,
                                                         typename S::T* __restrict__ out,
                                                         const uint32_t* __restrict__ geo,
@@ -801,12 +906,14 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
   RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
   if (outlet_wave) up = row_moments(edge_rows(in, p)[row]);
   float g[kQ];
-  gather_row<S, POL == kNtAll>(in, p, w, lane, g);
+  const uint32_t bc = BC ? p.bc : 0u;
+  gather_row<S, POL == kNtAll>(in, p, bc, w, lane, g);
+  if (BC && (p.bc & kBcFreeSlip) && on_side(w.y, w.z, p)) slip_row<S, POL == kNtAll>(in, p, w, lane, g);
   const uint32_t x = w.x0 + lane;
   float4 m;
   if (kind == kFluid) {
     if ((flag >> 3) != 0) {
-      const Neighbours nb(x, w.y, w.z, p);
+      const Neighbours nb(x, w.y, w.z, p, bc);
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
         if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, w.e0 + lane, flag, i);
@@ -841,7 +948,9 @@ constexpr uint32_t kXcds = 8;
 template <int OP, bool LES>
 constexpr int kStepWaves = (LES && OP == kOpTrt) ? 4 : 7;
 
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL>
+// BC: the instance for boundary options (p.bc != 0); the reference's boundaries take BC = false,
+// which compiles the option code out of the step.
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool BC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<OP, LES>))) void stream_collide_kernel(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
@@ -853,9 +962,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<
   const uint32_t n = logical * blockDim.x + threadIdx.x;
   if (n >= p.cells) return;  // whole wavefronts when nx is a multiple of 64
   if (p.nx % 64 == 0)
-    stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL>(in, out, geo, macro, p, n, seed);
+    stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL, BC>(in, out, geo, macro, p, n, seed);
   else
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL, true>(in, out, geo, macro, p, n, seed);
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL, true, BC>(in, out, geo, macro, p, n, seed);
 }
 
 }  // namespace pw

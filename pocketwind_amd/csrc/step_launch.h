@@ -14,15 +14,18 @@ constexpr int kStepBlock = 256;  // largest block of a step launch (the kernels'
 template <class S, int OP, bool LES, bool BB>
 void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const StepParams& p, uint32_t seed) {
   constexpr bool BOUZIDI = !BB;
-  auto pick = [&](auto with_macro) {
-    constexpr bool M = decltype(with_macro)::value;
+  auto pick = [&](auto with_macro, auto with_bc) {
+    constexpr bool M = decltype(with_macro)::value, BC = decltype(with_bc)::value;
     switch (s->kernel) {
-      case PW_KERNEL_CELL_NT: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtAll>;
-      case PW_KERNEL_CELL_NT_STORES: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtStores>;
-      default: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kCached>;
+      case PW_KERNEL_CELL_NT: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtAll, BC>;
+      case PW_KERNEL_CELL_NT_STORES: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtStores, BC>;
+      default: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kCached, BC>;
     }
   };
-  auto k = macro ? pick(std::true_type{}) : pick(std::false_type{});
+  // Boundary options select their own instances, so the reference's boundaries run a step
+  // without their code.
+  auto k = p.bc ? (macro ? pick(std::true_type{}, std::true_type{}) : pick(std::false_type{}, std::true_type{}))
+                : (macro ? pick(std::true_type{}, std::false_type{}) : pick(std::false_type{}, std::false_type{}));
   const uint32_t block = s->block;
   const uint64_t blocks = (s->cells + block - 1) / block;
   // Each XCD takes s->xcd_chunks chunks; the grid is rounded up to whole chunks (the extra

@@ -28,6 +28,8 @@ _OPERATORS = {"trt": 0, "reg": 1}
 _WALLS = {"bb": 0, "bouzidi": 1}
 _STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
 _WALL_MODELS = {"law": 0, "learned": 1}
+_SIDES = {"periodic": 0, "free-slip": 1}
+_OUTLETS = {"equilibrium": 0, "zero-gradient": 1}
 
 NAN_GUARD_INTERVAL = 200  # nan-guard.ts:6
 
@@ -55,6 +57,10 @@ class SolverParams:
     precision: str = "f32"
     ramp_steps: int = 0
     wall_model: str = "law"
+    # Boundary options with no reference counterpart (the reference's sides are periodic and
+    # its outlet is the equilibrium outlet): "free-slip" sides, a "zero-gradient" outlet.
+    sides: str = "periodic"
+    outlet: str = "equilibrium"
 
 
 @dataclass(frozen=True)
@@ -77,14 +83,16 @@ def _pack(params: SolverParams) -> N.Params:
     for name, table, value in (("operator", _OPERATORS, params.operator),
                                ("walls", _WALLS, params.walls),
                                ("precision", _STORAGE, params.precision),
-                               ("wall_model", _WALL_MODELS, params.wall_model)):
+                               ("wall_model", _WALL_MODELS, params.wall_model),
+                               ("sides", _SIDES, params.sides), ("outlet", _OUTLETS, params.outlet)):
         if value not in table:
             raise ValueError(f"Unknown {name} {value!r}.")
     if not isinstance(params.ramp_steps, (int, np.integer)) or params.ramp_steps < 0:
         raise ValueError("Ramp steps must be a non-negative safe integer.")
     return N.Params(float(params.tau0), float(params.u_in), int(bool(params.les)),
                     _OPERATORS[params.operator], _WALLS[params.walls], _STORAGE[params.precision],
-                    int(params.ramp_steps), _WALL_MODELS[params.wall_model])
+                    int(params.ramp_steps), _WALL_MODELS[params.wall_model], _SIDES[params.sides],
+                    _OUTLETS[params.outlet])
 
 
 class GpuSolver:

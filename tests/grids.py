@@ -3,6 +3,8 @@
 blob_grid        tests/browser/gpu-vs-cpu.spec.ts:76-116 (LCG seed 0x4a3b2c1d, radius 5 + 2u)
 link_fractions   tests/browser/gpu-vs-cpu.spec.ts:57-74  (seed 0x1b873593, every 7th q = 0.5)
 grid_from_kinds  tests/browser/force.spec.ts:40-72
+sphere_kinds     an analytic sphere in a channel with EQ inlet and outlet planes (no reference
+                 counterpart; the free-slip / zero-gradient channel cases)
 """
 from __future__ import annotations
 
@@ -38,16 +40,16 @@ def grid_from_kinds(layout, kinds) -> VoxelGrid:
     geometry = np.zeros(2 * cells, dtype=np.uint32)
     geometry[:cells] = kinds
     geometry[cells:] = NO_Q
-    boundary = []
-    for cell in range(cells):
-        if kinds[cell] != FLUID:
-            continue
-        mask = pull_wall_mask(kinds, layout, cell)
-        geometry[cell] = mask
-        if mask:
-            boundary.append(cell)
-    return VoxelGrid(nx, ny, nz, geometry, np.array(boundary, dtype=np.uint32),
-                     solid_cells=int(np.sum(kinds == SOLID)))
+    # pull_wall_mask for every cell at once: rolling by c brings the source x - c to x.
+    grid3 = kinds.reshape(nz, ny, nx) & CELL_TYPE_MASK
+    wall = (grid3 >= SOLID) & (grid3 <= FLOOR)
+    mask = np.zeros((nz, ny, nx), dtype=np.uint32)
+    for i, (cx, cy, cz) in enumerate(DIRECTIONS):
+        mask |= np.roll(wall, (cz, cy, cx), axis=(0, 1, 2)).astype(np.uint32) << np.uint32(3 + i)
+    fluid = kinds == FLUID
+    geometry[:cells][fluid] = mask.reshape(-1)[fluid]
+    boundary = np.flatnonzero(fluid & (mask.reshape(-1) != 0)).astype(np.uint32)
+    return VoxelGrid(nx, ny, nz, geometry, boundary, solid_cells=int(np.sum(kinds == SOLID)))
 
 
 def blob_kinds(layout=(32, 32, 32)) -> np.ndarray:
@@ -91,3 +93,14 @@ def blob_grid(walls: str = "bb") -> VoxelGrid:
     layout = (32, 32, 32)
     grid = grid_from_kinds(layout, blob_kinds(layout))
     return link_fractions(grid) if walls == "bouzidi" else grid
+
+
+def sphere_kinds(layout, centre, diameter) -> np.ndarray:
+    """Solid where the cell centre lies strictly inside the sphere; x = 0 and x = nx - 1 EQ."""
+    nx, ny, nz = layout
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r2 = (x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2
+    kinds = np.where(r2 < (diameter / 2) ** 2, SOLID, FLUID).astype(np.uint32)
+    kinds[:, :, 0] = EQ
+    kinds[:, :, nx - 1] = EQ
+    return kinds.reshape(-1)

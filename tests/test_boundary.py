@@ -10,10 +10,10 @@ import re
 import numpy as np
 import pytest
 
-from grids import blob_grid
+from grids import blob_grid, grid_from_kinds
 from pocketwind_amd import _native as N
-from pocketwind_amd.gpu_solver import GpuSolver, SolverParams
-from pocketwind_amd.lattice import NO_Q, VoxelGrid
+from pocketwind_amd.gpu_solver import GpuSolver, SolverParams, _pack
+from pocketwind_amd.lattice import NO_Q, SOLID, VoxelGrid
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) <= set(N.SIGNATURES), set(names) - set(N.SIGNATURES)
-    assert lib.pw_abi_version() == 4
+    assert lib.pw_abi_version() == 5
 
 
 def _create(grid, params, boundary=None):
@@ -104,3 +104,30 @@ def test_host_mirror_raises_value_error_like_range_error():
         GpuSolver.create(grid.layout, grid, SolverParams(0.8, 0.06, False, operator="bgk"))
     with pytest.raises(ValueError):
         GpuSolver.create((16, 32, 32), grid, SolverParams(0.8, 0.06, False))
+
+
+def test_boundary_options_are_checked_on_the_host():
+    """Free-slip sides need walls two cells clear of the sides; the learned closure and
+    batched tunnels take the reference's boundaries only; a zero-gradient outlet needs nx >= 2."""
+    blob = blob_grid("bb")  # its SUPPORT/FLOOR pair and blob sit well inside the 32^3 box
+    assert _create(blob, _params(sides=2))[0] == N.PW_EINVAL
+    assert _create(blob, _params(outlet=2))[0] == N.PW_EINVAL
+    status, msg = _create(blob, _params(les=1, wall_model=1, sides=1))
+    assert status == N.PW_EINVAL and "periodic sides" in msg
+    kinds = np.zeros((8, 8, 8), np.uint32)
+    kinds[1, 4, 4] = SOLID  # one cell from the z = 0 side
+    near = grid_from_kinds((8, 8, 8), kinds.reshape(-1))
+    status, msg = _create(near, _params(sides=1))
+    assert status == N.PW_EINVAL and "two cells from the sides" in msg
+    line = grid_from_kinds((1, 4, 4), np.zeros(16, np.uint32))
+    status, msg = _create(line, _params(outlet=1))
+    assert status == N.PW_EINVAL and "nx >= 2" in msg
+    with pytest.raises(ValueError):
+        _pack(SolverParams(0.8, 0.06, False, sides="closed"))
+
+
+def test_batched_tunnels_take_the_reference_boundaries_only():
+    from pocketwind_amd.batch import TunnelBatch
+    grid = blob_grid("bb")
+    with pytest.raises(ValueError, match="periodic"):
+        TunnelBatch.create([grid], [0.8], [0.06], SolverParams(0.8, 0.06, False, sides="free-slip"))
