@@ -82,6 +82,8 @@ typedef struct {
 } pw_force_sample;
 
 typedef struct pw_solver pw_solver;
+typedef struct pw_batch pw_batch;
+typedef struct pw_mlp pw_mlp;
 
 /* Last error text of the calling thread (the message the reference puts in RangeError). */
 const char *pw_last_error_message(void);
@@ -247,14 +249,19 @@ pw_status pw_particles_write(pw_particles *p, const float *in, size_t in_floats)
 /* The seed cross-section (y0, z0, y1, z1) and the fallback inlet node (x, y, z). */
 pw_status pw_particles_info(const pw_particles *p, float region[4], float fallback[3]);
 void pw_particles_destroy(pw_particles *p);
+/* The same particles bound to tunnel `tunnel` of a batch (no reference counterpart: the
+ * reference has one solver); updates run on the batch's stream. Destroy them before the batch. */
+pw_status pw_particles_create_batch(pw_batch *b, uint32_t tunnel, uint32_t count, uint32_t seed,
+                                    const double body_min[3], const double body_max[3],
+                                    pw_particles **out);
 
-/* Batched tunnels: `count` independent tunnels of any sizes advanced by one persistent launch
- * per step, even/odd step pairs replayed from a captured hipGraph. No reference counterpart
- * (the reference allows one live solver, src/app/solver-slot.ts:14-44); each tunnel follows
+/* Batched tunnels: `count` independent tunnels of any sizes advanced by one launch per step,
+ * even/odd step pairs replayed from a captured hipGraph. No reference counterpart (the
+ * reference allows one live solver, src/app/solver-slot.ts:14-44); each tunnel follows
  * pw_solver semantics exactly and is bit-identical to the same tunnel run alone. Tunnel t
  * takes geometries[t] (the packed geometry of pw_solver_create), boundary_cells[t], tau0[t]
- * and u_in[t]; operator, walls, LES and storage come from `shared` (no ramp). */
-typedef struct pw_batch pw_batch;
+ * and u_in[t]; operator, walls, LES, storage and wall model come from `shared` (no ramp,
+ * periodic sides, equilibrium outlet). */
 pw_status pw_batch_create(uint32_t count, const pw_layout *layouts, const uint32_t *const *geometries,
                           const size_t *geometry_words, const uint32_t *const *boundary_cells,
                           const size_t *n_boundary, const double *tau0, const double *u_in,
@@ -270,6 +277,16 @@ uint64_t pw_batch_step_count(const pw_batch *b);
 uint64_t pw_batch_cells(const pw_batch *b);
 pw_status pw_batch_time_steps(pw_batch *b, uint32_t count, float *elapsed_ms);
 void pw_batch_destroy(pw_batch *b);
+/* The learned wall closure of a batch created with wall_model PW_WALL_LEARNED: one network
+ * and normalisation for every tunnel, as pw_solver_set_closure / _interval (no reference
+ * counterpart). Each tunnel's closure runs on its own wall cells before the steps that
+ * evaluate it; a batched tunnel stays bit-identical to the same tunnel run alone with the
+ * same closure. */
+pw_status pw_batch_set_closure(pw_batch *b, const pw_mlp *net, const float *norm, size_t n_norm);
+pw_status pw_batch_set_closure_interval(pw_batch *b, uint32_t every);
+/* The momentum-exchange force of every tunnel now (fx, fy, fz per tunnel, FP64; the mean over
+ * both states as a force sample), waiting for the batch's queued work. */
+pw_status pw_batch_force(pw_batch *b, double *out, size_t out_doubles);
 
 /* Learned wall closure network: x (12 features) -> 64 -> 64 -> 3 with SiLU activations, bf16
  * matrix-core products accumulated in f32. No reference counterpart: the reference's wall
@@ -281,7 +298,6 @@ void pw_batch_destroy(pw_batch *b);
 #define PW_MLP_HIDDEN 64
 #define PW_MLP_OUTPUTS 3
 #define PW_MLP_PARAMS 5187
-typedef struct pw_mlp pw_mlp;
 pw_status pw_mlp_create(pw_mlp **out); /* all parameters zero */
 void pw_mlp_destroy(pw_mlp *m);
 /* Sets the f32 parameters (n == PW_MLP_PARAMS) and clears the Adam state. */

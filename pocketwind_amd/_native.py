@@ -85,6 +85,9 @@ SIGNATURES = {
     "pw_particles_write": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "pw_particles_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "pw_particles_destroy": (None, [C.c_void_p]),
+    "pw_particles_create_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                            C.POINTER(C.c_void_p)]),
     "pw_batch_create": (C.c_int, [C.c_uint32, C.POINTER(Layout), C.POINTER(C.c_void_p),
                                   C.POINTER(C.c_size_t), C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(Params),
@@ -97,6 +100,9 @@ SIGNATURES = {
     "pw_batch_cells": (C.c_uint64, [C.c_void_p]),
     "pw_batch_time_steps": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
     "pw_batch_destroy": (None, [C.c_void_p]),
+    "pw_batch_set_closure": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_batch_set_closure_interval": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "pw_batch_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "pw_solver_wall_cell_count": (C.c_size_t, [C.c_void_p]),
     "pw_solver_wall_cells": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "pw_solver_closure_features": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t]),

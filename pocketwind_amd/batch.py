@@ -73,6 +73,20 @@ class TunnelBatch:
         N.check(N.lib().pw_batch_read_populations(self._h, int(tunnel), out.ctypes.data, out.nbytes))
         return out
 
+    def set_closure(self, net, norm) -> None:
+        """Learned wall closure for every tunnel (the batch needs wall_model "learned")."""
+        norm = np.ascontiguousarray(norm, np.float32)
+        N.check(N.lib().pw_batch_set_closure(self._h, net._h, norm.ctypes.data, norm.size))
+
+    def force(self) -> np.ndarray:
+        """(tunnels, 3) momentum-exchange forces of the current state (waits for queued work)."""
+        out = np.empty((len(self.grids), 3), np.float64)
+        N.check(N.lib().pw_batch_force(self._h, out.ctypes.data, out.size))
+        return out
+
+    def set_closure_interval(self, every: int) -> None:
+        N.check(N.lib().pw_batch_set_closure_interval(self._h, int(every)))
+
     def time_steps(self, count: int) -> float:
         ms = C.c_float()
         N.check(N.lib().pw_batch_time_steps(self._h, int(count), C.byref(ms)))

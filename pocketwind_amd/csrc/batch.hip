@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -125,6 +126,14 @@ struct pw_batch {
   unsigned grid = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   hipEvent_t t0 = nullptr, t1 = nullptr;
+  // Learned wall closure (created with PW_WALL_LEARNED): the network, its normalisation and
+  // the evaluation interval, shared by every tunnel.
+  void* closure_net = nullptr;
+  float closure_norm[32] = {};
+  int closure_on = 0;
+  uint32_t closure_every = 1;
+  int closure_fresh = 0;
+  std::vector<std::pair<const uint32_t*, float4*>> closure_bufs;  // per tunnel: slot table, output
 };
 
 namespace {
@@ -138,15 +147,31 @@ void launch_step(pw_batch* b, int parity, bool macro) {
   }
 }
 
+template <class S>
+void launch_edges(pw_batch* b, int set) {
+  for (const Tunnel& tu : b->host)
+    if (tu.p.nx % 64 == 0)
+      hipLaunchKernelGGL(edge_moments_kernel<S>, dim3((tu.p.ny * tu.p.nz + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                         b->stream, (typename S::T*)tu.pops[set], tu.p);
+}
+
 void launch_init(pw_batch* b) {
   const unsigned grid = b->grid;
   for (int set = 0; set < 2; ++set) {
     switch (b->params.storage) {
-      case PW_STORAGE_F16: hipLaunchKernelGGL(batch_init_kernel<StoreF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); break;
-      case PW_STORAGE_BF16: hipLaunchKernelGGL(batch_init_kernel<StoreBF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); break;
-      default: hipLaunchKernelGGL(batch_init_kernel<StoreF32>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); break;
+      case PW_STORAGE_F16: hipLaunchKernelGGL(batch_init_kernel<StoreF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); launch_edges<StoreF16>(b, set); break;
+      case PW_STORAGE_BF16: hipLaunchKernelGGL(batch_init_kernel<StoreBF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); launch_edges<StoreBF16>(b, set); break;
+      default: hipLaunchKernelGGL(batch_init_kernel<StoreF32>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); launch_edges<StoreF32>(b, set); break;
     }
   }
+}
+
+// The closure of every tunnel on the state the next step reads.
+void launch_closures(pw_batch* b) {
+  for (const Tunnel& tu : b->host)
+    launch_closure_tunnel(b->params.storage, b->params.walls == PW_WALLS_BOUZIDI, b->stream, tu.pops[b->parity],
+                          tu.geo, tu.p, tu.wall_cells, tu.n_wall, b->closure_net, b->closure_norm,
+                          b->closure_bufs[&tu - b->host.data()].second);
 }
 
 void launch_force(pw_batch* b, double* dest) {
@@ -194,6 +219,8 @@ hipError_t capture_pair(pw_batch* b, int first_parity, hipGraphExec_t* exec) {
   return e;
 }
 
+pw_status step_plain(pw_batch* b, uint32_t count, int write_macro_on_last);
+
 }  // namespace
 
 extern "C" {
@@ -208,11 +235,11 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
     return PW_EINVAL;
   }
   *out = nullptr;
-  if (shared->ramp_steps != 0 || shared->wall_model != PW_WALL_LAW || shared->storage > PW_STORAGE_BF16 ||
+  if (shared->ramp_steps != 0 || shared->wall_model > PW_WALL_LEARNED || shared->storage > PW_STORAGE_BF16 ||
       shared->op > PW_OP_REG || shared->walls > PW_WALLS_BOUZIDI || shared->sides != PW_SIDES_PERIODIC ||
       shared->outlet != PW_OUTLET_EQUILIBRIUM) {
-    set_error("Batched tunnels take operator, walls, LES and storage options without a ramp, periodic "
-              "sides and the equilibrium outlet.");
+    set_error("Batched tunnels take operator, walls, LES, storage and wall-model options without a ramp, "
+              "periodic sides and the equilibrium outlet.");
     return PW_EINVAL;
   }
   // Validate every tunnel exactly as a single solver would, on the host.
@@ -261,11 +288,35 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
     Tunnel& tu = b->host[t];
     tu.p = sps[t];
     const size_t N = sps[t].cells;
+    // 19 N elements, then (row-wavefront tunnels) one float4 of upstream-outlet moments per row.
+    const size_t set_bytes = edge_offset_bytes(N, elem) + (size_t)sps[t].ny * sps[t].nz * sizeof(float4);
     for (int k = 0; k < 2; ++k) {
       void* p = nullptr;
-      if ((e = hipMalloc(&p, (size_t)kQ * sps[t].cells * elem)) != hipSuccess) return fail(e, "batch populations");
+      if ((e = hipMalloc(&p, set_bytes)) != hipSuccess) return fail(e, "batch populations");
       b->allocations.push_back(p);
       tu.pops[k] = p;
+    }
+    if (shared->wall_model == PW_WALL_LEARNED) {
+      std::vector<uint32_t> walls, slot(N, 0xffffffffu);
+      for (size_t c = 0; c < N; ++c)
+        if ((geometries[t][c] & kTypeMask) == kFluid && (geometries[t][c] >> 3) != 0) {
+          slot[c] = (uint32_t)walls.size();
+          walls.push_back((uint32_t)c);
+        }
+      uint32_t *wl = nullptr, *sl = nullptr;
+      float4* tau = nullptr;
+      if ((e = dev_alloc(b, &wl, walls.size())) != hipSuccess || (e = dev_alloc(b, &sl, N)) != hipSuccess ||
+          (e = dev_alloc(b, &tau, walls.size())) != hipSuccess)
+        return fail(e, "batch closure buffers");
+      if ((walls.size() && (e = hipMemcpy(wl, walls.data(), walls.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) ||
+          (e = hipMemcpy(sl, slot.data(), N * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "batch closure upload");
+      tu.wall_cells = wl;
+      tu.n_wall = (uint32_t)walls.size();
+      b->closure_bufs.push_back({sl, tau});  // given to the step once a network is attached
+    } else {
+      tu.wall_cells = nullptr;
+      tu.n_wall = 0;
     }
     uint32_t *geo = nullptr, *bnd = nullptr;
     float4* macro = nullptr;
@@ -300,10 +351,12 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
       (e = hipMemcpyAsync(b->seeds, seeds0, sizeof seeds0, hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
       (pt.size() && (e = hipMemcpyAsync(b->part_tunnel, pt.data(), pt.size() * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess))
     return fail(e, "batch tables");
-  int cus = 256;
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
-  const unsigned persistent = (unsigned)cus * 8;
-  b->grid = b->n_tiles < persistent ? b->n_tiles : persistent;
+  // One block per tile: a grid the device schedules as blocks retire (a persistent grid larger
+  // than the resident capacity would leave its last blocks to run after the others).
+  b->grid = (unsigned)std::min<uint64_t>(b->n_tiles, 0x7fffffffu);
+  if (shared->wall_model == PW_WALL_LEARNED &&
+      (e = dev_alloc(b, reinterpret_cast<char**>(&b->closure_net), mlp_packed_bytes())) != hipSuccess)
+    return fail(e, "batch closure network");
   launch_init(b);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e, "batch init");
   if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail(e, "batch init");
@@ -316,6 +369,88 @@ pw_status pw_batch_step(pw_batch* b, uint32_t count, int write_macro_on_last) {
     set_error("Null batch handle.");
     return PW_EINVAL;
   }
+  if (!b->closure_on) return step_plain(b, count, write_macro_on_last);
+  // The closure runs on the state a step reads, before the first step of a run and at every
+  // multiple of its interval, as in a single solver.
+  uint32_t done = 0;
+  while (done < count) {
+    if (!b->closure_fresh || b->steps % b->closure_every == 0) {
+      launch_closures(b);
+      b->closure_fresh = 1;
+    }
+    const uint32_t chunk =
+        (uint32_t)std::min<uint64_t>(b->closure_every - b->steps % b->closure_every, count - done);
+    if (pw_status st = step_plain(b, chunk, write_macro_on_last && done + chunk == count)) return st;
+    done += chunk;
+  }
+  return PW_OK;
+}
+
+pw_status pw_batch_set_closure(pw_batch* b, const pw_mlp* net, const float* norm, size_t n_norm) {
+  if (!b || b->params.wall_model != PW_WALL_LEARNED || !b->closure_net) {
+    set_error("The batch was not created with the learned wall model.");
+    return PW_EINVAL;
+  }
+  if (!net || !norm || n_norm != 27) {
+    set_error("A closure needs a network and 27 normalisation floats (12 means, 12 scales, output "
+              "scale, limit, velocity unit).");
+    return PW_EINVAL;
+  }
+  for (size_t k = 0; k < n_norm; ++k)
+    if (!std::isfinite(norm[k])) {
+      set_error("Closure normalisation must be finite.");
+      return PW_EINVAL;
+    }
+  if (!(norm[26] > 0.0f) || !(norm[25] > 0.0f)) {
+    set_error("Closure velocity unit and limit must be positive.");
+    return PW_EINVAL;
+  }
+  hipError_t e = hipMemcpyAsync(b->closure_net, mlp_packed(net), mlp_packed_bytes(), hipMemcpyDeviceToDevice, b->stream);
+  for (uint32_t t = 0; t < b->count; ++t) {
+    b->host[t].p.closure_slot = b->closure_bufs[t].first;
+    b->host[t].p.closure_tau = b->closure_bufs[t].second;
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->tunnels, b->host.data(), b->count * sizeof(Tunnel), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return device_error(e, "batch closure attach");
+  std::memcpy(b->closure_norm, norm, n_norm * sizeof(float));
+  b->closure_on = 1;
+  b->closure_fresh = 0;
+  return PW_OK;
+}
+
+pw_status pw_batch_force(pw_batch* b, double* out, size_t out_doubles) {
+  if (!b || !out || out_doubles < 3 * (size_t)b->count) {
+    set_error("Force output needs 3 doubles per tunnel.");
+    return PW_EINVAL;
+  }
+  double* dev = nullptr;
+  hipError_t e = hipMallocAsync((void**)&dev, 3 * (size_t)b->count * sizeof(double), b->stream);
+  if (e != hipSuccess) return device_error(e, "batch force buffer");
+  launch_force(b, dev);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dev, 3 * (size_t)b->count * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+  (void)hipFreeAsync(dev, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  return e == hipSuccess ? PW_OK : device_error(e, "batch force");
+}
+
+pw_status pw_batch_set_closure_interval(pw_batch* b, uint32_t every) {
+  if (!b || every == 0) {
+    set_error("The closure interval must be at least one step.");
+    return PW_EINVAL;
+  }
+  b->closure_every = every;
+  return PW_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// `count` steps without closure evaluations: whole even/odd pairs replay a captured graph.
+pw_status step_plain(pw_batch* b, uint32_t count, int write_macro_on_last) {
   uint32_t done = 0;
   hipError_t e;
   // Whole even/odd pairs replay a captured graph; the remainder and a macro step launch directly.
@@ -337,6 +472,20 @@ pw_status pw_batch_step(pw_batch* b, uint32_t count, int write_macro_on_last) {
   if ((e = hipGetLastError()) != hipSuccess) return device_error(e, "batch step");
   return PW_OK;
 }
+
+}  // namespace
+
+bool pw::batch_field(pw_batch* b, uint32_t tunnel, const float4** macro, const uint32_t** geo, pw_layout* layout,
+                     hipStream_t* stream) {
+  if (!b || tunnel >= b->count) return false;
+  *macro = b->host[tunnel].macro;
+  *geo = b->host[tunnel].geo;
+  *layout = b->layouts[tunnel];
+  *stream = b->stream;
+  return true;
+}
+
+extern "C" {
 
 pw_status pw_batch_step_sampled(pw_batch* b, uint32_t count, uint32_t every, double* out, size_t out_doubles) {
   if (!b || !out || every == 0) {

@@ -10,7 +10,7 @@ namespace batch {
 constexpr int kBlock = 256;
 
 struct Tunnel {
-  StepParams p;
+  StepParams p;  // p.closure_tau / p.closure_slot point at this tunnel's closure output once attached
   void* pops[2];
   const uint32_t* geo;
   float4* macro;
@@ -18,6 +18,8 @@ struct Tunnel {
   uint32_t n_boundary;
   uint32_t partial_base;  // first force partial of this tunnel
   uint32_t n_partials;
+  const uint32_t* wall_cells;  // learned closure: fluid cells with wall links, in cell order
+  uint32_t n_wall;
 };
 
 struct BatchLaunch {
@@ -34,8 +36,11 @@ struct BatchLaunch {
 // One step of every tunnel. The rounding seed is the step count, read on the device so a
 // replayed graph advances it: a launch of parity k reads seeds[k] and writes seeds[k ^ 1] for
 // the next launch, which in stream order starts after this one ends.
+// A tunnel whose nx is a multiple of 64 steps its cells as row wavefronts (the single solver's
+// fast path; its sets carry the upstream-outlet moments after the 19 N elements), others per
+// cell. Either way a tile is one block of one tunnel.
 template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
-__global__ __launch_bounds__(kBlock) void batch_step_kernel(const Tunnel* __restrict__ tunnels,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWaves<OP, LES>))) void batch_step_kernel(const Tunnel* __restrict__ tunnels,
                                                             const uint32_t* __restrict__ tile_tunnel,
                                                             const uint32_t* __restrict__ tile_first,
                                                             uint32_t n_tiles, int parity, uint32_t* seeds) {
@@ -48,7 +53,10 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(const Tunnel* __rest
     if (n >= tu.p.cells) continue;
     const auto* in = (const typename S::T*)tu.pops[parity];
     auto* out = (typename S::T*)tu.pops[parity ^ 1];
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, kNtAll>(in, out, tu.geo, tu.macro, tu.p, n, seed);
+    if (tu.p.nx % 64 == 0)
+      stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, kNtAll, false>(in, out, tu.geo, tu.macro, tu.p, n, seed);
+    else
+      stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, kNtAll>(in, out, tu.geo, tu.macro, tu.p, n, seed);
   }
 }
 

@@ -57,6 +57,28 @@ pw_status check(pw_solver* s) {
 
 }  // namespace
 
+template <class S>
+void launch_closure_tunnel_s(bool bz, hipStream_t stream, const void* state, const uint32_t* geo, const StepParams& p,
+                             const uint32_t* wall_cells, uint32_t n_wall, const void* net, const float* norm27,
+                             float4* tau) {
+  closure::Norm norm;
+  std::memcpy(&norm, norm27, sizeof norm);
+  auto k = bz ? closure::closure_kernel<S, true> : closure::closure_kernel<S, false>;
+  hipLaunchKernelGGL(k, dim3(closure_blocks(n_wall)), dim3(64), 0, stream, (const typename S::T*)state, geo, p,
+                     wall_cells, n_wall, (const mlp::Packed*)net, norm, tau, nullptr);
+}
+
+void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, const void* state,
+                           const uint32_t* geo, const StepParams& p, const uint32_t* wall_cells, uint32_t n_wall,
+                           const void* net, const float* norm, float4* tau) {
+  if (n_wall == 0) return;
+  switch (storage) {
+    case PW_STORAGE_F16: launch_closure_tunnel_s<StoreF16>(bouzidi, stream, state, geo, p, wall_cells, n_wall, net, norm, tau); break;
+    case PW_STORAGE_BF16: launch_closure_tunnel_s<StoreBF16>(bouzidi, stream, state, geo, p, wall_cells, n_wall, net, norm, tau); break;
+    default: launch_closure_tunnel_s<StoreF32>(bouzidi, stream, state, geo, p, wall_cells, n_wall, net, norm, tau); break;
+  }
+}
+
 void launch_closure(pw_solver* s, StepParams& p) {
   if (!s->closure_on || s->n_wall == 0) return;
   if (!s->closure_fresh || s->steps % s->closure_every == 0) {

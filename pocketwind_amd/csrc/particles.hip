@@ -135,7 +135,9 @@ using namespace pw;
 using namespace pw::part;
 
 struct pw_particles {
-  pw_solver* solver = nullptr;
+  pw_solver* solver = nullptr;  // the bound solver, or null when bound to a batched tunnel
+  Field field{};                // the macro field and flags the particles sample
+  hipStream_t stream = nullptr;  // the stream those are written on
   uint32_t count = 0, seed = 0;
   float region[4] = {0, 0, 0, 0};
   float fallback[3] = {0, 0, 0};
@@ -144,15 +146,13 @@ struct pw_particles {
 
 extern "C" {
 
-// particle-layout.ts:74-96 and :126-172 with the count limit of this library.
-pw_status pw_particles_create(pw_solver* s, uint32_t count, uint32_t seed, const double body_min[3],
-                              const double body_max[3], pw_particles** out) {
-  if (!s || !out || !body_min || !body_max) {
-    set_error("Null argument.");
-    return PW_EINVAL;
-  }
-  *out = nullptr;
-  const pw_layout L = s->layout;
+namespace {
+
+// particle-layout.ts:74-96 and :126-172 with the count limit of this library, for the field f
+// written on `stream`.
+pw_status create_particles(const Field& f, hipStream_t stream, uint32_t count, uint32_t seed,
+                           const double body_min[3], const double body_max[3], pw_particles** out) {
+  const pw_layout L{f.nx, f.ny, f.nz};
   const double ey = body_max[1] - body_min[1], ez = body_max[2] - body_min[2];
   double region[4] = {std::fmax(1, body_min[1] - ey * 0.15), std::fmax(1, body_min[2] - ez * 0.15),
                       std::fmin((double)L.ny - 2, body_max[1] + ey * 0.15),
@@ -169,9 +169,9 @@ pw_status pw_particles_create(pw_solver* s, uint32_t count, uint32_t seed, const
     return PW_EINVAL;
   }
   // Inlet candidates need the cell flags: read back the plane x = 2 rows once.
-  std::vector<uint32_t> flags((size_t)s->cells);
-  hipError_t e = hipMemcpyAsync(flags.data(), s->geometry, flags.size() * 4, hipMemcpyDeviceToHost, s->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  std::vector<uint32_t> flags((size_t)L.nx * L.ny * L.nz);
+  hipError_t e = hipMemcpyAsync(flags.data(), f.flags, flags.size() * 4, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) return device_error(e, "particle flags");
   std::vector<float> inlet;
   for (long z = (long)js_round(region[1]); z <= (long)js_round(region[3]); ++z)
@@ -200,7 +200,8 @@ pw_status pw_particles_create(pw_solver* s, uint32_t count, uint32_t seed, const
     set_error("Host allocation failed.");
     return PW_ENOMEM;
   }
-  h->solver = s;
+  h->field = f;
+  h->stream = stream;
   h->count = count;
   h->seed = seed;
   for (int k = 0; k < 4; ++k) h->region[k] = (float)region[k];
@@ -209,8 +210,8 @@ pw_status pw_particles_create(pw_solver* s, uint32_t count, uint32_t seed, const
     delete h;
     return device_error(e, "particle allocation");
   }
-  e = hipMemcpyAsync(h->data, host.data(), host.size() * sizeof(float4), hipMemcpyHostToDevice, s->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  e = hipMemcpyAsync(h->data, host.data(), host.size() * sizeof(float4), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) {
     (void)hipFree(h->data);
     delete h;
@@ -220,9 +221,42 @@ pw_status pw_particles_create(pw_solver* s, uint32_t count, uint32_t seed, const
   return PW_OK;
 }
 
+}  // namespace
+
+pw_status pw_particles_create(pw_solver* s, uint32_t count, uint32_t seed, const double body_min[3],
+                              const double body_max[3], pw_particles** out) {
+  if (!s || s->destroyed || !out || !body_min || !body_max) {
+    set_error("Null argument.");
+    return PW_EINVAL;
+  }
+  *out = nullptr;
+  const Field f{s->macro, s->geometry, s->layout.nx, s->layout.ny, s->layout.nz};
+  pw_status st = create_particles(f, s->stream, count, seed, body_min, body_max, out);
+  if (st == PW_OK) (*out)->solver = s;
+  return st;
+}
+
+pw_status pw_particles_create_batch(pw_batch* b, uint32_t tunnel, uint32_t count, uint32_t seed,
+                                    const double body_min[3], const double body_max[3], pw_particles** out) {
+  if (!b || !out || !body_min || !body_max) {
+    set_error("Null argument.");
+    return PW_EINVAL;
+  }
+  *out = nullptr;
+  const float4* macro = nullptr;
+  const uint32_t* geo = nullptr;
+  pw_layout L{};
+  hipStream_t stream = nullptr;
+  if (!batch_field(b, tunnel, &macro, &geo, &L, &stream)) {
+    set_error("Invalid tunnel index.");
+    return PW_EINVAL;
+  }
+  return create_particles(Field{macro, geo, L.nx, L.ny, L.nz}, stream, count, seed, body_min, body_max, out);
+}
+
 // particles.update(encoder, steps) (src/render/particles.ts:113-124).
 pw_status pw_particles_update(pw_particles* h, float steps) {
-  if (!h || !h->solver || h->solver->destroyed) {
+  if (!h || (h->solver && h->solver->destroyed)) {
     set_error("Invalid particle handle.");
     return PW_EINVAL;
   }
@@ -230,11 +264,9 @@ pw_status pw_particles_update(pw_particles* h, float steps) {
     set_error("Particle advection needs non-negative finite solver steps.");
     return PW_EINVAL;
   }
-  pw_solver* s = h->solver;
   UpdateParams p{h->count, h->seed, steps, {h->region[0], h->region[1], h->region[2], h->region[3]},
                  {h->fallback[1], h->fallback[2]}};
-  Field f{s->macro, s->geometry, s->layout.nx, s->layout.ny, s->layout.nz};
-  hipLaunchKernelGGL(update_kernel, dim3((h->count + 255) / 256), dim3(256), 0, s->stream, h->data, f, p);
+  hipLaunchKernelGGL(update_kernel, dim3((h->count + 255) / 256), dim3(256), 0, h->stream, h->data, h->field, p);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? PW_OK : device_error(e, "particle update");
 }
@@ -244,8 +276,8 @@ pw_status pw_particles_read(pw_particles* h, float* out, size_t out_floats) {
     set_error("Particle output needs 8 floats per particle.");
     return PW_EINVAL;
   }
-  hipError_t e = hipMemcpyAsync(out, h->data, 32 * (size_t)h->count, hipMemcpyDeviceToHost, h->solver->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->solver->stream);
+  hipError_t e = hipMemcpyAsync(out, h->data, 32 * (size_t)h->count, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   return e == hipSuccess ? PW_OK : device_error(e, "particle readback");
 }
 
@@ -254,8 +286,8 @@ pw_status pw_particles_write(pw_particles* h, const float* in, size_t in_floats)
     set_error("Particle input must hold 8 floats per particle.");
     return PW_EINVAL;
   }
-  hipError_t e = hipMemcpyAsync(h->data, in, 32 * (size_t)h->count, hipMemcpyHostToDevice, h->solver->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->solver->stream);
+  hipError_t e = hipMemcpyAsync(h->data, in, 32 * (size_t)h->count, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   return e == hipSuccess ? PW_OK : device_error(e, "particle upload");
 }
 

@@ -84,6 +84,7 @@ pw_status validate_tunnel(const pw_layout& layout, const uint32_t* geometry, siz
 
 struct pw_solver;
 struct pw_mlp;
+struct pw_batch;
 
 namespace pw {
 // Learned wall closure (closure.hip): the network's device copy for the matrix cores, and
@@ -92,6 +93,14 @@ namespace pw {
 const void* mlp_packed(const pw_mlp* m);
 size_t mlp_packed_bytes();
 void launch_closure(pw_solver* s, StepParams& p);
+// The closure kernel over one tunnel's wall-cell list (batched tunnels): `norm` holds the 27
+// normalisation floats, `net` the packed network.
+void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, const void* state,
+                           const uint32_t* geo, const StepParams& p, const uint32_t* wall_cells, uint32_t n_wall,
+                           const void* net, const float* norm, float4* tau);
+// The macro field, flags, layout and stream of one batched tunnel (false for a bad index).
+bool batch_field(pw_batch* b, uint32_t tunnel, const float4** macro, const uint32_t** geo, pw_layout* layout,
+                 hipStream_t* stream);
 // `count` steps, the last writing the macros (and the guard word after them) when asked.
 pw_status step_run(pw_solver* s, uint32_t count, int write_macro_on_last);
 }  // namespace pw

@@ -35,6 +35,22 @@ class Particles:
                                             hi.ctypes.data_as(C.POINTER(C.c_double)), C.byref(out)))
         return Particles(out.value, solver, int(count))
 
+    @staticmethod
+    def create_in_batch(batch, tunnel: int, body_bounds, count: int, seed: int,
+                        max_count: int = MAX_PARTICLES):
+        """The same particles over tunnel ``tunnel`` of a TunnelBatch (its last macro field)."""
+        lo, hi = (np.asarray(v, np.float64) for v in body_bounds)
+        if not isinstance(count, (int, np.integer)) or count < 1 or count > max_count:
+            raise ValueError("Particle count, seed or geometry buffer is invalid.")
+        if not isinstance(seed, (int, np.integer)) or seed < 0 or seed > 0xFFFFFFFF:
+            raise ValueError("Particle count, seed or geometry buffer is invalid.")
+        out = C.c_void_p()
+        N.check(N.lib().pw_particles_create_batch(batch._h, int(tunnel), int(count), int(seed),
+                                                  lo.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  hi.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  C.byref(out)))
+        return Particles(out.value, batch, int(count))
+
     def update(self, steps: float) -> None:
         if not np.isfinite(np.float32(steps)) or steps < 0:
             raise ValueError("Particle advection needs non-negative finite solver steps.")

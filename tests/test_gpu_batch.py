@@ -58,3 +58,56 @@ def test_batch_rejects_invalid_tunnel():
     grid, tau, u = tunnel((32, 32, 32), "sphere", 1)
     with pytest.raises(ValueError, match="Tunnel 1"):
         TunnelBatch.create([grid, grid], [tau, 0.5], [u, u], SolverParams(0.6, 0.05, False))
+
+
+def test_batch_with_learned_closure_and_particles_equals_single_tunnels():
+    """f16 REG + LES tunnels with the learned wall closure (every 8 steps) and 4096 particles
+    on each: the batch (row-wavefront and per-cell tunnels in one launch) matches single
+    solvers bit for bit, and particles bound to a batched tunnel match particles bound to the
+    single solver."""
+    import os
+    from pocketwind_amd import closure
+    from pocketwind_amd.particles import Particles
+    net, norm, _ = closure.load(os.path.join(os.path.dirname(closure.__file__), "closure_c2.npz"))
+    cases = [((128, 64, 64), "bluff", 5), ((96, 48, 48), "sphere", 6)]
+    grids = []
+    for layout, kind, seed in cases:
+        mesh = {"sphere": bodies.sphere_mesh, "bluff": bodies.bluff_body}[kind]()
+        body, _ = bodies.place_mesh(mesh, None, layout[0] / 4, layout, False)
+        grids.append((voxelize(body, None, layout, False, link_fractions=False), body))
+    shared = SolverParams(0.53, 0.05, True, operator="reg", precision="f16", wall_model="learned")
+    batch = TunnelBatch.create([g for g, _ in grids], [0.53] * 2, [0.05] * 2, shared)
+    bparts = []
+    try:
+        batch.set_closure(net, norm)
+        batch.set_closure_interval(8)
+        for k, (g, body) in enumerate(grids):
+            lo, hi = body.reshape(-1, 3).min(0), body.reshape(-1, 3).max(0)
+            bparts.append(Particles.create_in_batch(batch, k, (lo, hi), 4096, 7, max_count=4096))
+        samples = batch.step_sampled(64, 8)
+        batch.step(1, True)
+        for p in bparts:
+            p.update(1.0)
+        for k, (g, body) in enumerate(grids):
+            s = GpuSolver.create(g.layout, g, shared)
+            s.set_kernel("cell-nt")
+            s.set_closure(net, norm)
+            s.set_closure_interval(8)
+            lo, hi = body.reshape(-1, 3).min(0), body.reshape(-1, 3).max(0)
+            sp = Particles.create(s, (lo, hi), 4096, 7, max_count=4096)
+            single = s.step_sampled(64, 8)
+            s.step(1, True)
+            sp.update(1.0)
+            used = int(np.sum(s.closure_output()[:, 3] > 0))
+            print(f"tunnel {k}: {used} of {s.wall_cells().size} wall cells on the network")
+            assert used > 0
+            assert np.array_equal(single, samples[k]), f"tunnel {k} force samples differ"
+            assert np.array_equal(s.read_populations().view(np.uint16), batch.read_populations(k).view(np.uint16))
+            assert np.array_equal(s.read_macro(), batch.read_macro(k))
+            assert np.array_equal(sp.read(), bparts[k].read())
+            sp.destroy()
+            s.destroy()
+    finally:
+        for p in bparts:
+            p.destroy()
+        batch.destroy()
