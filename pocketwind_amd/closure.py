@@ -256,6 +256,56 @@ def load(path):
         return net, d["norm"].astype(np.float32), json.loads(str(d["meta"]))
 
 
+def evaluate_cd(reynolds: float = 2000.0, layout=(256, 128, 128), length: int = 64, closure_file=None,
+                u_in: float = 0.05, flow_throughs: int = 6, average: int = 2, log=print) -> dict:
+    """Drag of the slanted bluff body (``bodies.bluff_body``, nose at 0.25 nx, centred) at
+    Reynolds number `reynolds` on its length, REG + Smagorinsky LES, halfway bounce-back:
+    f16 storage with the learned closure (every 8 steps) and f32 with the wall law on `layout`,
+    and f32 with the wall law on the twice finer grid (the reference result the closure is
+    judged against). Cd = mean fx over the last `average` of `flow_throughs` flow-throughs
+    (runner.ts:234-244) on the voxel frontal area."""
+    import os
+    from . import bodies, drag
+    from .gpu_solver import GpuSolver, SolverParams
+    from .voxelize import voxelize
+    if closure_file is None:
+        closure_file = os.path.join(os.path.dirname(os.path.abspath(__file__)), "closure_c2.npz")
+    net, norm, _ = load(closure_file)
+
+    def cd(grid_layout, grid_length, precision, learned):
+        body, _ = bodies.place_mesh(bodies.bluff_body(), None, grid_length, grid_layout, False)
+        grid = voxelize(body, None, grid_layout, False, link_fractions=True)
+        nu = u_in * grid_length / reynolds
+        s = GpuSolver.create(grid_layout, grid, SolverParams(3 * nu + 0.5, u_in, True, operator="reg",
+                                                             precision=precision,
+                                                             wall_model="learned" if learned else "law"))
+        try:
+            if learned:
+                s.set_closure(net, norm)
+                s.set_closure_interval(8)
+            flow = drag.flow_through_steps(grid.nx, u_in)
+            samples = s.step_sampled(flow_throughs * flow, 8)
+            fx, _, _ = drag.window_mean_force(samples, flow_throughs * flow, average * flow)
+            used = int(np.sum(s.closure_output()[:, 3] > 0)) if learned else 0
+            walls = int(s.wall_cells().size)
+        finally:
+            s.destroy()
+        value = fx * drag.cd_per_lattice_force(grid.frontal_cells, u_in)
+        log(f"  {grid_layout} {precision} {'learned' if learned else 'wall law'}: Cd {value:.6f}"
+            + (f" ({used} of {walls} wall cells on the network)" if learned else ""))
+        return value, used, walls
+
+    fine = tuple(2 * n for n in layout)
+    learned, used, walls = cd(layout, length, "f16", True)
+    law, _, _ = cd(layout, length, "f32", False)
+    ref, _, _ = cd(fine, 2 * length, "f32", False)
+    net.destroy()
+    return {"reynolds": reynolds, "layout": list(layout), "length_cells": length,
+            "cd_f16_learned": learned, "cd_f32_law": law, "cd_f32_law_2x_grid": ref,
+            "learned_vs_2x_grid_pct": 100 * (learned / ref - 1), "law_vs_2x_grid_pct": 100 * (law / ref - 1),
+            "wall_cells": walls, "wall_cells_on_network": used}
+
+
 def main(argv=None):
     """Train the closure on coarse/fine pairs of random bodies and save it.
 
@@ -275,7 +325,17 @@ def main(argv=None):
     ap.add_argument("--flow-throughs", type=float, default=4.0)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--evaluate", type=float, default=None, metavar="RE",
+                    help="instead of training, compare Cd of the learned closure with the wall law "
+                         "and the twice finer grid at this Reynolds number (the closure file --out)")
     args = ap.parse_args(argv)
+    if args.evaluate is not None:
+        rec = evaluate_cd(args.evaluate, tuple(args.coarse), args.length, args.out,
+                          log=lambda m: print(m, flush=True))
+        print(json.dumps(rec))
+        if args.report:
+            json.dump(rec, open(args.report, "w"), indent=1)
+        return 0
     t0 = time.time()
     pairs = []
     for k, seed in enumerate(list(range(args.train)) + [1000 + j for j in range(args.held)]):
