@@ -146,12 +146,10 @@ def body_cd(grid, params, closure=None):
     return fx * drag.cd_per_lattice_force(grid.frontal_cells, U_IN)
 
 
-def cpu_baseline(grid, seconds_target: float = 15.0):
-    """Oracle (FP64, one thread) stepping the same body grid for about `seconds_target`."""
+def _oracle_mlups(grid, threads: int, seconds_target: float):
     import oracle
-    cells = grid.cells
     cpu = oracle.Solver(grid.layout, TAU0, oracle.REG, True, flags=grid.flags,
-                        equilibrium_state=(1.0, (U_IN, 0.0, 0.0)), threads=1)
+                        equilibrium_state=(1.0, (U_IN, 0.0, 0.0)), threads=threads)
     cpu.step(1)
     steps, elapsed = 0, 0.0
     t0 = time.perf_counter()
@@ -159,10 +157,22 @@ def cpu_baseline(grid, seconds_target: float = 15.0):
         cpu.step(1)
         steps += 1
         elapsed = time.perf_counter() - t0
-    mlups = cells * steps / elapsed / 1e6
+    return grid.cells * steps / elapsed / 1e6, steps, elapsed
+
+
+def cpu_baseline(grid, seconds_target: float = 15.0, parallel_seconds: float = 8.0):
+    """Oracle (FP64) stepping the same body grid for about `seconds_target` on one thread (the
+    reference's CPU path is single-threaded JavaScript), and for about `parallel_seconds` with
+    OpenMP over the host cores this process may use (at most 16, one GPU's share)."""
+    mlups, steps, elapsed = _oracle_mlups(grid, 1, seconds_target)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    par, par_steps, par_elapsed = _oracle_mlups(grid, threads, parallel_seconds)
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
             "sample": f"FP64 C oracle (restated CpuSolver), the same 256x128x128 bluff-body REG+LES "
-                      f"tunnel, {steps} steps in {elapsed:.1f} s on one thread"}
+                      f"tunnel, {steps} steps in {elapsed:.1f} s on one thread",
+            "parallel": {"value": round(par, 3), "unit": "MLUPS", "cores": threads,
+                         "sample": f"the same, OpenMP over {threads} threads, {par_steps} steps in "
+                                   f"{par_elapsed:.1f} s"}}
 
 
 def main():
