@@ -1,8 +1,8 @@
-"""TunnelBatch: many independent tunnels advanced by one persistent launch per step.
+"""TunnelBatch: many independent tunnels advanced by one launch per step.
 
 Each tunnel follows the GpuSolver semantics (src/lbm/gpu-solver.ts) with its own geometry,
-relaxation time and inlet speed; operator, walls, LES and storage are shared. Even/odd step
-pairs replay a captured hipGraph. There is no reference counterpart: the reference runs one
+relaxation time and inlet speed; operator, walls, LES, storage and wall model are shared (the
+learned closure's network too, ``set_closure``). Even/odd step pairs replay a captured hipGraph. There is no reference counterpart: the reference runs one
 solver at a time (src/app/solver-slot.ts:14-44).
 """
 from __future__ import annotations

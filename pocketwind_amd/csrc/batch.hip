@@ -1,13 +1,14 @@
-// batch.hip -- many independent tunnels of uneven size stepped by one persistent launch.
+// batch.hip -- many independent tunnels of uneven size stepped by one launch per step.
 //
 // The reference runs one solver at a time (src/app/solver-slot.ts:14-44); batching has no
 // reference counterpart. Each tunnel keeps the single-solver semantics exactly (the same
-// per-cell step code, its own tau, inlet speed, geometry, ping-pong sets and
-// macros), so a tunnel stepped in a batch is bit-identical to the same tunnel stepped alone.
+// step code, row wavefronts or per cell as the single solver chooses, its own tau, inlet
+// speed, geometry, ping-pong sets, macros and learned-closure output), so a tunnel stepped in
+// a batch is bit-identical to the same tunnel stepped alone.
 //
 // Work decomposition: the cells of all tunnels are cut into tiles of one block's worth of
-// cells; a tile table maps a tile to its tunnel. A persistent grid of blocks strides over the
-// tiles, so one launch advances every tunnel by one step. All tunnels step in lockstep, so
+// cells; a tile table maps a tile to its tunnel and the grid holds one block per tile, so one
+// launch advances every tunnel by one step. All tunnels step in lockstep, so
 // one parity bit selects every tunnel's input set; the two launches of an even/odd step pair
 // are captured once into a hipGraph and replayed.
 #include <hip/hip_runtime.h>

@@ -1,4 +1,4 @@
-// batch_step.h -- the persistent step kernel of batched tunnels and its launch, instantiated
+// batch_step.h -- the step kernel of batched tunnels and its launch, instantiated
 // once per storage type (batch_f32.hip, batch_f16.hip, batch_bf16.hip).
 #pragma once
 

@@ -1,4 +1,4 @@
-"""Batched tunnels: every tunnel advanced inside one persistent launch (and its replayed
+"""Batched tunnels: every tunnel advanced inside one launch per step (and its replayed
 hipGraph) is bit-identical to the same tunnel advanced alone by pw_solver with the same step
 variant, in populations, macros and FP64 force samples."""
 from __future__ import annotations
