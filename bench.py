@@ -146,6 +146,32 @@ def body_cd(grid, params, closure=None):
     return fx * drag.cd_per_lattice_force(grid.frontal_cells, U_IN)
 
 
+def voxelize_timing(layout=LAYOUT, length=BODY_LENGTH, repeats=5):
+    """Device voxelisation of the 217 088-triangle noisy body (pw_voxelize, host copies in and
+    out included, median of `repeats`) beside the oracle's restatement of voxelize.ts on one
+    thread; the reference's own budget is 400 ms for 200k triangles at this grid
+    (voxelize.bench.ts:7-16)."""
+    import oracle
+    from pocketwind_amd import bodies
+    from pocketwind_amd.voxelize import voxelize
+    body, _ = bodies.place_mesh(bodies.noisy_bluff_body(), None, length, layout, False)
+    voxelize(body, None, layout, False, link_fractions=True)  # warm-up (module load, allocations)
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        grid = voxelize(body, None, layout, False, link_fractions=True)
+        times.append((time.perf_counter() - t0) * 1e3)
+    t0 = time.perf_counter()
+    ref = oracle.voxelize(body, None, layout, False, link_fractions=True)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    return {"triangles": int(body.size // 9), "layout": list(layout), "link_fractions": True,
+            "device_ms": round(sorted(times)[len(times) // 2], 2),
+            "bit_exact": bool(np.array_equal(grid.geometry, ref.geometry)
+                              and np.array_equal(grid.boundary_cells, ref.boundary_cells)),
+            "cpu_baseline": {"value_ms": round(cpu_ms, 1), "cores": 1, "kind": "port",
+                             "sample": "FP64 C oracle (restated voxelize.ts and wall-links.ts), the same mesh"}}
+
+
 def _oracle_mlups(grid, threads: int, seconds_target: float):
     import oracle
     cpu = oracle.Solver(grid.layout, TAU0, oracle.REG, True, flags=grid.flags,
@@ -280,6 +306,8 @@ def main():
     net.destroy()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(grid)
+        line["voxelize"] = voxelize_timing()
+        print(f"# voxelize: {line['voxelize']}", file=sys.stderr, flush=True)
     print(json.dumps(line))
     return 0
 

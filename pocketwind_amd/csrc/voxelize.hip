@@ -3,14 +3,15 @@
 //
 // Follows src/geometry/voxelize.ts:23-214 (three-projection parity vote with half-open edge
 // ownership, EQ/floor planes, counts), src/geometry/wall-links.ts:36-131 (pull masks, q
-// offsets, body boundary list, clamped q) and src/geometry/triangle-bins.ts:14-162 (4-cell
-// triangle bins, Moller-Trumbore link hits). Every expression keeps the reference's operand
+// offsets, body boundary list, clamped q) and src/geometry/triangle-bins.ts:14-162 (triangle
+// bins, Moller-Trumbore link hits; the bins here are kBinCells wide, not 4: a link's nearest hit
+// is the minimum over every triangle whose box meets the link's box, which any binning finds). Every expression keeps the reference's operand
 // order and is evaluated in IEEE double with contraction disabled, so no product is fused
 // into an FMA; division is the correctly rounded f64 sequence.
 //
 // GPU mapping: one lane per triangle rasterises its columns (a count pass, a scan, a fill pass
 // into per-column hit lists), one lane per column sorts its hits and votes its cells, one lane
-// per cell classifies, marks pull walls and casts its link rays through the bins.
+// per cell classifies and marks pull walls, one lane per wall link casts its ray through the bins.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -29,6 +30,11 @@ namespace pw {
 namespace vox {
 
 constexpr int kBlock = 256;
+// Width of a triangle bin in cells. A dense mesh puts hundreds of triangles in a 4-cell bin;
+// one-cell bins leave a link a few dozen candidates (measured on the 217 088-triangle body at
+// 256x128x128: the link-fraction pass 28.8 ms with a lane per cell and 4-cell bins, 14.4 ms
+// with a lane per link, and the whole voxelisation 26 ms with 2-cell bins, 23 ms with 1-cell).
+constexpr int kBinCells = 1;
 
 struct Proj {
   int u, v, a;  // axis indices of column u, column v and depth
@@ -208,7 +214,7 @@ __global__ void bin_bounds_kernel(const float* __restrict__ tris, uint32_t n_tri
     const float lo = fminf(fminf(p, q), r), hi = fmaxf(fmaxf(p, q), r);
     bounds[6 * (size_t)t + ax] = lo;
     bounds[6 * (size_t)t + 3 + ax] = hi;
-    long mn = (long)floor((double)lo / 4), mx = (long)floor((double)hi / 4);
+    long mn = (long)floor((double)lo / kBinCells), mx = (long)floor((double)hi / kBinCells);
     lo_b[ax] = mn < 0 ? 0 : mn;
     hi_b[ax] = mx > nb[ax] - 1 ? nb[ax] - 1 : mx;
   }
@@ -253,8 +259,8 @@ __device__ bool nearest_hit(const Bins& b, const double o[3], const double d[3],
   for (int ax = 0; ax < 3; ++ax) {
     lo[ax] = fmin(o[ax], o[ax] + d[ax]);
     hi[ax] = fmax(o[ax], o[ax] + d[ax]);
-    bmin[ax] = (long)floor(lo[ax] / 4);
-    bmax[ax] = (long)floor(hi[ax] / 4);
+    bmin[ax] = (long)floor(lo[ax] / kBinCells);
+    bmax[ax] = (long)floor(hi[ax] / kBinCells);
     if (bmin[ax] < 0) bmin[ax] = 0;
     if (bmax[ax] > nb[ax] - 1) bmax[ax] = nb[ax] - 1;
   }
@@ -278,39 +284,50 @@ __device__ bool nearest_hit(const Bins& b, const double o[3], const double d[3],
   return true;
 }
 
-// wall-links.ts:65-92 and :114-125 for every link of one cell, in ascending direction order.
+// The cell of every wall link: link base[c] + k belongs to cell c (links are packed per cell in
+// cell order, ascending direction order within a cell).
+__global__ void link_cells_kernel(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ bases,
+                                  uint32_t* __restrict__ link_cell, size_t n) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const uint32_t mask = flags[c] >> 3;
+  if (mask == 0) return;
+  const uint32_t base = bases[c], count = __builtin_popcount(mask);
+  for (uint32_t k = 0; k < count; ++k) link_cell[base + k] = (uint32_t)c;
+}
+
+// wall-links.ts:65-92 and :114-125, one lane per wall link (the links are sparse in the lattice:
+// a lane per cell would leave most lanes of a wavefront idle while a few cast their rays).
 __global__ void link_fraction_kernel(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ bases,
+                                     const uint32_t* __restrict__ link_cell, uint32_t n_links,
                                      float* __restrict__ q, uint32_t nx, uint32_t ny, uint32_t nz, int floor,
                                      int measure, Bins body, Bins supports, int have_supports,
                                      unsigned long long* __restrict__ missed) {
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t N = (size_t)nx * ny * nz;
-  if (c >= N) return;
+  const uint32_t link = blockIdx.x * blockDim.x + threadIdx.x;
+  if (link >= n_links) return;
+  const size_t c = link_cell[link];
   const uint32_t flag = flags[c];
-  if ((flag >> 3) == 0) return;
-  const uint32_t base = bases[c];
+  // Direction of the link: the (link - base)-th set wall bit of the cell, in ascending order.
+  uint32_t rank = link - bases[c];
+  int i = 1;
+  for (; i < kQ; ++i)
+    if (has_wall(flag, i) && rank-- == 0) break;
   const long x = c % nx, y = (c / nx) % ny, z = c / ((size_t)nx * ny);
-  uint32_t k = 0;
-  unsigned long long misses = 0;
-  for (int i = 1; i < kQ; ++i) {
-    if (!has_wall(flag, i)) continue;
-    double frac = 0.5;
-    if (measure) {
-      const long wx = x - cx(i), wy = ((y - cy(i)) % (long)ny + ny) % ny, wz = ((z - cz(i)) % (long)nz + nz) % nz;
-      const size_t w = wx + (size_t)nx * (wy + (size_t)ny * wz);
-      const uint32_t wt = flags[w] & kTypeMask;
-      if (wt != kFloor) {
-        const Bins& bins = (wt == kSupport && have_supports) ? supports : body;
-        const double o[3] = {(double)wx + cx(i), (double)wy + cy(i), (double)wz + cz(i)};
-        const double d[3] = {(double)-cx(i), (double)-cy(i), (double)-cz(i)};
-        double t;
-        if (nearest_hit(bins, o, d, &t)) frac = t;
-        else ++misses;
-      }
+  double frac = 0.5;
+  if (measure) {
+    const long wx = x - cx(i), wy = ((y - cy(i)) % (long)ny + ny) % ny, wz = ((z - cz(i)) % (long)nz + nz) % nz;
+    const size_t w = wx + (size_t)nx * (wy + (size_t)ny * wz);
+    const uint32_t wt = flags[w] & kTypeMask;
+    if (wt != kFloor) {
+      const Bins& bins = (wt == kSupport && have_supports) ? supports : body;
+      const double o[3] = {(double)wx + cx(i), (double)wy + cy(i), (double)wz + cz(i)};
+      const double d[3] = {(double)-cx(i), (double)-cy(i), (double)-cz(i)};
+      double t;
+      if (nearest_hit(bins, o, d, &t)) frac = t;
+      else atomicAdd(missed, 1ull);
     }
-    q[base + k++] = (float)fmax(1e-3, fmin(1.0, frac));
   }
-  if (misses) atomicAdd(missed, misses);
+  q[link] = (float)fmax(1e-3, fmin(1.0, frac));
 }
 
 __global__ void bases_kernel(const uint32_t* __restrict__ count, const uint32_t* __restrict__ prefix,
@@ -399,9 +416,9 @@ hipError_t vote_mesh(Arena& ar, const float* tris, uint32_t n_tris, const uint32
 hipError_t build_bins(Arena& ar, const float* tris, uint32_t n_tris, const uint32_t size[3], hipStream_t st, Bins* out) {
   Bins b{};
   b.tris = tris;
-  b.nbx = (size[0] + 3) / 4;
-  b.nby = (size[1] + 3) / 4;
-  b.nbz = (size[2] + 3) / 4;
+  b.nbx = (size[0] + kBinCells - 1) / kBinCells;
+  b.nby = (size[1] + kBinCells - 1) / kBinCells;
+  b.nbz = (size[2] + kBinCells - 1) / kBinCells;
   const size_t nbins = (size_t)b.nbx * b.nby * b.nbz;
   float* bounds = ar.alloc<float>(6 * (size_t)n_tris);
   uint32_t* count = ar.alloc<uint32_t>(nbins + 1);
@@ -509,8 +526,14 @@ extern "C" pw_status pw_voxelize(const float* body, size_t body_tris, const floa
       if ((e = build_bins(ar, d_body, (uint32_t)body_tris, size, st, &bb)) != hipSuccess) goto fail;
       if (supports && (e = build_bins(ar, d_sup, (uint32_t)support_tris, size, st, &sb)) != hipSuccess) goto fail;
     }
-    hipLaunchKernelGGL(link_fraction_kernel, dim3(grid_for(N)), dim3(kBlock), 0, st, flags, bases, q, size[0], size[1],
-                       size[2], floor, link_fractions, bb, sb, supports ? 1 : 0, totals + 3);
+    if (n_links) {
+      uint32_t* link_cell = ar.alloc<uint32_t>(n_links);
+      if ((e = ar.err) != hipSuccess) goto fail;
+      hipLaunchKernelGGL(link_cells_kernel, dim3(grid_for(N)), dim3(kBlock), 0, st, flags, bases, link_cell, N);
+      hipLaunchKernelGGL(link_fraction_kernel, dim3(grid_for(n_links)), dim3(kBlock), 0, st, flags, bases, link_cell,
+                         (uint32_t)n_links, q, size[0], size[1], size[2], floor, link_fractions, bb, sb,
+                         supports ? 1 : 0, totals + 3);
+    }
     if ((e = hipGetLastError()) != hipSuccess) goto fail;
     out->geometry_words = 2 * N + n_links;
     out->geometry = (uint32_t*)std::malloc(out->geometry_words * 4);
