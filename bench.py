@@ -150,7 +150,7 @@ def voxelize_timing(layout=LAYOUT, length=BODY_LENGTH, repeats=5):
     """Device voxelisation of the 217 088-triangle noisy body (pw_voxelize, host copies in and
     out included, median of `repeats`) beside the oracle's restatement of voxelize.ts on one
     thread; the reference's own budget is 400 ms for 200k triangles at this grid
-    (voxelize.bench.ts:7-16)."""
+    (tests/unit/voxelize/voxelize.bench.ts:7-16)."""
     import oracle
     from pocketwind_amd import bodies
     from pocketwind_amd.voxelize import voxelize
