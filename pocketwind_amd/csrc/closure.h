@@ -136,22 +136,27 @@ __device__ __forceinline__ void wall_features(const typename S::T* __restrict__ 
   }
 }
 
-// One wavefront per 64 wall cells: features (optionally written raw to `records`), then, when
+// Wall cells per wavefront: one 16-row matrix-core tile. The wall cells are few (thousands),
+// so one tile per wavefront spreads them over many more wavefronts than four tiles would, and
+// each wavefront runs one network pass instead of four in sequence.
+constexpr int kCellsPerWave = 16;
+
+// One wavefront per 16 wall cells: features (optionally written raw to `records`), then, when
 // a network is attached (P non-null), its stress for every cell into tau (w = 1 valid, 0 use
-// the wall law).
+// the wall law). Lanes 0-15 own the cells; all 64 lanes run the network's tile.
 template <class S, bool BOUZIDI>
 __global__ __launch_bounds__(64) void closure_kernel(const typename S::T* __restrict__ in,
                                                      const uint32_t* __restrict__ geo, StepParams p,
                                                      const uint32_t* __restrict__ cells, uint32_t n_cells,
                                                      const mlp::Packed* __restrict__ P, Norm norm,
                                                      float4* __restrict__ tau, float* __restrict__ records) {
-  __shared__ __attribute__((aligned(16))) uint16_t xs[64 * mlp::kInPad];
+  __shared__ __attribute__((aligned(16))) uint16_t xs[kCellsPerWave * mlp::kInPad];
   __shared__ __attribute__((aligned(16))) uint16_t h1[16 * mlp::kHidden];
   __shared__ __attribute__((aligned(16))) uint16_t h2[16 * mlp::kHidden];
-  __shared__ float ys[64 * 4];
+  __shared__ float ys[kCellsPerWave * 4];
   const int l = threadIdx.x;
-  const uint32_t k = blockIdx.x * 64 + l;
-  const bool valid = k < n_cells;
+  const uint32_t k = blockIdx.x * kCellsPerWave + l;
+  const bool valid = l < kCellsPerWave && k < n_cells;
   float f[kFeatures], law[3];
   bool in_range = valid;
   if (valid) {
@@ -162,22 +167,21 @@ __global__ __launch_bounds__(64) void closure_kernel(const typename S::T* __rest
     }
   }
   if (!P) return;
-  for (int j = 0; j < mlp::kInPad; ++j) {
-    float v = 0.0f;
-    if (valid && j < kFeatures) {
-      v = (f[j] - norm.mean[j]) * norm.scale[j];
-      in_range = in_range && fabsf(v) <= norm.limit;
+  if (l < kCellsPerWave)
+    for (int j = 0; j < mlp::kInPad; ++j) {
+      float v = 0.0f;
+      if (valid && j < kFeatures) {
+        v = (f[j] - norm.mean[j]) * norm.scale[j];
+        in_range = in_range && fabsf(v) <= norm.limit;
+      }
+      xs[l * mlp::kInPad + j] = mlp::bf16_bits(v);
     }
-    xs[l * mlp::kInPad + j] = mlp::bf16_bits(v);
-  }
   mlp::wave_sync();
   const int r = l & 15, gq = l >> 4;
-  for (int t = 0; t < 4; ++t) {
-    const mlp::f32x4_t y = mlp::forward16(P, xs + 16 * t * mlp::kInPad, h1, h2);
-    if (r < 3)
-      for (int j = 0; j < 4; ++j) ys[(16 * t + 4 * gq + j) * 4 + r] = y[j];
-    mlp::wave_sync();
-  }
+  const mlp::f32x4_t y = mlp::forward16(P, xs, h1, h2);
+  if (r < 3)
+    for (int j = 0; j < 4; ++j) ys[(4 * gq + j) * 4 + r] = y[j];
+  mlp::wave_sync();
   if (!valid) return;
   const float tx = ys[l * 4] * norm.out_scale, ty = ys[l * 4 + 1] * norm.out_scale, tz = ys[l * 4 + 2] * norm.out_scale;
   const bool ok = in_range && isfinite(tx) && isfinite(ty) && isfinite(tz);

@@ -16,7 +16,7 @@ using closure::Norm;
 
 namespace {
 
-inline unsigned closure_blocks(size_t n) { return (unsigned)((n + 63) / 64); }
+inline unsigned closure_blocks(size_t n) { return (unsigned)((n + closure::kCellsPerWave - 1) / closure::kCellsPerWave); }
 
 template <class S>
 void launch_closure_s(pw_solver* s, const void* state, const void* net, float4* tau, float* records) {
