@@ -131,3 +131,17 @@ def test_batched_tunnels_take_the_reference_boundaries_only():
     grid = blob_grid("bb")
     with pytest.raises(ValueError, match="periodic"):
         TunnelBatch.create([grid], [0.8], [0.06], SolverParams(0.8, 0.06, False, sides="free-slip"))
+
+
+def test_batch_closure_particles_and_force_entry_points_reject_null_handles():
+    lib = N.lib()
+    norm = np.zeros(27, np.float32)
+    out = C.c_void_p()
+    lo = (C.c_double * 3)(1, 1, 1)
+    hi = (C.c_double * 3)(2, 2, 2)
+    assert lib.pw_batch_set_closure(None, None, norm.ctypes.data, 27) == N.PW_EINVAL
+    assert lib.pw_batch_set_closure_interval(None, 8) == N.PW_EINVAL
+    assert lib.pw_particles_create_batch(None, 0, 16, 1, lo, hi, C.byref(out)) == N.PW_EINVAL
+    assert out.value is None
+    buf = np.zeros(3)
+    assert lib.pw_batch_force(None, buf.ctypes.data, 3) == N.PW_EINVAL
