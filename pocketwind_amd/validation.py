@@ -10,10 +10,14 @@ CaseSpec and the case builders
                    confinement, sensitivity and smoke cases; flow-through and window lengths)
 row_from_run       rowFromRun (src/validation/runner.ts:258-321): window mean of the force,
                    Cd on the stated and voxel areas, Maskell bound, reference error, pass rules
-run_laminar_sphere the laminar sphere case run on this solver (src/app/case-run.ts:200-250):
-                   forces in newtons every 8 steps
+case_scales, case_meshes, case_grid, run_case
+                   one case run on this solver as runCase does it (src/app/case-run.ts:97-250,
+                   src/validation/links.ts:14-35): Reynolds number and lattice viscosity, the
+                   placed and voxelised preset with its floor and stilts, forces in newtons
+                   every 8 steps
 score_results      scripts/score-results.mjs: match rows to reference values by case
-main               `python -m pocketwind_amd.validation score <file>`
+main               `python -m pocketwind_amd.validation score <file>`, and
+                   `python -m pocketwind_amd.validation run <out.json> [ids] [--tiers ...]`
 """
 from __future__ import annotations
 
@@ -24,6 +28,8 @@ import time
 from dataclasses import asdict, dataclass, field, replace
 from decimal import ROUND_HALF_UP, Decimal
 from typing import Dict, List, Optional, Sequence, Tuple, Union
+
+import numpy as np
 
 from . import drag
 from .field_export import js_number
@@ -466,27 +472,62 @@ def row_from_run(spec: CaseSpec, run: SolverRun, context: RunContext) -> dict:
     return {k: row[k] for k in order}
 
 
-def run_laminar_sphere(spec: CaseSpec, steps: Optional[int] = None) -> SolverRun:
-    """The laminar sphere case on this solver, as runCase does it (case-run.ts:200-250): the
-    24 x 16 lat-long preset with D = bodyLengthCells placed at 0.25 nx, lattice viscosity
-    u_lu D / Re (no floor), a force sample every 8 steps converted to newtons with
-    rho_air (speed / u_lu)^2 dx^2, and the measured frontal area and blockage."""
-    if spec.case not in ("sphere-re-100", "sphere-confinement"):
-        raise ValueError("run_laminar_sphere runs the laminar sphere cases.")
+def case_scales(spec: CaseSpec) -> Tuple[float, float, float]:
+    """caseScales (case-run.ts:97-112) on the parameters caseHash (links.ts:14-35) passes:
+    (Re, speed in m/s, lattice viscosity). Only the sphere cases pass their Reynolds number;
+    the others take it from the speed and the body length in millimetres (six decimals)."""
+    length_m = float(f"{spec.bodyLengthMm:.6f}") / 1000
+    passed_re = spec.re if spec.shape == "sphere" else None
+    re = passed_re if passed_re is not None else spec.speedMps * length_m / NU_AIR
+    speed = spec.speedMps if passed_re is None else re * NU_AIR / length_m
+    return re, speed, spec.uLu * spec.bodyLengthCells / re
+
+
+def case_meshes(spec: CaseSpec):
+    """voxelBody's placement (case-run.ts:121-147): (body, supports) of the case's preset
+    placed at bodyLengthCells with the case's floor. The Ahmed body's stilts are support
+    cells, left out of the drag, unless the stilts sensitivity row counts them (case-run.ts:93,
+    links.ts:32): then they are joined to the body."""
     from . import bodies
-    from .gpu_solver import GpuSolver, SolverParams
-    from .voxelize import voxelize
+    mesh = bodies.preset_mesh(spec.shape)
+    if mesh is None:
+        raise ValueError(f"The preset {spec.shape} has no built-in geometry.")
     layout = (spec.grid["nx"], spec.grid["ny"], spec.grid["nz"])
-    body, _ = bodies.place_mesh(bodies.sphere_mesh(), None, spec.bodyLengthCells, layout, False)
-    grid = voxelize(body, None, layout, False, link_fractions=spec.walls == "bouzidi")
-    viscosity = spec.uLu * spec.bodyLengthCells / spec.re
+    body, supports = bodies.place_mesh(mesh, bodies.preset_supports(spec.shape), spec.bodyLengthCells, layout,
+                                       spec.floor)
+    if spec.sensitivity == "stilts" and spec.stilts and supports is not None:
+        return np.concatenate([body, supports]), None
+    return body, supports
+
+
+def case_grid(spec: CaseSpec):
+    """The case's grid from the device voxelizer (link fractions for interpolated walls)."""
+    from .voxelize import voxelize
+    body, supports = case_meshes(spec)
+    layout = (spec.grid["nx"], spec.grid["ny"], spec.grid["nz"])
+    grid = voxelize(body, supports, layout, spec.floor, link_fractions=spec.walls == "bouzidi")
+    if grid.solid_cells == 0:
+        raise ValueError("The body covers no grid cells at this size.")
+    return grid
+
+
+def run_case(spec: CaseSpec, steps: Optional[int] = None) -> SolverRun:
+    """One case on this solver, as runCase does it (case-run.ts:200-250): the case's grid,
+    tau0 = 3 nu + 1/2 from its own Reynolds number (no interactive floor), its operator, walls,
+    storage and LES switch, a force sample every 8 steps converted to newtons with
+    rho_air (speed / u_lu)^2 dx^2, and the measured frontal area and blockage. A non-finite
+    force ends the run with an error, as the reference's does."""
+    from .gpu_solver import GpuSolver, SolverParams
+    layout = (spec.grid["nx"], spec.grid["ny"], spec.grid["nz"])
+    grid = case_grid(spec)
+    _, speed, viscosity = case_scales(spec)
     params = SolverParams(3 * viscosity + 0.5, spec.uLu, spec.les, operator=spec.operator, walls=spec.walls,
                           precision=spec.storage)
     solver = GpuSolver.create(layout, grid, params)
     if not isinstance(solver, GpuSolver):
         raise MemoryError(solver.message)
     total = spec.steps if steps is None else steps
-    newtons = RHO_AIR * (spec.speedMps / spec.uLu) ** 2 * spec.cellSizeM ** 2
+    newtons = RHO_AIR * (speed / spec.uLu) ** 2 * spec.cellSizeM ** 2
     try:
         started = time.perf_counter()
         samples = solver.step_sampled(total, FORCE_SAMPLE_STEPS)
@@ -500,6 +541,13 @@ def run_laminar_sphere(spec: CaseSpec, steps: Optional[int] = None) -> SolverRun
         forces.append(ForceSample(int(step), fx * newtons, fy * newtons, fz * newtons))
     return SolverRun(forces, total, 1000 * elapsed / total, voxelAreaM2=grid.frontal_cells * spec.cellSizeM ** 2,
                      blockage=grid.frontal_cells / (layout[1] * layout[2]))
+
+
+def run_laminar_sphere(spec: CaseSpec, steps: Optional[int] = None) -> SolverRun:
+    """The laminar sphere cases (the 24 x 16 lat-long preset, D = bodyLengthCells, Re 100)."""
+    if spec.case not in ("sphere-re-100", "sphere-confinement"):
+        raise ValueError("run_laminar_sphere runs the laminar sphere cases.")
+    return run_case(spec, steps)
 
 
 def convergence_rows(rows: Sequence[dict], operators: Sequence[str] = OPERATORS) -> List[dict]:
@@ -597,16 +645,25 @@ def foreign_results(solver: str, rows: Sequence[dict]) -> dict:
                                         "relStdPct": r["relStdPct"], "steps": r["steps"]} for r in rows]}
 
 
-def main(argv: Optional[List[str]] = None) -> int:
-    argv = sys.argv[1:] if argv is None else argv
-    if len(argv) != 2 or argv[0] != "score":
-        sys.stderr.write("Usage: python -m pocketwind_amd.validation score <results.json | foreign-results.json>\n")
-        return 2
+def run_cases(specs: Sequence[CaseSpec], context: RunContext, log=print) -> List[dict]:
+    """Run each case on this solver and turn it into a result row (the force history kept)."""
+    rows = []
+    for spec in specs:
+        started = time.perf_counter()
+        row = row_from_run(spec, run_case(spec), context)
+        log(f"{spec.id}: Cd {row['cd']:.6f} (voxel {row['cdVoxel']:.6f}), relStd {row['relStdPct']:.3f} %, "
+            f"drift {row['driftPct']:.3f} %, {row['msPerStep']:.3f} ms/step, "
+            f"{time.perf_counter() - started:.1f} s")
+        rows.append(row)
+    return rows
+
+
+def _score(path: str) -> int:
     try:
-        with open(argv[1], encoding="utf-8") as f:
+        with open(path, encoding="utf-8") as f:
             parsed = json.load(f)
     except (OSError, ValueError) as e:
-        sys.stderr.write(f"score-results: cannot read {argv[1]}: {e}\n")
+        sys.stderr.write(f"score-results: cannot read {path}: {e}\n")
         return 1
     out, err, status = score_results(parsed)
     for line in err:
@@ -614,6 +671,45 @@ def main(argv: Optional[List[str]] = None) -> int:
     for line in out:
         sys.stdout.write(line + "\n")
     return status
+
+
+def _run(argv: List[str]) -> int:
+    import argparse
+    import datetime
+    ap = argparse.ArgumentParser(prog="python -m pocketwind_amd.validation run",
+                                 description="Run validation cases on the device and write their rows.")
+    ap.add_argument("out", help="output file: a foreign results document with the full rows under 'results'")
+    ap.add_argument("ids", nargs="*", help="case ids (default: every default row of the chosen tiers)")
+    ap.add_argument("--tiers", nargs="+", default=list(DEFAULT_TIERS), choices=list(TIERS))
+    a = ap.parse_args(argv)
+    cases = build_cases(a.tiers) + [smoke_case()]
+    if a.ids:
+        known = {c.id: c for c in cases}
+        unknown = [i for i in a.ids if i not in known]
+        if unknown:
+            sys.stderr.write(f"Unknown case ids: {', '.join(unknown)}\n")
+            return 2
+        specs = [known[i] for i in a.ids]
+    else:
+        specs = [c for c in cases if c.isDefault]
+    date = datetime.datetime.now(datetime.timezone.utc).strftime("%Y-%m-%dT%H:%M:%S.000Z")
+    rows = run_cases(specs, RunContext("pocketwind_amd", date), log=lambda m: print(m, flush=True))
+    doc = foreign_results("pocketwind_amd", rows)
+    doc["results"] = [{k: v for k, v in r.items() if k != "forceHistory"} for r in rows]
+    with open(a.out, "w", encoding="utf-8") as f:
+        json.dump(doc, f, indent=1)
+    return 0
+
+
+def main(argv: Optional[List[str]] = None) -> int:
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) == 2 and argv[0] == "score":
+        return _score(argv[1])
+    if argv and argv[0] == "run":
+        return _run(argv[1:])
+    sys.stderr.write("Usage: python -m pocketwind_amd.validation score <results.json | foreign-results.json>\n"
+                     "       python -m pocketwind_amd.validation run <out.json> [case ids] [--tiers S M L XL]\n")
+    return 2
 
 
 if __name__ == "__main__":

@@ -144,6 +144,25 @@ def test_rows_from_the_stored_mean_forces_reproduce_the_stored_verdicts(row):
     assert (out["fx"], out["fy"], out["fz"]) == (row["fx"], row["fy"], row["fz"])
 
 
+@pytest.mark.parametrize("row", STORED, ids=[r["id"] for r in STORED])
+def test_case_placement_reproduces_every_stored_voxel_area_and_blockage(row):
+    """The placed preset (floor, stilts as supports or counted in the drag) voxelised by the
+    oracle, which the device voxelizer matches bit for bit, covers exactly the frontal cells of
+    the reference's run; the case's Reynolds number and speed are the stored ones."""
+    import oracle
+    spec = spec_for(row)
+    body, supports = V.case_meshes(spec)
+    layout = (spec.grid["nx"], spec.grid["ny"], spec.grid["nz"])
+    grid = oracle.voxelize(body, supports, layout, spec.floor, link_fractions=False)
+    assert grid.frontal_cells * spec.cellSizeM ** 2 == row["voxelAreaM2"]
+    assert grid.frontal_cells / (layout[1] * layout[2]) == row["blockage"]
+    assert (supports is None) == (spec.shape != "ahmed-25" or row["id"] == "sensitivity-stilts-included")
+    re, speed, _ = V.case_scales(spec)
+    assert re == pytest.approx(row["re"] if row["re"] is not None else 40 * row["bodyLengthMm"] / 1000 / 1.5e-5,
+                               rel=1e-15)
+    assert speed == row["speedMps"]
+
+
 # ------------------------------------------------------------------------- runner unit cases
 
 CONTEXT = V.RunContext("abc1234", "2026-11-06T00:00:00.000Z", {"vendor": "test"}, "test-agent")
