@@ -6,15 +6,17 @@ results.json).
 Laminar sphere at Re 100, tier S: a steady flow, so Cd within 0.5 % of the stored row; the rows
 are then written as a foreign results document and scored.
 
-Turbulent rows (Ahmed body at Re 2.78e6 with floor and stilts, cube face-on, sphere at Re 20 000;
-REG + Smagorinsky LES with the wall law; tiers S to XL, f32 and the f16 row): the same voxel area
-and blockage, the same pass verdict, and Cd within 3 % of the stored row. Both runs use the same
+Every stored validation, confinement and sensitivity row: the same voxel area and blockage and
+the same pass verdict. The laminar sphere in the XL domain is steady: Cd within 0.5 %. The
+turbulent rows (Ahmed body at Re 2.78e6 with floor and stilts, cube face-on, sphere at Re 20 000;
+REG + Smagorinsky LES with the wall law; tiers S to XL, f32 and the f16 row; the domain, body
+length, inlet speed and stilts sensitivity rows): Cd within 3 % of the stored row. Both runs use the same
 scheme from the same initial state, but the reference sums in a different float32 order, so the
 chaotic wakes decorrelate within a flow-through and the two Cd values are two samples of a window
 mean of a signal whose relative standard deviation is 1.2-4 % (the rows' relStdPct); over the
 window's two flow-throughs that mean carries about 1 % sampling error, so two runs differ by
-about 1.4 % rms and 3 % is about two of those. Measured: at most 1.84 % (Ahmed, tier S), every
-M, L and XL row within 0.7 %."""
+about 1.4 % rms and 3 % is about two of those. Measured on the validation rows: at most 1.84 %
+(Ahmed, tier S), every M, L and XL row within 0.7 %."""
 from __future__ import annotations
 
 import json
@@ -28,11 +30,8 @@ pytestmark = pytest.mark.gpu
 
 REF = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_results.json")))
 STORED = {r["id"]: r for r in REF["laminar"]}
-TURBULENT = {r["id"]: r for r in REF["validation"]}
+OTHER = {r["id"]: r for s in ("validation", "confinement", "sensitivity") for r in REF[s]}
 CASES = {c.id: c for c in V.build_cases() + [V.smoke_case()]}
-TURBULENT_IDS = ["smoke-ahmed-25-S", "ahmed-25-S-reg-default", "cube-S", "sphere-S", "ahmed-25-M-reg-default",
-                 "cube-M", "sphere-M", "ahmed-25-L-reg-default", "ahmed-25-L-storage-f16", "cube-L", "sphere-L",
-                 "ahmed-25-XL-reg-default"]
 
 
 def test_laminar_sphere_rows_match_the_reference_rows():
@@ -56,15 +55,15 @@ def test_laminar_sphere_rows_match_the_reference_rows():
     assert status == 0 and out[-1] == "unmatched: 0" and len(out) == 3
 
 
-@pytest.mark.parametrize("case_id", TURBULENT_IDS)
-def test_turbulent_rows_match_the_reference_rows(case_id):
-    spec, ref = CASES[case_id], TURBULENT[case_id]
+@pytest.mark.parametrize("case_id", list(OTHER))
+def test_validation_confinement_and_sensitivity_rows_match_the_reference_rows(case_id):
+    spec, ref = CASES[case_id], OTHER[case_id]
     row = V.row_from_run(spec, V.run_case(spec), V.RunContext("test", "2026-01-01T00:00:00.000Z"))
     print(f"{case_id}: Cd {row['cd']:.6f} (reference {ref['cd']:.6f}, {100 * (row['cd'] / ref['cd'] - 1):+.2f} %), "
           f"relStd {row['relStdPct']:.3f} % (reference {ref['relStdPct']:.3f} %), "
           f"{row['msPerStep']:.3f} ms/step (reference {ref['msPerStep']:.3f})")
     assert (row["voxelAreaM2"], row["blockage"]) == (ref["voxelAreaM2"], ref["blockage"])
-    assert abs(row["cd"] / ref["cd"] - 1) < 0.03
+    assert abs(row["cd"] / ref["cd"] - 1) < (0.03 if spec.les else 0.005)
     # Ahmed rows fail against the experiment's 0.285 in both; the reason quotes each run's own error.
     assert row["pass"] == ref["pass"] and len(row["reasons"]) == len(ref["reasons"])
     assert len(row["forceHistory"]) == spec.steps // 8 and row["steps"] == spec.steps
