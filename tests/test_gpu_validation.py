@@ -67,3 +67,24 @@ def test_validation_confinement_and_sensitivity_rows_match_the_reference_rows(ca
     # Ahmed rows fail against the experiment's 0.285 in both; the reason quotes each run's own error.
     assert row["pass"] == ref["pass"] and len(row["reasons"]) == len(ref["reasons"])
     assert len(row["forceHistory"]) == spec.steps // 8 and row["steps"] == spec.steps
+
+
+SKIPPED = [r for r in REF["skipped"] if "operator" in r]
+
+
+@pytest.mark.parametrize("row", SKIPPED, ids=[f"{r['case']}-{r['tier']}-{r['operator']}" for r in SKIPPED])
+def test_the_reference_s_unstable_trt_runs_fail_here_too(row):
+    """The reference skips two TRT runs whose force turned non-finite (a cube in the XL domain at
+    step 616, the Ahmed body at tier L at step 552); TRT at tau0 = 1/2 + 3 nu with nu near 1e-6 has
+    no stabilising viscosity. Here the same cases end with the same error, at a step within 15 % of
+    the reference's (the blow-up grows from float32 rounding, whose order differs between the two
+    implementations). Measured: steps 672 and 576."""
+    import re as regex
+    spec = next(c for c in V.build_cases() if c.case == row["case"] and c.tier == row["tier"]
+                and c.operator == row["operator"] and c.group == row["group"])
+    with pytest.raises(ValueError, match=r"The force was not finite at step \d+ of \d+\.") as err:
+        V.run_case(spec)
+    step = int(regex.search(r"step (\d+)", str(err.value)).group(1))
+    ref_step = int(regex.search(r"step (\d+)", row["reason"]).group(1))
+    print(f"{spec.id}: {err.value} (reference: {row['reason']})")
+    assert str(err.value).endswith(f"of {spec.steps}.") and abs(step / ref_step - 1) < 0.15
