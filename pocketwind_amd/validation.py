@@ -15,6 +15,8 @@ case_scales, case_meshes, case_grid, run_case
                    src/validation/links.ts:14-35): Reynolds number and lattice viscosity, the
                    placed and voxelised preset with its floor and stilts, forces in newtons
                    every 8 steps
+run_grid_check     the app's grid check (src/app/grid-check.ts, case-run.ts:257-320) on this
+                   solver: S, M, L at the live run's Re and body share, band from the means
 score_results      scripts/score-results.mjs: match rows to reference values by case
 main               `python -m pocketwind_amd.validation score <file>`, and
                    `python -m pocketwind_amd.validation run <out.json> [ids] [--tiers ...]`
@@ -548,6 +550,110 @@ def run_laminar_sphere(spec: CaseSpec, steps: Optional[int] = None) -> SolverRun
     if spec.case not in ("sphere-re-100", "sphere-confinement"):
         raise ValueError("run_laminar_sphere runs the laminar sphere cases.")
     return run_case(spec, steps)
+
+
+# --------------------------------------------------------------------------------------------
+# Grid check (the app's "Check grid" button)
+
+GRID_CHECK_TIERS = ("S", "M", "L")  # grid-check.ts:13
+GRID_CHECK_FLOW_THROUGHS = 4  # grid-check.ts:15
+NU_MIN_LU = 1e-2  # units.ts:7, the interactive viscosity floor
+LENGTH_CAP_NX = 0.4  # transform.ts:17
+BLOCKAGE_MAX = 0.08  # transform.ts:18
+DEFAULT_SPEED_MPS = 40  # params.ts:4
+# Preset body length and registry reference area (params.ts:18-19).
+PRESET_SIZES = {"ahmed-25": (1044.0, 112_000.0), "sphere": (100.0, math.pi * 50 * 50)}
+
+
+def default_length_cells(mesh, layout) -> float:
+    """defaultLengthCells (transform.ts:80-91): the longest extent at 0.4 nx, or less so that
+    the frontal box stays below 8 % blockage (rounded down to whole cells)."""
+    from . import bodies
+    nx, ny, nz = layout
+    lo, hi = bodies.mesh_bounds(mesh)
+    ext = hi - lo
+    longest = float(max(ext))
+    frontal = float(min(ext[0], ext[1]) * ext[2])
+    blockage_length = math.inf if frontal == 0 else math.floor(longest * math.sqrt(BLOCKAGE_MAX * ny * nz / frontal))
+    return min(LENGTH_CAP_NX * nx, blockage_length)
+
+
+def run_length_cells(preset: str, mesh, layout) -> float:
+    """runLengthCells (run-plan.ts:34-45): the Ahmed body at half the streamwise domain."""
+    if preset == "ahmed-25":
+        return max(1, drag.round_half_up(layout[0] * AHMED_BODY_LENGTH_FRACTION))
+    return default_length_cells(mesh, layout)
+
+
+def grid_check_length_cells(length_cells: float, nx: int, tier: str) -> int:
+    """gridCheckLengthCells (grid-check.ts:65-71): the same body share of the domain on every grid."""
+    if not (length_cells > 0 and nx > 0):
+        raise ValueError("The run being checked needs a positive body length and grid length.")
+    return max(1, drag.round_half_up(length_cells / nx * TIERS[tier].nx))
+
+
+def grid_check_steps(tier: str, flow_throughs: float = GRID_CHECK_FLOW_THROUGHS) -> Tuple[int, int]:
+    """gridCheckSteps (grid-check.ts:74-86): (total steps, newest steps averaged)."""
+    if not (math.isfinite(flow_throughs) and flow_throughs > 0):
+        raise ValueError("The run length must be a positive number of flow-throughs.")
+    flow = drag.flow_through_steps(TIERS[tier].nx)
+    return math.ceil(flow_throughs * flow), math.ceil(flow_throughs / 2 * flow)
+
+
+def mean_of_last(history: Sequence[Tuple[int, float]], average_steps: int) -> Optional[float]:
+    """meanOfLast (grid-check.ts:131-141): mean Cd of the points after the last step minus
+    `average_steps`, or None when nothing finite is left."""
+    if not history:
+        return None
+    last = history[-1][0]
+    window = [cd for step, cd in history if step > last - average_steps]
+    mean = _mean(window)
+    return mean if math.isfinite(mean) else None
+
+
+def run_grid_check(preset: str, current_tier: str = "S", floor: bool = True, storage: str = "f16",
+                   flow_throughs: float = GRID_CHECK_FLOW_THROUGHS, log=None) -> dict:
+    """runBodyGridCheck (case-run.ts:257-320) with runGridCheck (grid-check.ts:201-228) on this
+    solver, at the app's defaults (40 m/s, floor on, REG + LES, halfway walls, 16-bit storage
+    where the device has it): the run at `current_tier` fixes Re = u_lu L / nu with the
+    interactive floor nu >= 1e-2; tiers S, M and L then run the body at the same share of the
+    domain at that Re for `flow_throughs` flow-throughs each, Cd on the preset's registry area
+    in cells, averaged over the newest half; the band is read off the three means."""
+    from . import bodies
+    from .gpu_solver import GpuSolver, SolverParams
+    from .voxelize import voxelize
+    length_mm, area_mm2 = PRESET_SIZES[preset]
+    mesh, supports = bodies.preset_mesh(preset), bodies.preset_supports(preset)
+    current = TIERS[current_tier]
+    current_cells = run_length_cells(preset, mesh, (current.nx, current.ny, current.nz))
+    nu_live = max(NU_MIN_LU, U_LU * current_cells / (DEFAULT_SPEED_MPS * (length_mm / 1000) / NU_AIR))
+    re = U_LU * current_cells / nu_live
+    rows = []
+    for tier in GRID_CHECK_TIERS:
+        t = TIERS[tier]
+        layout = (t.nx, t.ny, t.nz)
+        length = grid_check_length_cells(current_cells, current.nx, tier)
+        body, placed = bodies.place_mesh(mesh, supports, length, layout, floor)
+        grid = voxelize(body, placed, layout, floor, link_fractions=False)
+        cell_mm = length_mm / length
+        cd_per_force = 1 / (0.5 * U_LU * U_LU * (area_mm2 / (cell_mm * cell_mm)))
+        params = SolverParams(3 * (U_LU * length / re) + 0.5, U_LU, True, operator=default_operator(True),
+                              walls=DEFAULT_WALLS, precision=storage)
+        solver = GpuSolver.create(layout, grid, params)
+        if not isinstance(solver, GpuSolver):
+            raise MemoryError(solver.message)
+        total, average = grid_check_steps(tier, flow_throughs)
+        try:
+            samples = solver.step_sampled(total, FORCE_SAMPLE_STEPS)
+        finally:
+            solver.destroy()
+        history = [(int(step), fx * cd_per_force) for step, fx, _, _ in samples]
+        cd = mean_of_last(history, average)
+        if log is not None:
+            log(f"grid check {preset} tier {tier}: {length} cells, Re {re:g}, {total} steps, Cd {cd}")
+        rows.append({"tier": tier, "lengthCells": length, "cd": cd})
+    band = grid_band([{"h": 1 / r["lengthCells"], "cd": math.nan if r["cd"] is None else r["cd"]} for r in rows])
+    return {"case": preset, **band, "re": re, "rows": rows}
 
 
 def convergence_rows(rows: Sequence[dict], operators: Sequence[str] = OPERATORS) -> List[dict]:

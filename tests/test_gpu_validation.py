@@ -88,3 +88,19 @@ def test_the_reference_s_unstable_trt_runs_fail_here_too(row):
     ref_step = int(regex.search(r"step (\d+)", row["reason"]).group(1))
     print(f"{spec.id}: {err.value} (reference: {row['reason']})")
     assert str(err.value).endswith(f"of {spec.steps}.") and abs(step / ref_step - 1) < 0.15
+
+
+@pytest.mark.parametrize("check", REF["gridCheck"], ids=[c["case"] for c in REF["gridCheck"]])
+def test_grid_checks_match_the_reference_s(check):
+    """The app's grid check from tier S at its defaults (floor on, REG + LES, 16-bit storage, the
+    interactive viscosity floor fixing Re 320 for the Ahmed body and 90 for the sphere): the same
+    body lengths, Cd per tier within 0.1 % of the stored ones (steady flows at these Re; measured
+    within 0.01 %) and the same band verdict."""
+    out = V.run_grid_check(check["case"])
+    print(f"{check['case']}: {out['kind']} {out.get('reason')}, "
+          + ", ".join(f"{r['tier']} {r['cd']:.6f} (reference {q['cd']:.6f})" for r, q in zip(out["rows"], check["rows"])))
+    assert out["re"] == pytest.approx(check["re"], rel=1e-14)
+    assert [r["lengthCells"] for r in out["rows"]] == [r["lengthCells"] for r in check["rows"]]
+    for r, q in zip(out["rows"], check["rows"]):
+        assert abs(r["cd"] / q["cd"] - 1) < 1e-3
+    assert (out["kind"], out.get("reason")) == (check["kind"], check.get("reason"))

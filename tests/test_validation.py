@@ -89,6 +89,23 @@ def test_grid_check_rows_of_the_reference_results():
         assert band["kind"] == check["kind"] and band.get("reason") == check.get("reason")
 
 
+def test_grid_check_lengths_steps_and_reynolds_number_of_the_reference_results():
+    """The stored grid checks (the Ahmed body and the sphere from tier S at the app's defaults):
+    the body lengths per tier and the Reynolds number fixed by the interactive viscosity floor."""
+    from pocketwind_amd import bodies
+    from pocketwind_amd.lattice import TIERS
+    s = TIERS["S"]
+    for check in REF["gridCheck"]:
+        mesh = bodies.preset_mesh(check["case"])
+        current = V.run_length_cells(check["case"], mesh, (s.nx, s.ny, s.nz))
+        lengths = [V.grid_check_length_cells(current, s.nx, r["tier"]) for r in check["rows"]]
+        assert lengths == [r["lengthCells"] for r in check["rows"]]
+        assert V.U_LU * current / V.NU_MIN_LU == pytest.approx(check["re"], rel=1e-14)
+    assert [V.grid_check_steps(t) for t in "SML"] == [(10240, 5120), (15360, 7680), (20480, 10240)]
+    assert V.mean_of_last([(8, 1.0), (16, 2.0), (24, 4.0)], 16) == 3.0
+    assert V.mean_of_last([(8, 1.0), (16, float("nan"))], 16) is None and V.mean_of_last([], 8) is None
+
+
 # ------------------------------------------------------------------------- cases
 
 def spec_for(row):
