@@ -15,6 +15,8 @@ case_scales, case_meshes, case_grid, run_case
                    src/validation/links.ts:14-35): Reynolds number and lattice viscosity, the
                    placed and voxelised preset with its floor and stilts, forces in newtons
                    every 8 steps
+run_body_drag      runBodyDrag (src/lbm/reference/cases.ts:264-321) as scripts/drag-cpu.ts runs
+                   it, on this solver: Cd per flow-through of the Ahmed body at 64 x 32 x 32
 run_grid_check     the app's grid check (src/app/grid-check.ts, case-run.ts:257-320) on this
                    solver: S, M, L at the live run's Re and body share, band from the means
 score_results      scripts/score-results.mjs: match rows to reference values by case
@@ -550,6 +552,68 @@ def run_laminar_sphere(spec: CaseSpec, steps: Optional[int] = None) -> SolverRun
     if spec.case not in ("sphere-re-100", "sphere-confinement"):
         raise ValueError("run_laminar_sphere runs the laminar sphere cases.")
     return run_case(spec, steps)
+
+
+# --------------------------------------------------------------------------------------------
+# Body drag per flow-through (scripts/drag-cpu.ts on this solver)
+
+DRAG_LAYOUT = (64, 32, 32)  # drag-cpu.ts:11
+DRAG_BODY_LENGTH_CELLS = 24  # drag-cpu.ts:12
+DRAG_FLOW_THROUGHS = 4  # drag-cpu.ts:13
+
+
+def drag_flow_throughs(samples, steps_per_flow_through: int, flow_throughs: int, cd_per_force: float) -> List[dict]:
+    """The per-flow-through binning of runBodyDrag (cases.ts:293-321): a sample at step k goes
+    to flow-through floor(k / steps_per_flow_through) (the last step's sample falls past the
+    end and is dropped); each bin reports its last step, sample count, mean Cd and relative
+    standard deviation (n - 1, in percent; None below two samples)."""
+    bins: List[List[float]] = [[] for _ in range(flow_throughs)]
+    last = [0] * flow_throughs
+    for step, fx in samples:
+        k = int(step) // steps_per_flow_through
+        if k < flow_throughs:
+            bins[k].append(fx * cd_per_force)
+            last[k] = int(step)
+    out = []
+    for k, values in enumerate(bins):
+        mean = _mean(values) if values else None
+        rel = None
+        if mean is not None and len(values) >= 2:
+            rel = 100 * math.sqrt(sum((v - mean) ** 2 for v in values) / (len(values) - 1)) / abs(mean)
+        out.append({"flowThrough": k + 1, "step": last[k], "samples": len(values), "meanCd": mean, "relStdPct": rel})
+    return out
+
+
+def run_body_drag(operator: str = "trt", layout=DRAG_LAYOUT, body_length_cells: float = DRAG_BODY_LENGTH_CELLS,
+                  flow_throughs: int = DRAG_FLOW_THROUGHS, les: bool = True, floor: bool = True,
+                  precision: str = "f32") -> dict:
+    """runBodyDrag (cases.ts:264-321) as scripts/drag-cpu.ts drives it (drag-cpu.ts:71-110), on
+    this solver: the Ahmed preset with its stilts as supports, placed at `body_length_cells` on
+    the floor, tau0 from the viscosity at 40 m/s (the interactive floor 1e-2 at this size), the
+    uniform inflow state, a force sample every 8 steps binned per flow-through of
+    ceil(nx / 0.05) steps, Cd on the voxel frontal cells."""
+    from . import bodies
+    from .gpu_solver import GpuSolver, SolverParams
+    from .voxelize import voxelize
+    nu = max(NU_MIN_LU, U_LU * body_length_cells / (DEFAULT_SPEED_MPS * (AHMED_BODY_LENGTH_MM / 1000) / NU_AIR))
+    tau0 = nu * 3 + 0.5
+    body, supports = bodies.place_mesh(bodies.ahmed_mesh(), bodies.stilts_mesh(), body_length_cells, layout, floor)
+    grid = voxelize(body, supports, layout, floor, link_fractions=False)
+    if grid.solid_cells == 0:
+        raise ValueError("The body does not cover any grid cells.")
+    per_flow = math.ceil(layout[0] / U_LU)
+    cd_per_force = 1 / (0.5 * U_LU * U_LU * grid.frontal_cells)
+    solver = GpuSolver.create(layout, grid, SolverParams(tau0, U_LU, les, operator=operator, precision=precision))
+    if not isinstance(solver, GpuSolver):
+        raise MemoryError(solver.message)
+    try:
+        samples = solver.step_sampled(flow_throughs * per_flow, FORCE_SAMPLE_STEPS)
+    finally:
+        solver.destroy()
+    finite = bool(np.isfinite(samples[:, 1:]).all())
+    return {"operator": operator, "tau0": tau0, "frontalCells": grid.frontal_cells,
+            "blockagePct": 100 * grid.frontal_cells / (layout[1] * layout[2]), "stepsPerFlowThrough": per_flow,
+            "finite": finite, "flowThroughs": drag_flow_throughs(samples[:, :2], per_flow, flow_throughs, cd_per_force)}
 
 
 # --------------------------------------------------------------------------------------------

@@ -104,3 +104,24 @@ def test_grid_checks_match_the_reference_s(check):
     for r, q in zip(out["rows"], check["rows"]):
         assert abs(r["cd"] / q["cd"] - 1) < 1e-3
     assert (out["kind"], out.get("reason")) == (check["kind"], check.get("reason"))
+
+
+BODY_DRAG = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "body_drag_oracle.json")))
+
+
+@pytest.mark.parametrize("operator", ["trt", "reg"])
+def test_body_drag_per_flow_through_against_the_oracle(operator):
+    """The case scripts/drag-cpu.ts runs (Ahmed body with stilts on the floor, 24 cells in
+    64 x 32 x 32, LES, tau0 0.53, four flow-throughs), on the device in f32 against the FP64
+    oracle (tests/golden/make_body_drag.py): the same frontal cells and bins; the first
+    flow-through's mean Cd (the start-up transient, relative spread 60 %) within 0.1 % and the
+    settled ones within 1e-4 (measured: 2.4e-4 and 1.3e-4 for the first, at most 7e-6 after)."""
+    out = V.run_body_drag(operator)
+    ref = BODY_DRAG["operators"][operator]
+    print(f"{operator}: " + ", ".join(f"{b['meanCd']:.6f} (oracle {r['meanCd']:.6f})"
+                                      for b, r in zip(out["flowThroughs"], ref["flowThroughs"])))
+    assert out["frontalCells"] == BODY_DRAG["frontal_cells"] and out["finite"] and ref["finite"]
+    assert out["tau0"] == pytest.approx(BODY_DRAG["tau0"], rel=1e-15)
+    for k, (b, r) in enumerate(zip(out["flowThroughs"], ref["flowThroughs"])):
+        assert (b["step"], b["samples"]) == (r["step"], r["samples"])
+        assert abs(b["meanCd"] / r["meanCd"] - 1) < (1e-3 if k == 0 else 1e-4)

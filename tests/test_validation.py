@@ -282,3 +282,19 @@ def test_scores_the_reference_results_file_and_the_cli(tmp_path):
                                            (-0.0, 2, "0.00"), (-0.001, 2, "-0.00"), (5.2631578947, 2, "5.26")])
 def test_to_fixed_follows_number_to_fixed(x, digits, text):
     assert V.to_fixed(x, digits) == text
+
+
+def test_body_drag_binning_follows_the_flow_through_rule():
+    """runBodyDrag's bins (cases.ts:293-321) over a sample every 8 steps of four flow-throughs
+    of 1280 steps: 159, 160, 160, 160 samples ending at steps 1272, 2552, 3832, 5112 (the step
+    5120 sample falls past the end), as the oracle fixture's bins record."""
+    G = json.load(open(os.path.join(GOLDEN, "body_drag_oracle.json")))
+    samples = [(s, 2.0 if s < 1280 else (1.0 if s % 16 else 3.0)) for s in range(8, 5121, 8)]
+    bins = V.drag_flow_throughs(samples, 1280, 4, 0.5)
+    oracle_bins = G["operators"]["trt"]["flowThroughs"]
+    assert [(b["step"], b["samples"]) for b in bins] == [(b["step"], b["samples"]) for b in oracle_bins]
+    assert bins[0]["meanCd"] == 1.0 and bins[0]["relStdPct"] == 0.0
+    # 80 samples of 0.5 and 80 of 1.5 about the mean 1: std = sqrt(160 * 0.25 / 159).
+    assert bins[1]["meanCd"] == 1.0 and bins[1]["relStdPct"] == pytest.approx(100 * math.sqrt(40 / 159), rel=1e-12)
+    assert V.drag_flow_throughs([(8, 1.0)], 1280, 2, 1.0)[1] == {"flowThrough": 2, "step": 0, "samples": 0,
+                                                                 "meanCd": None, "relStdPct": None}
