@@ -172,6 +172,35 @@ def voxelize_timing(layout=LAYOUT, length=BODY_LENGTH, repeats=5):
                              "sample": "FP64 C oracle (restated voxelize.ts and wall-links.ts), the same mesh"}}
 
 
+def drag_cpu_case(oracle_steps: int = 480):
+    """The case scripts/drag-cpu.ts runs on the reference's CPU solver (the Ahmed body with its
+    stilts on the floor, 24 cells in 64 x 32 x 32, LES, TRT, four flow-throughs of 1280 steps,
+    drag-cpu.ts:11-15): the device's whole run (voxelisation and set-up included) with its Cd
+    per flow-through, beside the oracle stepping the same grid on one thread."""
+    import oracle
+    from pocketwind_amd import bodies
+    from pocketwind_amd import validation as V
+    t0 = time.perf_counter()
+    out = V.run_body_drag("trt")
+    device_s = time.perf_counter() - t0
+    layout = V.DRAG_LAYOUT
+    cells = layout[0] * layout[1] * layout[2]
+    steps = V.DRAG_FLOW_THROUGHS * out["stepsPerFlowThrough"]
+    body, sup = bodies.place_mesh(bodies.ahmed_mesh(), bodies.stilts_mesh(), V.DRAG_BODY_LENGTH_CELLS, layout, True)
+    g = oracle.voxelize(body, sup, layout, True, link_fractions=False)
+    cpu = oracle.Solver(layout, out["tau0"], oracle.TRT, True, flags=g.geometry[:cells],
+                        equilibrium_state=(1.0, (U_IN, 0.0, 0.0)), threads=1)
+    t0 = time.perf_counter()
+    cpu.step(oracle_steps)
+    cpu_s = time.perf_counter() - t0
+    return {"layout": list(layout), "operator": "trt", "steps": steps, "device_s_with_setup": round(device_s, 3),
+            "device_mlups": round(cells * steps / device_s / 1e6, 1),
+            "cd_per_flow_through": [round(b["meanCd"], 6) for b in out["flowThroughs"]],
+            "cpu_baseline": {"value": round(cells * oracle_steps / cpu_s / 1e6, 3), "unit": "MLUPS", "cores": 1,
+                             "kind": "port", "sample": f"FP64 C oracle (restated CpuSolver), the same grid and "
+                                                       f"operator, {oracle_steps} steps on one thread"}}
+
+
 def _oracle_mlups(grid, threads: int, seconds_target: float):
     import oracle
     cpu = oracle.Solver(grid.layout, TAU0, oracle.REG, True, flags=grid.flags,
@@ -307,7 +336,9 @@ def main():
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(grid)
         line["voxelize"] = voxelize_timing()
+        line["drag_cpu_case"] = drag_cpu_case()
         print(f"# voxelize: {line['voxelize']}", file=sys.stderr, flush=True)
+        print(f"# drag-cpu case: {line['drag_cpu_case']}", file=sys.stderr, flush=True)
     print(json.dumps(line))
     return 0
 
