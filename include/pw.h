@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-#define PW_ABI_VERSION 5
+#define PW_ABI_VERSION 6
 
 /* Errors (src/lbm/gpu-resources.ts:3-27, gpu-force.ts:192-205). EINVAL is the reference's
  * RangeError, ENOMEM its AllocationFailure, EBUSY its 'busy' readback. */
@@ -26,7 +26,7 @@ typedef enum {
   PW_ENOMEM = 2,
   PW_EBUSY = 3,
   PW_EDEVICE = 4,
-  PW_ESTATE = 5 /* handle destroyed or used out of order */
+  PW_ESTATE = 5 /* a solver handle that was destroyed (or never created) */
 } pw_status;
 
 /* Cell kinds in bits 0-2 of a geometry flag; bit 3+i marks that incoming direction i pulls
@@ -91,6 +91,17 @@ int pw_abi_version(void);
 /* Number of visible HIP devices (0 without a GPU; no kernel is launched). */
 int pw_device_count(void);
 
+/* packSolverParams (src/lbm/gpu-solver.ts:80-116): the 48-byte f32 uniform of struct Params
+ * (src/lbm/shaders/common.wgsl:31-46) as the reference packs it: u32 nx, ny, nz, cells; f32
+ * u_in, nu = fround((fround(tau0) - 1/2) / 3), 18 sqrt(2) Cs^2, lambda = 3/16; u32 les,
+ * walls == Bouzidi, write_macro, step. PW_EINVAL (RangeError) for the same inputs the
+ * reference rejects: empty grids, indices outside u32, tau0 not above 1/2 after f32 rounding,
+ * non-finite f32 speeds. The step kernels take these values (and constants derived from them)
+ * as launch arguments. Host only: no device call. */
+#define PW_PARAMS_BYTES 48
+pw_status pw_pack_params(const pw_layout *layout, const pw_params *params, int write_macro,
+                         uint64_t step, void *out, size_t out_bytes);
+
 /* GpuSolver.create (src/lbm/gpu-solver.ts:158-335): validates the packed geometry
  * [flags N][q offsets N][f32 q per link] (src/lbm/geometry-contract.ts:39-66) and the
  * boundary list (src/lbm/gpu-force.ts:76-81), copies both to the device, allocates two
@@ -119,7 +130,9 @@ pw_status pw_solver_reset(pw_solver *s);
 /* GpuSolver.stepCount (src/lbm/gpu-solver.ts:342-344). */
 uint64_t pw_solver_step_count(const pw_solver *s);
 
-/* GpuSolver.destroy (src/lbm/gpu-solver.ts:459-464). Waits for queued work. */
+/* GpuSolver.destroy (src/lbm/gpu-solver.ts:459-464). Waits for queued work. The library keeps a
+ * set of live solver handles: after destroy, calls with the handle return PW_ESTATE (getters
+ * return 0) without dereferencing it, until a later create happens to reuse the address. */
 void pw_solver_destroy(pw_solver *s);
 
 /* macroBuffer readback (docs/numerics.md "Macro buffer"): 4*N floats (rho, ux, uy, uz). */
@@ -184,6 +197,10 @@ typedef enum {
 } pw_kernel;
 pw_status pw_solver_set_kernel(pw_solver *s, uint32_t kernel);
 uint32_t pw_solver_kernel(const pw_solver *s);
+/* An explicit step variant: kernel PW_KERNEL_CELL .. PW_KERNEL_CELL_NT_STORES, `block` threads
+ * per block (64, 128 or 256) and `xcd_chunks` (1 or 16) chunks per XCD, without timing. */
+pw_status pw_solver_set_step_variant(pw_solver *s, uint32_t kernel, uint32_t block,
+                                     uint32_t xcd_chunks);
 /* Threads per block of the step launches. */
 uint32_t pw_solver_block_size(const pw_solver *s);
 /* Chunks of the lattice each of the 8 XCDs steps (1: one contiguous eighth each; 16: chunks
@@ -224,7 +241,7 @@ typedef struct {
   uint64_t missed_links;
 } pw_voxel_grid;
 
-/* voxelize(body, supports, layout, {floor, linkFractions}) (src/geometry/voxelize.ts:187-214,
+/* voxelize(body, supports, layout, {floor, linkFractions}) (src/geometry/voxelize.ts:172-199,
  * wall-links.ts:99-131) on the GPU: bit-exact flags, q offsets, link fractions and boundary
  * list. body/supports are 9 floats per triangle in lattice cells; supports may be NULL. */
 pw_status pw_voxelize(const float *body, size_t body_tris, const float *supports,
@@ -233,7 +250,7 @@ pw_status pw_voxelize(const float *body, size_t body_tris, const float *supports
 void pw_voxel_grid_free(pw_voxel_grid *grid);
 
 /* Streakline particles bound to a solver's macro field and flags (createParticles,
- * src/render/particles.ts:30-138; seeds src/render/particle-layout.ts:126-172). A record is
+ * src/render/particles.ts:30-138; seeds src/render/particle-layout.ts:66-112). A record is
  * 8 floats: pos.xyz, age, prev.xyz, generation. count may be 1 .. 2^24 (the reference caps
  * it at 65536). Updates run on the solver's stream after its queued steps. Destroy the
  * particles before their solver. */
@@ -254,6 +271,18 @@ void pw_particles_destroy(pw_particles *p);
 pw_status pw_particles_create_batch(pw_batch *b, uint32_t tunnel, uint32_t count, uint32_t seed,
                                     const double body_min[3], const double body_max[3],
                                     pw_particles **out);
+/* createParticles(device, format, frame, {grid, bodyBounds, macroBuffer, count, seed})
+ * (src/render/particles.ts:30-138) over a caller's field: `geometry` is the grid's host flag
+ * array (at least nx*ny*nz words, copied to the device), `macro` a device buffer of 4 floats
+ * (rho, ux, uy, uz) per cell that the particles read on every update. Updates run on a stream
+ * of the particles' own: finish writing `macro` (for example hipDeviceSynchronize) before an
+ * update, and call pw_particles_synchronize before changing it again. */
+pw_status pw_particles_create_field(const pw_layout *layout, const uint32_t *geometry,
+                                    size_t geometry_words, const float *macro, uint32_t count,
+                                    uint32_t seed, const double body_min[3],
+                                    const double body_max[3], pw_particles **out);
+/* Waits for the particles' queued updates (the reference's queue submission completing). */
+pw_status pw_particles_synchronize(pw_particles *p);
 
 /* Batched tunnels: `count` independent tunnels of any sizes advanced by one launch per step,
  * even/odd step pairs replayed from a captured hipGraph. No reference counterpart (the

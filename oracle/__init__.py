@@ -8,11 +8,11 @@ This package is the checker. Only ``tests/``, ``__graft_entry__.smoke()`` and th
 Reference correspondence (file:line, relative to the pocketwind repository):
   Solver            src/lbm/reference/cpu-solver.ts:69-323
   equilibrium etc.  src/lbm/reference/collision.ts:21-224
-  voxelize          src/geometry/voxelize.ts:187-214, wall-links.ts:99-131
+  voxelize          src/geometry/voxelize.ts:172-199, wall-links.ts:99-131
   nearest_link_hit  src/geometry/triangle-bins.ts:136-162 (via nearestLinkHit)
   preset_mesh       src/geometry/presets.ts:179-184
   place_mesh        src/geometry/transform.ts:118-143
-  particle_*        src/render/particle-layout.ts:66-172, shaders/particles_update.wgsl
+  particle_*        src/render/particle-layout.ts:14-112, shaders/particles_update.wgsl
   encode_f16        src/lbm/shaders/prelude-f16.wgsl:7-22
 """
 from __future__ import annotations
@@ -119,6 +119,9 @@ def _declare(L):
     L.pwo_place_mesh.restype = C.c_int
     L.pwo_particle_hash.argtypes = [C.c_uint32]
     L.pwo_particle_hash.restype = C.c_uint32
+    L.pwo_particle_needs_reseed.argtypes = [_d, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.c_uint32]
+    L.pwo_particle_needs_reseed.restype = C.c_int
     L.pwo_seed_region.argtypes = [_d, _d, C.c_uint32, C.c_uint32, C.c_uint32, _d]
     L.pwo_seed_region.restype = C.c_int
     L.pwo_particle_seeds.argtypes = [_u32, C.c_uint32, C.c_uint32, C.c_uint32, _d, _d,
@@ -349,6 +352,11 @@ def place_mesh(mesh, supports, length_cells, layout, floor):
 # ---------------------------------------------------------------- particles
 def particle_hash(v: int) -> int:
     return lib().pwo_particle_hash(v & 0xFFFFFFFF)
+
+
+def particle_needs_reseed(position, age, layout, flag) -> bool:
+    p = np.asarray(position, np.float64)
+    return bool(lib().pwo_particle_needs_reseed(_ptr(p, _d), float(age), *layout, int(flag) & 0xFFFFFFFF))
 
 
 def seed_region(bmin, bmax, layout):

@@ -2,7 +2,7 @@
  * particles.c -- CPU oracle for streakline particles.
  *
  * TEST INFRASTRUCTURE ONLY (the checker). Restates:
- *   src/render/particle-layout.ts:74-172      seed region, reseed rule, seeds, lowbias32 hash
+ *   src/render/particle-layout.ts:14-112      seed region, reseed rule, lowbias32 hash, seeds
  *   src/render/shaders/particles_update.wgsl  trilinear advection step, in f32 arithmetic with
  *                                             WGSL's mix(a, b, t) = a (1 - t) + b t
  */
@@ -13,7 +13,7 @@
 
 #include "pw_oracle.h"
 
-/* particle-layout.ts:118-123 */
+/* particle-layout.ts:58-63 */
 uint32_t pwo_particle_hash(uint32_t value) {
   uint32_t r = value;
   r = (r ^ (r >> 16)) * 0x7feb352du;
@@ -21,7 +21,7 @@ uint32_t pwo_particle_hash(uint32_t value) {
   return r ^ (r >> 16);
 }
 
-/* particle-layout.ts:74-96 */
+/* particle-layout.ts:14-36 */
 int pwo_seed_region(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny,
                     uint32_t nz, double region[4]) {
   const double ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
@@ -37,7 +37,7 @@ int pwo_seed_region(const double bmin[3], const double bmax[3], uint32_t nx, uin
   return 0;
 }
 
-/* particle-layout.ts:99-116 */
+/* particle-layout.ts:39-56 */
 static int needs_reseed(const double p[3], double age, uint32_t nx, uint32_t ny, uint32_t nz,
                         uint32_t flag) {
   return !isfinite(p[0]) || !isfinite(p[1]) || !isfinite(p[2]) || p[0] < 0 ||
@@ -45,10 +45,15 @@ static int needs_reseed(const double p[3], double age, uint32_t nx, uint32_t ny,
          p[2] >= (double)nz - 1 || age > 4000 || (flag & PWO_TYPE_MASK) != PWO_FLUID;
 }
 
+int pwo_particle_needs_reseed(const double p[3], double age, uint32_t nx, uint32_t ny, uint32_t nz,
+                              uint32_t flag) {
+  return needs_reseed(p, age, nx, ny, nz, flag);
+}
+
 /* JS Math.round: round half up. */
 static double js_round(double v) { return floor(v + 0.5); }
 
-/* particle-layout.ts:126-172 (count limit left to the caller). */
+/* particle-layout.ts:66-112 (count limit left to the caller). */
 int pwo_particle_seeds(const uint32_t *flags, uint32_t nx, uint32_t ny, uint32_t nz,
                        const double bmin[3], const double bmax[3], uint32_t count, uint32_t seed,
                        float *data, float fallback[3], double region[4], char *err,

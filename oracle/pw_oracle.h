@@ -112,6 +112,8 @@ int pwo_place_mesh(const float *mesh, size_t tris, const float *supports, size_t
 
 /* ---- particle-layout.ts / particles_update.wgsl ---- */
 uint32_t pwo_particle_hash(uint32_t value);
+int pwo_particle_needs_reseed(const double p[3], double age, uint32_t nx, uint32_t ny, uint32_t nz,
+                              uint32_t flag);
 int pwo_seed_region(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny,
                     uint32_t nz, double region[4]);
 int pwo_particle_seeds(const uint32_t *flags, uint32_t nx, uint32_t ny, uint32_t nz,

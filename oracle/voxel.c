@@ -2,10 +2,10 @@
  * voxel.c -- CPU oracle for the triangle-mesh voxelizer, wall links and preset bodies.
  *
  * TEST INFRASTRUCTURE ONLY (the checker). Restates, in IEEE double with no contraction:
- *   src/geometry/voxelize.ts:23-214      three-projection parity vote, EQ/floor planes, counts
+ *   src/geometry/voxelize.ts:23-199      three-projection parity vote, EQ/floor planes, counts
  *   src/geometry/wall-links.ts:36-131    pull wall masks, q offsets, boundary cells, q fractions
  *   src/geometry/triangle-bins.ts:14-162 4-cell triangle bins and Moller-Trumbore link hits
- *   src/geometry/presets.ts:146-327      sphere, Ahmed body, stilts and cube meshes
+ *   src/geometry/presets.ts:4-189        sphere, Ahmed body, stilts and cube meshes
  *   src/geometry/transform.ts:118-143    placement in lattice cells (flow +x, up z)
  */
 #include <float.h>
@@ -185,7 +185,7 @@ static void free_bins(bins_t *b) {
   free(b->items);
 }
 
-/* triangle-bins.ts:191-225: Moller-Trumbore; t is the fraction of the unnormalised link. */
+/* triangle-bins.ts:60-94: Moller-Trumbore; t is the fraction of the unnormalised link. */
 static int intersect(const float *p, const double o[3], const double d[3], double *t_out) {
   const double ax = p[0], ay = p[1], az = p[2];
   const double e1x = (double)p[3] - ax, e1y = (double)p[4] - ay, e1z = (double)p[5] - az;
@@ -209,7 +209,7 @@ static int intersect(const float *p, const double o[3], const double d[3], doubl
   return 1;
 }
 
-/* triangle-bins.ts:259-292 */
+/* triangle-bins.ts:98-162 */
 static int nearest_hit(const bins_t *b, const double o[3], const double d[3], double *t_out) {
   double lo[3], hi[3];
   long bmin[3], bmax[3];
@@ -343,7 +343,7 @@ static int valid_mesh(const float *p, size_t tris) {
   return 1;
 }
 
-/* voxelize.ts:187-214 */
+/* voxelize.ts:172-199 */
 int pwo_voxelize(const float *body, size_t body_tris, const float *supports, size_t support_tris,
                  uint32_t nx, uint32_t ny, uint32_t nz, int floor, int link_fractions,
                  pwo_voxel_grid *out, char *err, size_t err_len) {
@@ -441,7 +441,7 @@ static void corners(const station *st, double c[4][3]) {
   memcpy(c, v, sizeof v);
 }
 
-/* presets.ts:206-228 */
+/* presets.ts:63-90 */
 static void loft(tri_sink *s, const station *sts, int n) {
   double near[4][3], far[4][3];
   for (int k = 0; k + 1 < n; ++k) {
@@ -459,7 +459,7 @@ static void loft(tri_sink *s, const station *sts, int n) {
   quad(s, last[0], last[3], last[2], last[1]);
 }
 
-/* presets.ts:235-258 (dimensions in millimetres, presets.ts:147-160) */
+/* presets.ts:92-115 (dimensions in millimetres, presets.ts:4-17) */
 static void ahmed(tri_sink *s) {
   const double radius = 100, half_width = 389.0 / 2, bottom = 50, top = 50 + 288;
   station sts[16];
@@ -477,7 +477,7 @@ static void ahmed(tri_sink *s) {
   loft(s, sts, n);
 }
 
-/* presets.ts:261-284 */
+/* presets.ts:118-133 */
 static void cylinder(tri_sink *s, double x, double y, double diameter, double height) {
   const double r = diameter / 2;
   for (int seg = 0; seg < 16; ++seg) {
@@ -502,7 +502,7 @@ static void stilts(tri_sink *s) {
   }
 }
 
-/* presets.ts:295-319 */
+/* presets.ts:152-177 */
 static void sphere(tri_sink *s) {
   const double radius = 50;
   for (int ring = 0; ring < 16; ++ring)

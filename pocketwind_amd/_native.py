@@ -52,6 +52,8 @@ SIGNATURES = {
     "pw_device_count": (C.c_int, []),
     "pw_solver_create": (C.c_int, [C.POINTER(Layout), _u32p, C.c_size_t, _u32p, C.c_size_t,
                                    C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "pw_pack_params": (C.c_int, [C.POINTER(Layout), C.POINTER(Params), C.c_int, C.c_uint64, C.c_void_p,
+                                 C.c_size_t]),
     "pw_solver_step": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int]),
     "pw_solver_encode_force": (C.c_int, [C.c_void_p]),
     "pw_solver_read_force": (C.c_int, [C.c_void_p, C.POINTER(ForceSample)]),
@@ -69,6 +71,7 @@ SIGNATURES = {
     "pw_solver_field_vtk_blocks": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_size_t, C.c_void_p,
                                              C.c_size_t]),
     "pw_solver_block_size": (C.c_uint32, [C.c_void_p]),
+    "pw_solver_set_step_variant": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     "pw_solver_xcd_chunks": (C.c_uint32, [C.c_void_p]),
     "pw_solver_kernel": (C.c_uint32, [C.c_void_p]),
     "pw_solver_synchronize": (C.c_int, [C.c_void_p]),
@@ -88,6 +91,10 @@ SIGNATURES = {
     "pw_particles_create_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                             C.POINTER(C.c_double), C.POINTER(C.c_double),
                                             C.POINTER(C.c_void_p)]),
+    "pw_particles_create_field": (C.c_int, [C.POINTER(Layout), _u32p, C.c_size_t, C.c_void_p, C.c_uint32,
+                                            C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                            C.POINTER(C.c_void_p)]),
+    "pw_particles_synchronize": (C.c_int, [C.c_void_p]),
     "pw_batch_create": (C.c_int, [C.c_uint32, C.POINTER(Layout), C.POINTER(C.c_void_p),
                                   C.POINTER(C.c_size_t), C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(Params),

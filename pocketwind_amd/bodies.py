@@ -1,6 +1,6 @@
 """Triangle meshes of the tunnel bodies and their placement in lattice cells.
 
-Presets restate src/geometry/presets.ts:146-327 (sphere 24 x 16 lat-long, Ahmed 25-degree body
+Presets restate src/geometry/presets.ts:4-189 (sphere 24 x 16 lat-long, Ahmed 25-degree body
 lofted from rectangular stations, its four stilts, cube); placement restates
 src/geometry/transform.ts:118-143 for flow along +x with z up. Coordinates are evaluated in
 float64 and stored as float32, as the reference's Float32Array meshes are. ``bluff_body``
@@ -19,7 +19,7 @@ import numpy as np
 
 def _loft(stations):
     """Join rectangular stations (x, y_half, z_min, z_max) into a closed solid
-    (presets.ts:191-228)."""
+    (presets.ts:48-90)."""
     def corners(s):
         x, yh, z0, z1 = s
         return [(x, -yh, z0), (x, yh, z0), (x, yh, z1), (x, -yh, z1)]
@@ -51,7 +51,7 @@ def _rounded_nose(radius, half_width, bottom, top, segments=12):
 
 
 def ahmed_mesh() -> np.ndarray:
-    """presets.ts:235-258, millimetres."""
+    """presets.ts:92-115, millimetres."""
     nose = _rounded_nose(100, 389 / 2, 50, 50 + 288)
     slant = (25 * math.pi) / 180
     start = 1044 - 222 * math.cos(slant)
@@ -60,7 +60,7 @@ def ahmed_mesh() -> np.ndarray:
 
 
 def stilts_mesh() -> np.ndarray:
-    """presets.ts:261-284: four 30 mm stilts, 16-sided prisms from z = 0 to 50."""
+    """presets.ts:118-141: four 30 mm stilts, 16-sided prisms from z = 0 to 50."""
     tris = []
     for x in (202, 672):
         for y in (-163.5, 163.5):
@@ -84,7 +84,7 @@ def cube_mesh() -> np.ndarray:
 
 
 def sphere_mesh() -> np.ndarray:
-    """presets.ts:295-319: radius 50, 24 segments, 16 rings; the poles have one vertex."""
+    """presets.ts:152-177: radius 50, 24 segments, 16 rings; the poles have one vertex."""
     def vertex(ring, seg):
         phi = (ring / 16) * math.pi
         theta = (seg / 24) * 2 * math.pi
@@ -108,7 +108,7 @@ PRESETS = {"sphere": sphere_mesh, "ahmed-25": ahmed_mesh, "cube": cube_mesh}
 
 
 def preset_mesh(name: str):
-    """presetMesh (presets.ts:322-327); None for an unknown preset."""
+    """presetMesh (presets.ts:179-185); None for an unknown preset."""
     maker = PRESETS.get(name)
     return None if maker is None else maker()
 

@@ -48,7 +48,7 @@ void launch_wall_forces_s(pw_solver* s, double* out) {
 }
 
 pw_status check(pw_solver* s) {
-  if (!s || s->destroyed) {
+  if (!solver_live(s)) {
     set_error("Null or destroyed solver handle.");
     return PW_EINVAL;
   }
@@ -95,7 +95,7 @@ using namespace pw;
 
 extern "C" {
 
-size_t pw_solver_wall_cell_count(const pw_solver* s) { return (s && !s->destroyed) ? s->n_wall : 0; }
+size_t pw_solver_wall_cell_count(const pw_solver* s) { return solver_live(s) ? s->n_wall : 0; }
 
 pw_status pw_solver_wall_cells(pw_solver* s, uint32_t* out, size_t n) {
   if (pw_status st = check(s)) return st;

@@ -37,7 +37,7 @@ extern "C" {
 
 pw_status pw_solver_field_vtk_blocks(pw_solver* s, double velocity_scale, void* rho_be, size_t rho_bytes,
                                      void* velocity_be, size_t velocity_bytes) {
-  if (!s || s->destroyed) {
+  if (!solver_live(s)) {
     set_error("Null or destroyed solver handle.");
     return PW_EINVAL;
   }

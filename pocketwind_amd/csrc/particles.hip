@@ -3,7 +3,7 @@
 // Follows src/render/shaders/particles_update.wgsl:1-60 (one lane per particle: trilinear
 // velocity of the macro field times the solver steps, displacement capped at one cell,
 // respawn at x = 2 through the lowbias32 hash when the particle leaves the fluid, reaches the
-// outlet or ages past 4000 frames) and src/render/particle-layout.ts:74-172 (seed region,
+// outlet or ages past 4000 frames) and src/render/particle-layout.ts:14-112 (seed region,
 // inlet candidates, initial records). Particles read the solver's macro buffer and flags on
 // the solver's own stream, so an update always sees the macros of the last step batch.
 // Arithmetic is f32 with WGSL's mix(a, b, t) = a (1 - t) + b t and no contraction.
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void update_kernel(float4* __restrict__ partic
   }
 }
 
-// particle-layout.ts:99-116
+// particle-layout.ts:39-56
 bool needs_reseed(double x, double y, double z, double age, const pw_layout& L, uint32_t flag) {
   return !std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z) || x < 0 || x >= (double)L.nx - 2 ||
          y < 0 || y >= (double)L.ny - 1 || z < 0 || z >= (double)L.nz - 1 || age > kMaxAge ||
@@ -142,21 +142,24 @@ struct pw_particles {
   float region[4] = {0, 0, 0, 0};
   float fallback[3] = {0, 0, 0};
   float4* data = nullptr;
+  uint32_t* owned_flags = nullptr;  // the device copy of a caller's flags (field-bound particles)
+  bool owns_stream = false;         // the stream was created for these particles
 };
 
 extern "C" {
 
 namespace {
 
-// particle-layout.ts:74-96 and :126-172 with the count limit of this library, for the field f
+// particle-layout.ts:14-36 and :66-112 with the count limit of this library, for the field f
 // written on `stream`.
-pw_status create_particles(const Field& f, hipStream_t stream, uint32_t count, uint32_t seed,
-                           const double body_min[3], const double body_max[3], pw_particles** out) {
-  const pw_layout L{f.nx, f.ny, f.nz};
+// particle-layout.ts:14-36 (seed region) and the count limit; host checks only.
+pw_status seed_arguments(const pw_layout& L, uint32_t count, const double body_min[3], const double body_max[3],
+                         double region[4]) {
   const double ey = body_max[1] - body_min[1], ez = body_max[2] - body_min[2];
-  double region[4] = {std::fmax(1, body_min[1] - ey * 0.15), std::fmax(1, body_min[2] - ez * 0.15),
-                      std::fmin((double)L.ny - 2, body_max[1] + ey * 0.15),
-                      std::fmin((double)L.nz - 2, body_max[2] + ez * 0.15)};
+  region[0] = std::fmax(1, body_min[1] - ey * 0.15);
+  region[1] = std::fmax(1, body_min[2] - ez * 0.15);
+  region[2] = std::fmin((double)L.ny - 2, body_max[1] + ey * 0.15);
+  region[3] = std::fmin((double)L.nz - 2, body_max[2] + ez * 0.15);
   bool finite = true;
   for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(body_min[k]) && std::isfinite(body_max[k]);
   if (L.nx < 5 || L.ny < 5 || L.nz < 5 || !finite || !(ey > 0) || !(ez > 0) || region[0] >= region[2] ||
@@ -168,6 +171,14 @@ pw_status create_particles(const Field& f, hipStream_t stream, uint32_t count, u
     set_error("Particle count, seed or geometry buffer is invalid.");
     return PW_EINVAL;
   }
+  return PW_OK;
+}
+
+pw_status create_particles(const Field& f, hipStream_t stream, uint32_t count, uint32_t seed,
+                           const double body_min[3], const double body_max[3], pw_particles** out) {
+  const pw_layout L{f.nx, f.ny, f.nz};
+  double region[4];
+  if (pw_status st = seed_arguments(L, count, body_min, body_max, region)) return st;
   // Inlet candidates need the cell flags: read back the plane x = 2 rows once.
   std::vector<uint32_t> flags((size_t)L.nx * L.ny * L.nz);
   hipError_t e = hipMemcpyAsync(flags.data(), f.flags, flags.size() * 4, hipMemcpyDeviceToHost, stream);
@@ -225,7 +236,7 @@ pw_status create_particles(const Field& f, hipStream_t stream, uint32_t count, u
 
 pw_status pw_particles_create(pw_solver* s, uint32_t count, uint32_t seed, const double body_min[3],
                               const double body_max[3], pw_particles** out) {
-  if (!s || s->destroyed || !out || !body_min || !body_max) {
+  if (!solver_live(s) || !out || !body_min || !body_max) {
     set_error("Null argument.");
     return PW_EINVAL;
   }
@@ -254,9 +265,61 @@ pw_status pw_particles_create_batch(pw_batch* b, uint32_t tunnel, uint32_t count
   return create_particles(Field{macro, geo, L.nx, L.ny, L.nz}, stream, count, seed, body_min, body_max, out);
 }
 
+// createParticles(device, format, frame, {grid, bodyBounds, macroBuffer, count, seed})
+// (src/render/particles.ts:30-138): the caller's macro buffer on the device and the grid's
+// host flags, which are copied to the device. The particles step on a stream of their own.
+pw_status pw_particles_create_field(const pw_layout* layout, const uint32_t* geometry, size_t geometry_words,
+                                    const float* macro, uint32_t count, uint32_t seed, const double body_min[3],
+                                    const double body_max[3], pw_particles** out) {
+  if (!layout || !geometry || !macro || !out || !body_min || !body_max) {
+    set_error("Null argument.");
+    return PW_EINVAL;
+  }
+  *out = nullptr;
+  const size_t N = (size_t)layout->nx * layout->ny * layout->nz;
+  if (geometry_words < N) {  // particle-layout.ts:80
+    set_error("Particle count, seed or geometry buffer is invalid.");
+    return PW_EINVAL;
+  }
+  double region[4];
+  if (pw_status st = seed_arguments(*layout, count, body_min, body_max, region)) return st;
+  hipStream_t stream = nullptr;
+  uint32_t* flags = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return device_error(e, "particle stream");
+  if ((e = hipMalloc((void**)&flags, N * 4)) != hipSuccess) {
+    (void)hipStreamDestroy(stream);
+    return device_error(e, "particle flag allocation");
+  }
+  e = hipMemcpyAsync(flags, geometry, N * 4, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  pw_status st = e == hipSuccess ? PW_OK : device_error(e, "particle flag upload");
+  if (st == PW_OK)
+    st = create_particles(Field{(const float4*)macro, flags, layout->nx, layout->ny, layout->nz}, stream, count,
+                          seed, body_min, body_max, out);
+  if (st != PW_OK) {
+    (void)hipFree(flags);
+    (void)hipStreamDestroy(stream);
+    return st;
+  }
+  (*out)->owned_flags = flags;
+  (*out)->owns_stream = true;
+  return PW_OK;
+}
+
+// Waits for the particles' queued updates.
+pw_status pw_particles_synchronize(pw_particles* h) {
+  if (!h) {
+    set_error("Null particle handle.");
+    return PW_EINVAL;
+  }
+  hipError_t e = hipStreamSynchronize(h->stream);
+  return e == hipSuccess ? PW_OK : device_error(e, "particle synchronize");
+}
+
 // particles.update(encoder, steps) (src/render/particles.ts:113-124).
 pw_status pw_particles_update(pw_particles* h, float steps) {
-  if (!h || (h->solver && h->solver->destroyed)) {
+  if (!h || (h->solver && !solver_live(h->solver))) {
     set_error("Invalid particle handle.");
     return PW_EINVAL;
   }
@@ -303,7 +366,10 @@ pw_status pw_particles_info(const pw_particles* h, float region[4], float fallba
 
 void pw_particles_destroy(pw_particles* h) {
   if (!h) return;
+  if (h->owns_stream) (void)hipStreamSynchronize(h->stream);
   (void)hipFree(h->data);  // hipFree waits for work still using the buffer
+  if (h->owned_flags) (void)hipFree(h->owned_flags);
+  if (h->owns_stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 

@@ -9,8 +9,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "force_kernels.h"
@@ -19,6 +21,17 @@
 namespace pw {
 
 static thread_local std::string g_error;
+
+// Handles returned by pw_solver_create and not yet destroyed. Entry points check membership
+// rather than a flag inside the handle, so a destroyed handle is never dereferenced.
+static std::mutex g_live_mutex;
+static std::unordered_set<const pw_solver*> g_live;
+
+bool solver_live(const pw_solver* s) {
+  if (!s) return false;
+  std::lock_guard<std::mutex> lock(g_live_mutex);
+  return g_live.count(s) != 0;
+}
 
 void set_error(const std::string& message) { g_error = message; }
 
@@ -350,8 +363,8 @@ pw_status check_handle(pw_solver* s) {
     set_error("Null solver handle.");
     return PW_EINVAL;
   }
-  if (s->destroyed) {
-    set_error("The solver has been destroyed.");
+  if (!solver_live(s)) {
+    set_error("The solver has been destroyed (or was never created).");
     return PW_ESTATE;
   }
   return PW_OK;
@@ -426,6 +439,30 @@ int pw_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
+}
+
+pw_status pw_pack_params(const pw_layout* layout, const pw_params* params, int write_macro, uint64_t step, void* out,
+                         size_t out_bytes) {
+  if (!layout || !params || !out || out_bytes < PW_PARAMS_BYTES) {
+    set_error("Null argument or an output smaller than the 48-byte uniform.");
+    return PW_EINVAL;
+  }
+  if (layout->nx < 1 || layout->ny < 1 || layout->nz < 1) {
+    set_error("Grid dimensions must be positive safe integers.");
+    return PW_EINVAL;
+  }
+  StepParams sp{};
+  if (!pack_params(*layout, *params, params->u_in, &sp)) return PW_EINVAL;
+  // struct Params (common.wgsl:31-46): four u32, four f32, four u32.
+  const uint32_t head[4] = {sp.nx, sp.ny, sp.nz, sp.cells};
+  const float mid[4] = {sp.u_in, sp.nu, sp.cs_term, sp.lambda};
+  const uint32_t tail[4] = {params->les ? 1u : 0u, params->walls == PW_WALLS_BOUZIDI ? 1u : 0u, write_macro ? 1u : 0u,
+                            (uint32_t)step};
+  char* o = static_cast<char*>(out);
+  std::memcpy(o, head, 16);
+  std::memcpy(o + 16, mid, 16);
+  std::memcpy(o + 32, tail, 16);
+  return PW_OK;
 }
 
 pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, size_t geometry_words,
@@ -519,6 +556,10 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   launch_init_both(s, sp);
   if ((e = hipGetLastError()) != hipSuccess) return fail(device_error(e, "init launch"));
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return fail(device_error(e, "create"));
+  {
+    std::lock_guard<std::mutex> lock(g_live_mutex);
+    g_live.insert(s);
+  }
   *out = s;
   return PW_OK;
 }
@@ -623,12 +664,14 @@ pw_status pw_solver_reset(pw_solver* s) {
   return PW_OK;
 }
 
-uint64_t pw_solver_step_count(const pw_solver* s) { return s ? s->steps : 0; }
+uint64_t pw_solver_step_count(const pw_solver* s) { return solver_live(s) ? s->steps : 0; }
 
 void pw_solver_destroy(pw_solver* s) {
-  if (!s || s->destroyed) return;
+  {
+    std::lock_guard<std::mutex> lock(g_live_mutex);
+    if (!s || !g_live.erase(s)) return;
+  }
   free_all(s);
-  s->destroyed = 1;
   delete s;
 }
 
@@ -643,7 +686,7 @@ pw_status pw_solver_read_macro(pw_solver* s, float* out, size_t out_floats) {
   return PW_OK;
 }
 
-size_t pw_solver_population_bytes(const pw_solver* s) { return s ? (size_t)kQ * s->cells * s->elem_bytes : 0; }
+size_t pw_solver_population_bytes(const pw_solver* s) { return solver_live(s) ? (size_t)kQ * s->cells * s->elem_bytes : 0; }
 
 pw_status pw_solver_read_populations(pw_solver* s, void* out, size_t out_bytes) {
   if (pw_status st = check_handle(s)) return st;
@@ -726,9 +769,22 @@ pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
   return PW_OK;
 }
 
-uint32_t pw_solver_kernel(const pw_solver* s) { return s ? s->kernel : 0; }
-uint32_t pw_solver_block_size(const pw_solver* s) { return s ? s->block : 0; }
-uint32_t pw_solver_xcd_chunks(const pw_solver* s) { return s ? s->xcd_chunks : 0; }
+uint32_t pw_solver_kernel(const pw_solver* s) { return solver_live(s) ? s->kernel : 0; }
+uint32_t pw_solver_block_size(const pw_solver* s) { return solver_live(s) ? s->block : 0; }
+uint32_t pw_solver_xcd_chunks(const pw_solver* s) { return solver_live(s) ? s->xcd_chunks : 0; }
+
+pw_status pw_solver_set_step_variant(pw_solver* s, uint32_t kernel, uint32_t block, uint32_t xcd_chunks) {
+  if (pw_status st = check_handle(s)) return st;
+  if (kernel < PW_KERNEL_CELL || kernel > PW_KERNEL_CELL_NT_STORES || (block != 64 && block != 128 && block != 256) ||
+      (xcd_chunks != 1 && xcd_chunks != 16)) {
+    set_error("A step variant needs a kernel 1-3, a block of 64, 128 or 256 and 1 or 16 XCD chunks.");
+    return PW_EINVAL;
+  }
+  s->kernel = kernel;
+  s->block = block;
+  s->xcd_chunks = xcd_chunks;
+  return PW_OK;
+}
 
 pw_status pw_solver_synchronize(pw_solver* s) {
   if (pw_status st = check_handle(s)) return st;

@@ -101,6 +101,8 @@ void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, c
 // The macro field, flags, layout and stream of one batched tunnel (false for a bad index).
 bool batch_field(pw_batch* b, uint32_t tunnel, const float4** macro, const uint32_t** geo, pw_layout* layout,
                  hipStream_t* stream);
+// True while `s` is a handle returned by pw_solver_create and not yet destroyed.
+bool solver_live(const pw_solver* s);
 // `count` steps, the last writing the macros (and the guard word after them) when asked.
 pw_status step_run(pw_solver* s, uint32_t count, int write_macro_on_last);
 }  // namespace pw
@@ -131,7 +133,6 @@ struct pw_solver {
   uint64_t force_step = 0;
   int ramp_settled = 0;
   long long first_fluid = -1;
-  int destroyed = 0;
   uint32_t* wall_cells = nullptr;  // fluid cells with wall links, in cell order
   size_t n_wall = 0;
   uint32_t* closure_slot = nullptr;  // wall-cell index of every cell (learned model only)

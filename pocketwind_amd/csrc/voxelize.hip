@@ -1,7 +1,7 @@
 // voxelize.hip -- triangle-mesh voxelizer, pull wall masks and interpolated-wall link fractions
 // on the GPU, bit-exact against the reference's float64 JavaScript.
 //
-// Follows src/geometry/voxelize.ts:23-214 (three-projection parity vote with half-open edge
+// Follows src/geometry/voxelize.ts:23-199 (three-projection parity vote with half-open edge
 // ownership, EQ/floor planes, counts), src/geometry/wall-links.ts:36-131 (pull masks, q
 // offsets, body boundary list, clamped q) and src/geometry/triangle-bins.ts:14-162 (triangle
 // bins, Moller-Trumbore link hits; the bins here are kBinCells wide, not 4: a link's nearest hit
@@ -200,7 +200,7 @@ struct Bins {
   long nbx, nby, nbz;
 };
 
-// triangle-bins.ts:148-185: f32 bounds and the bin range of one triangle.
+// triangle-bins.ts:17-54: f32 bounds and the bin range of one triangle.
 __global__ void bin_bounds_kernel(const float* __restrict__ tris, uint32_t n_tris, float* __restrict__ bounds,
                                   int* __restrict__ range, long nbx, long nby, long nbz,
                                   uint32_t* __restrict__ count, int mode, const uint32_t* __restrict__ start,
@@ -227,7 +227,7 @@ __global__ void bin_bounds_kernel(const float* __restrict__ tris, uint32_t n_tri
       }
 }
 
-// triangle-bins.ts:191-225
+// triangle-bins.ts:60-94
 __device__ bool intersect(const float* p, const double o[3], const double d[3], double* t_out) {
   const double ax = p[0], ay = p[1], az = p[2];
   const double e1x = (double)p[3] - ax, e1y = (double)p[4] - ay, e1z = (double)p[5] - az;
@@ -251,7 +251,7 @@ __device__ bool intersect(const float* p, const double o[3], const double d[3], 
   return true;
 }
 
-// triangle-bins.ts:259-292
+// triangle-bins.ts:98-162
 __device__ bool nearest_hit(const Bins& b, const double o[3], const double d[3], double* t_out) {
   double lo[3], hi[3];
   long bmin[3], bmax[3];

@@ -5,7 +5,7 @@ write_vtk            writeVtk (vtk.ts:27-80): header lines, big-endian float32 d
 lattice_to_field     latticeToField (field-export.ts:22-30): velocity * speedMps / U_LU, rounded
                      to float32 as Float32Array.map does
 field_vtk_bytes      fieldVtkBytes (field-export.ts:33-35)
-field_file_name      fieldFileName (field-export.ts:38-41)
+field_file_name      fieldFileName (field-export.ts:37-40)
 solver_field_vtk     the same file for a solver's macro buffer, with the binary blocks made on
                      the device in one pass (pw_solver_field_vtk_blocks)
 """
@@ -147,7 +147,7 @@ _JS_SPACE = ("\t\n\v\f\r \u00a0\u1680\u2000\u2001\u2002\u2003\u2004\u2005\u2006\
 
 
 def field_file_name(name: str) -> str:
-    """fieldFileName (field-export.ts:38-41)."""
+    """fieldFileName (field-export.ts:37-40)."""
     safe = _RESERVED.sub("-", name).strip(_JS_SPACE)
     return f"{'untitled' if safe == '' else safe}-field.vtk"
 

@@ -238,6 +238,11 @@ class GpuSolver:
         or re-tune variant and block size ('auto')."""
         N.check(N.lib().pw_solver_set_kernel(self._live(), self.KERNELS[kernel]))
 
+    def set_variant(self, kernel: str, block: int, xcd_chunks: int) -> None:
+        """An explicit step variant: population cache policy, threads per block (64, 128,
+        256) and chunks per XCD (1, 16), without timing."""
+        N.check(N.lib().pw_solver_set_step_variant(self._live(), self.KERNELS[kernel], block, xcd_chunks))
+
     @property
     def kernel(self) -> str:
         code = int(N.lib().pw_solver_kernel(self._live()))

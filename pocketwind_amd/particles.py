@@ -1,7 +1,7 @@
 """Streakline particles over a solver's live velocity (src/render/particles.ts:30-138).
 
 ``Particles.create(solver, body_bounds, count, seed)`` seeds inlet particles as
-particleSeeds does (src/render/particle-layout.ts:126-172); ``update(steps)`` advects them on
+particleSeeds does (src/render/particle-layout.ts:66-112); ``update(steps)`` advects them on
 the solver's stream by trilinear sampling of the macro field written by the last step batch.
 The reference caps ``count`` at 65 536; ``max_count`` raises that cap up to 2^24.
 """
@@ -51,8 +51,33 @@ class Particles:
                                                   C.byref(out)))
         return Particles(out.value, batch, int(count))
 
+    @staticmethod
+    def create_on_field(layout, geometry, macro_ptr: int, body_bounds, count: int, seed: int,
+                        max_count: int = MAX_PARTICLES):
+        """createParticles over a caller's macro buffer (src/render/particles.ts:30-138):
+        ``geometry`` is the grid's host flag array, ``macro_ptr`` the device address of 4
+        floats per cell. The caller finishes writing the buffer before each update."""
+        lo, hi = (np.asarray(v, np.float64) for v in body_bounds)
+        if not isinstance(count, (int, np.integer)) or count < 1 or count > max_count:
+            raise ValueError("Particle count, seed or geometry buffer is invalid.")
+        if not isinstance(seed, (int, np.integer)) or seed < 0 or seed > 0xFFFFFFFF:
+            raise ValueError("Particle count, seed or geometry buffer is invalid.")
+        geo = np.ascontiguousarray(geometry, np.uint32)
+        out = C.c_void_p()
+        N.check(N.lib().pw_particles_create_field(C.byref(N.Layout(*layout)), geo.ctypes.data_as(N._u32p),
+                                                  geo.size, C.c_void_p(int(macro_ptr)), int(count),
+                                                  int(seed), lo.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  hi.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  C.byref(out)))
+        return Particles(out.value, None, int(count))
+
+    def synchronize(self) -> None:
+        N.check(N.lib().pw_particles_synchronize(self._h))
+
     def update(self, steps: float) -> None:
-        if not np.isfinite(np.float32(steps)) or steps < 0:
+        with np.errstate(over="ignore"):
+            finite = np.isfinite(np.float32(steps))
+        if not finite or steps < 0:
             raise ValueError("Particle advection needs non-negative finite solver steps.")
         N.check(N.lib().pw_particles_update(self._h, float(steps)))
 

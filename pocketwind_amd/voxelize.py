@@ -1,6 +1,6 @@
 """voxelize(): the reference's voxelizer API over libpocketwind's GPU voxelizer.
 
-Mirrors src/geometry/voxelize.ts:187-214: ``voxelize(body, supports, layout, floor,
+Mirrors src/geometry/voxelize.ts:172-199: ``voxelize(body, supports, layout, floor,
 link_fractions)`` returns a :class:`VoxelGrid` with the packed geometry
 [flags N][q offsets N][f32 q per link], the body boundary-cell list and the counts.
 Invalid input raises ``ValueError`` (the reference's RangeError).

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) <= set(N.SIGNATURES), set(names) - set(N.SIGNATURES)
-    assert lib.pw_abi_version() == 5
+    assert lib.pw_abi_version() == 6
 
 
 def _create(grid, params, boundary=None):
@@ -145,3 +145,23 @@ def test_batch_closure_particles_and_force_entry_points_reject_null_handles():
     assert out.value is None
     buf = np.zeros(3)
     assert lib.pw_batch_force(None, buf.ctypes.data, 3) == N.PW_EINVAL
+
+
+def test_unknown_or_destroyed_handles_are_rejected_without_being_read():
+    """A handle that pw_solver_create did not return (or that was destroyed) is looked up in
+    the library's set of live handles, never dereferenced: every call returns PW_ESTATE and
+    the getters return 0."""
+    lib = N.lib()
+    stale = C.c_void_p(0x1000)  # never a live handle; dereferencing it would fault the process
+    assert lib.pw_solver_step(stale, 1, 0) == N.PW_ESTATE
+    assert "destroyed" in N.last_error()
+    assert lib.pw_solver_reset(stale) == N.PW_ESTATE
+    assert lib.pw_solver_encode_force(stale) == N.PW_ESTATE
+    assert lib.pw_solver_set_step_variant(stale, 1, 256, 1) == N.PW_ESTATE
+    assert lib.pw_solver_step_count(stale) == 0
+    assert lib.pw_solver_population_bytes(stale) == 0
+    assert lib.pw_solver_wall_cell_count(stale) == 0
+    lib.pw_solver_destroy(stale)  # ignored
+    out = C.c_void_p()
+    assert lib.pw_particles_create(stale, 1, 1, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1),
+                                   C.byref(out)) == N.PW_EINVAL

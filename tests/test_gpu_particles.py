@@ -1,5 +1,5 @@
 """Streakline particles on the GPU against the f32 oracle restatement of
-particles_update.wgsl: seeds equal particleSeeds (particle-layout.ts:126-172), and advection
+particles_update.wgsl: seeds equal particleSeeds (particle-layout.ts:66-112), and advection
 over a solver's live macro field (interpolation, one-cell cap, respawn by hash at the inlet,
 age limit) is bit-identical to the oracle for the same macro field and flags. Also the
 reference's browser known values (tests/browser/particles.spec.ts: no particle ever sits in a
@@ -112,3 +112,44 @@ def test_rejects_invalid_counts_and_regions():
         big.destroy()
     finally:
         solver.destroy()
+
+
+def test_browser_spec_known_values_on_the_device():
+    """tests/browser/particles.spec.ts:19-127 through pw_particles_create_field (the reference's
+    createParticles over a caller's macro buffer): the same seven records and expectations the
+    oracle meets in test_oracle_particles.py."""
+    import torch
+
+    from test_oracle_particles import (SPEC_BOUNDS, SPEC_LAYOUT, SPEC_SEED, check_spec_results,
+                                       spec_macros, spec_record)
+    graded, uniform = spec_macros()
+    macro = torch.from_numpy(graded).to("cuda:0")
+    torch.cuda.synchronize()
+    p = Particles.create_on_field(SPEC_LAYOUT, np.zeros(4096, np.uint32), macro.data_ptr(), SPEC_BOUNDS, 1,
+                                  SPEC_SEED)
+    try:
+        region, fallback = p.info()
+        assert np.allclose(region, [3.1, 3.1, 10.9, 10.9])
+
+        def record(steps, **kw):
+            p.write(spec_record(**kw)[None, :])
+            p.update(steps)
+            return p.read()[0]
+
+        results = {"interpolated": record(2)}
+        macro.copy_(torch.from_numpy(uniform))
+        torch.cuda.synchronize()
+        results.update(capped=record(9), stationary=record(0), age_limit=record(0, age=3999),
+                       expired=record(0, age=4000), outlet=record(0, x=14),
+                       next_generation=record(0, age=4000, generation=1))
+        print({k: [round(float(v), 6) for v in r] for k, r in results.items()})
+        check_spec_results(results)
+        # The device equals the f32 oracle bit for bit on the same records.
+        flags = np.zeros(4096, np.uint32)
+        ref = O.particles_update(spec_record(), graded, flags, SPEC_LAYOUT, SPEC_SEED, 2, region,
+                                 fallback[1:])
+        macro.copy_(torch.from_numpy(graded))
+        torch.cuda.synchronize()
+        assert np.array_equal(record(2).view(np.uint32), ref.view(np.uint32))
+    finally:
+        p.destroy()
