@@ -201,6 +201,11 @@ uint32_t pw_solver_kernel(const pw_solver *s);
  * per block (64, 128 or 256) and `xcd_chunks` (1 or 16) chunks per XCD, without timing. */
 pw_status pw_solver_set_step_variant(pw_solver *s, uint32_t kernel, uint32_t block,
                                      uint32_t xcd_chunks);
+/* Test hook: with `on`, step with one cell per lane even when nx is a multiple of 64 (whose
+ * steps otherwise give each wavefront one 64-cell x-row), in the same population layout. The
+ * two paths run the same collision code; comparing them pins the row path to the per-cell
+ * path. No reference counterpart. */
+pw_status pw_debug_cell_path(pw_solver *s, int on);
 /* Threads per block of the step launches. */
 uint32_t pw_solver_block_size(const pw_solver *s);
 /* Chunks of the lattice each of the 8 XCDs steps (1: one contiguous eighth each; 16: chunks

@@ -133,6 +133,7 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
   out->x_tile = out->run == 64 ? 18u * 64u : 0u;
   out->div_nx = make_fast_div(L.nx);
   out->div_ny = make_fast_div(L.ny);
+  out->rows = L.nx % 64 == 0 ? 1u : 0u;
   out->bc = (prm.sides == PW_SIDES_FREE_SLIP ? kBcFreeSlip : 0u) |
             (prm.outlet == PW_OUTLET_ZERO_GRADIENT ? kBcZeroGradient : 0u);
   return true;
@@ -772,6 +773,13 @@ pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
 uint32_t pw_solver_kernel(const pw_solver* s) { return solver_live(s) ? s->kernel : 0; }
 uint32_t pw_solver_block_size(const pw_solver* s) { return solver_live(s) ? s->block : 0; }
 uint32_t pw_solver_xcd_chunks(const pw_solver* s) { return solver_live(s) ? s->xcd_chunks : 0; }
+
+pw_status pw_debug_cell_path(pw_solver* s, int on) {
+  if (pw_status st = check_handle(s)) return st;
+  PW_TRY(hipStreamSynchronize(s->stream), "cell path");
+  s->sp.rows = (!on && s->layout.nx % 64 == 0) ? 1u : 0u;
+  return PW_OK;
+}
 
 pw_status pw_solver_set_step_variant(pw_solver* s, uint32_t kernel, uint32_t block, uint32_t xcd_chunks) {
   if (pw_status st = check_handle(s)) return st;

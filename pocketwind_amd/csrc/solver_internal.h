@@ -65,6 +65,7 @@ struct StepParams {
   float wm0;       // TRT odd rate 1 / (lambda / (tau0 - 1/2) + 1/2) in f32
   FastDiv div_nx, div_ny;
   uint32_t bc;  // kBcFreeSlip | kBcZeroGradient, or 0 for the reference's periodic y/z sides
+  uint32_t rows;  // 1: one wavefront per 64-cell x-row (nx a multiple of 64); 0: one cell per lane
   // Learned wall closure (null: the wall law): shear stress per wall cell (xyz, w > 0 when
   // valid) and the wall-cell index of every cell.
   const float4* closure_tau;

@@ -961,7 +961,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<
   const uint32_t logical = ((local / chunk) * kXcds + b % kXcds) * chunk + local % chunk;
   const uint32_t n = logical * blockDim.x + threadIdx.x;
   if (n >= p.cells) return;  // whole wavefronts when nx is a multiple of 64
-  if (p.nx % 64 == 0)
+  if (p.rows)
     stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL, BC>(in, out, geo, macro, p, n, seed);
   else
     stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL, true, BC>(in, out, geo, macro, p, n, seed);

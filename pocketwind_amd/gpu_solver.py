@@ -238,6 +238,10 @@ class GpuSolver:
         or re-tune variant and block size ('auto')."""
         N.check(N.lib().pw_solver_set_kernel(self._live(), self.KERNELS[kernel]))
 
+    def force_cell_path(self, on: bool = True) -> None:
+        """Test hook: one cell per lane even on row-wavefront grids (nx a multiple of 64)."""
+        N.check(N.lib().pw_debug_cell_path(self._live(), int(bool(on))))
+
     def set_variant(self, kernel: str, block: int, xcd_chunks: int) -> None:
         """An explicit step variant: population cache policy, threads per block (64, 128,
         256) and chunks per XCD (1, 16), without timing."""

@@ -18,6 +18,31 @@ from pocketwind_amd.voxelize import voxelize
 pytestmark = pytest.mark.gpu
 
 
+class DeviceBuffer:
+    """A caller's device buffer, allocated through the HIP runtime the library itself links
+    (plumbing for the field-bound particles; uploads are synchronous)."""
+
+    def __init__(self, nbytes: int):
+        import ctypes as C
+        self._C = C
+        self._hip = C.CDLL("libamdhip64.so.7")
+        ptr = C.c_void_p()
+        assert self._hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) == 0
+        self.ptr, self.nbytes = ptr.value, nbytes
+
+    def upload(self, array) -> None:
+        data = np.ascontiguousarray(array)
+        assert data.nbytes == self.nbytes
+        assert self._hip.hipMemcpy(self._C.c_void_p(self.ptr), self._C.c_void_p(data.ctypes.data),
+                                   self._C.c_size_t(self.nbytes), 1) == 0  # hipMemcpyHostToDevice
+        assert self._hip.hipDeviceSynchronize() == 0
+
+    def free(self) -> None:
+        if self.ptr:
+            self._hip.hipFree(self._C.c_void_p(self.ptr))
+            self.ptr = None
+
+
 def sphere_case(layout=(64, 32, 32)):
     body, _ = bodies.place_mesh(bodies.sphere_mesh(), None, 8, layout, False)
     grid = voxelize(body, None, layout, False, link_fractions=False)
@@ -118,14 +143,12 @@ def test_browser_spec_known_values_on_the_device():
     """tests/browser/particles.spec.ts:19-127 through pw_particles_create_field (the reference's
     createParticles over a caller's macro buffer): the same seven records and expectations the
     oracle meets in test_oracle_particles.py."""
-    import torch
-
     from test_oracle_particles import (SPEC_BOUNDS, SPEC_LAYOUT, SPEC_SEED, check_spec_results,
                                        spec_macros, spec_record)
     graded, uniform = spec_macros()
-    macro = torch.from_numpy(graded).to("cuda:0")
-    torch.cuda.synchronize()
-    p = Particles.create_on_field(SPEC_LAYOUT, np.zeros(4096, np.uint32), macro.data_ptr(), SPEC_BOUNDS, 1,
+    macro = DeviceBuffer(graded.nbytes)
+    macro.upload(graded)
+    p = Particles.create_on_field(SPEC_LAYOUT, np.zeros(4096, np.uint32), macro.ptr, SPEC_BOUNDS, 1,
                                   SPEC_SEED)
     try:
         region, fallback = p.info()
@@ -137,8 +160,7 @@ def test_browser_spec_known_values_on_the_device():
             return p.read()[0]
 
         results = {"interpolated": record(2)}
-        macro.copy_(torch.from_numpy(uniform))
-        torch.cuda.synchronize()
+        macro.upload(uniform)
         results.update(capped=record(9), stationary=record(0), age_limit=record(0, age=3999),
                        expired=record(0, age=4000), outlet=record(0, x=14),
                        next_generation=record(0, age=4000, generation=1))
@@ -148,8 +170,8 @@ def test_browser_spec_known_values_on_the_device():
         flags = np.zeros(4096, np.uint32)
         ref = O.particles_update(spec_record(), graded, flags, SPEC_LAYOUT, SPEC_SEED, 2, region,
                                  fallback[1:])
-        macro.copy_(torch.from_numpy(graded))
-        torch.cuda.synchronize()
+        macro.upload(graded)
         assert np.array_equal(record(2).view(np.uint32), ref.view(np.uint32))
     finally:
         p.destroy()
+        macro.free()
