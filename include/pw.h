@@ -220,6 +220,17 @@ uint32_t pw_solver_xcd_chunks(const pw_solver *s);
 pw_status pw_solver_step_sampled(pw_solver *s, uint32_t count, uint32_t every, double *out,
                                  size_t out_doubles);
 
+/* pw_solver_step_sampled without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No reference counterpart. The same steps, force samples and output as
+ * pw_solver_step_sampled, with the sample history kept in a fixed device array of the library
+ * (4096 samples of 3 doubles, 96 KiB per GPU, held until the process ends) instead of a per-call
+ * allocation; a call asking for more than 4096 samples returns PW_EINVAL before stepping.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (from several
+ * threads or solvers at once) race. Blocks until done. */
+pw_status pw_solver_step_sampled_noalloc(pw_solver *s, uint32_t count, uint32_t every,
+                                         double *out, size_t out_doubles);
+
 /* Wait for all queued work of the handle. */
 pw_status pw_solver_synchronize(pw_solver *s);
 

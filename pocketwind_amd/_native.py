@@ -77,6 +77,8 @@ SIGNATURES = {
     "pw_solver_kernel": (C.c_uint32, [C.c_void_p]),
     "pw_solver_synchronize": (C.c_int, [C.c_void_p]),
     "pw_solver_step_sampled": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "pw_solver_step_sampled_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                                 C.c_size_t]),
     "pw_solver_time_steps": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
     "pw_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_uint32, C.POINTER(C.c_double)]),
     "pw_voxelize": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.POINTER(Layout),

@@ -841,6 +841,88 @@ pw_status pw_solver_step_sampled(pw_solver* s, uint32_t count, uint32_t every, d
   return PW_OK;
 }
 
+}  // extern "C"
+
+namespace pw {
+namespace {
+
+// Scratch of pw_solver_step_sampled_noalloc: the FP64 force history (fx, fy, fz per sample)
+// of one call, at most kScratchSamples samples (96 KiB). Every GPU holds its own copy for the
+// life of the process; all calls on one GPU share it, so calls that overlap on one GPU race.
+constexpr size_t kScratchSamples = 4096;
+static __device__ __align__(256) double scratch[3 * kScratchSamples];
+
+// Device address of `scratch` on each device, looked up once per device with that device
+// current. The host-side name `scratch` is only a stand-in for the symbol; kernels receive
+// this address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_on(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    int current = 0;
+    hipError_t e = hipGetDevice(&current);
+    if (e != hipSuccess) return e;
+    if (current != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (current != device) (void)hipSetDevice(current);
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+}  // namespace
+}  // namespace pw
+
+extern "C" {
+
+pw_status pw_solver_step_sampled_noalloc(pw_solver* s, uint32_t count, uint32_t every, double* out,
+                                         size_t out_doubles) {
+  if (pw_status st = check_handle(s)) return st;
+  if (every == 0 || !out) {
+    set_error("Sampling needs a positive interval and an output buffer.");
+    return PW_EINVAL;
+  }
+  // Samples fall on the multiples of `every` in (steps, steps + count].
+  const uint64_t first = (s->steps / every + 1) * every;
+  const uint64_t last = s->steps + count;
+  const size_t samples = last >= first ? (size_t)((last - first) / every + 1) : 0;
+  if (samples > kScratchSamples) {
+    set_error("The allocation-free sampled stepper holds at most 4096 samples per call.");
+    return PW_EINVAL;
+  }
+  if (out_doubles < 4 * samples) {
+    set_error("Sample output needs 4 doubles per sample.");
+    return PW_EINVAL;
+  }
+  double* hist = nullptr;
+  PW_TRY(scratch_on(s->device, &hist), "scratch address");
+  size_t k = 0;
+  for (uint32_t n = 0; n < count; ++n) {
+    if (pw_status st = step_run(s, 1, 0)) return st;
+    if (s->steps % every == 0) launch_force(s, hist + 3 * k++);
+  }
+  PW_TRY(hipGetLastError(), "sampled steps");
+  // The history lands in the tail of `out` and is spread to (step, fx, fy, fz) records from the
+  // front: record j ends at 4j + 3 < samples + 3(j + 1), where unread entries start.
+  if (samples) PW_TRY(hipMemcpyAsync(out + samples, hist, 3 * samples * sizeof(double), hipMemcpyDeviceToHost, s->stream), "sample readback");
+  PW_TRY(hipStreamSynchronize(s->stream), "sampled steps");
+  for (size_t j = 0; j < samples; ++j) {
+    const double fx = out[samples + 3 * j], fy = out[samples + 3 * j + 1], fz = out[samples + 3 * j + 2];
+    out[4 * j] = (double)(first + j * every);
+    out[4 * j + 1] = fx;
+    out[4 * j + 2] = fy;
+    out[4 * j + 3] = fz;
+  }
+  return PW_OK;
+}
+
 pw_status pw_solver_time_steps(pw_solver* s, uint32_t count, float* elapsed_ms) {
   if (pw_status st = check_handle(s)) return st;
   PW_TRY(hipEventRecord(s->t0, s->stream), "timing event");
