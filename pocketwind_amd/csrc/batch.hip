@@ -328,6 +328,13 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
     if (n_boundary[t] && (e = hipMemcpyAsync(bnd, boundary_cells[t], n_boundary[t] * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return fail(e, "batch upload");
     tu.geo = geo;
     tu.macro = macro;
+    if (tu.p.rows) {
+      const std::vector<uint32_t> desc = wave_descriptors(layouts[t], geometries[t]);
+      uint32_t* wd = nullptr;
+      if ((e = dev_alloc(b, &wd, desc.size())) != hipSuccess) return fail(e, "batch descriptors");
+      if ((e = hipMemcpy(wd, desc.data(), desc.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "batch upload");
+      tu.p.wave_desc = wd;
+    }
     tu.boundary = bnd;
     tu.n_boundary = (uint32_t)n_boundary[t];
     tu.partial_base = part_base;

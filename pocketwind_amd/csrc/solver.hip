@@ -134,10 +134,32 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
   out->div_nx = make_fast_div(L.nx);
   out->div_ny = make_fast_div(L.ny);
   out->rows = L.nx % 64 == 0 ? 1u : 0u;
+  out->wave_desc = nullptr;
   out->bc = (prm.sides == PW_SIDES_FREE_SLIP ? kBcFreeSlip : 0u) |
             (prm.outlet == PW_OUTLET_ZERO_GRADIENT ? kBcZeroGradient : 0u);
   return true;
 }
+
+}  // namespace
+
+std::vector<uint32_t> wave_descriptors(const pw_layout& L, const uint32_t* flags) {
+  const size_t runs = (size_t)L.nx * L.ny * L.nz / 64;
+  std::vector<uint32_t> desc(runs, 0u);
+  for (size_t r = 0; r < runs; ++r) {
+    const uint32_t* f = flags + 64 * r;
+    uint32_t d = 0;
+    for (uint32_t l = 0; l < 64 && d != kWaveGeneral; ++l) {
+      if (f[l] == 0u) continue;
+      if (l == 0 && f[l] == kEq) d |= kWaveEqFirst;
+      else if (l == 63 && f[l] == kEq) d |= kWaveEqLast;
+      else d = kWaveGeneral;
+    }
+    desc[r] = d;
+  }
+  return desc;
+}
+
+namespace {
 
 // Free-slip sides: no wall cell within two cells of a side, so that the wall masks over wrapped
 // neighbours equal those over the step's sources and no wall gather crosses a side.
@@ -345,6 +367,7 @@ void free_all(pw_solver* s) {
   if (s->geometry) hipFree(s->geometry);
   if (s->boundary) hipFree(s->boundary);
   if (s->wall_cells) hipFree(s->wall_cells);
+  if (s->wave_desc) hipFree(s->wave_desc);
   if (s->closure_slot) hipFree(s->closure_slot);
   if (s->closure_tau) hipFree(s->closure_tau);
   if (s->closure_net) hipFree(s->closure_net);
@@ -529,6 +552,15 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   if ((e = hipMalloc((void**)&s->macro, (N + 1) * sizeof(float4))) != hipSuccess) return fail(device_error(e, "macro allocation"));
   if ((e = hipMalloc((void**)&s->boundary, (n_boundary ? n_boundary : 1) * 4)) != hipSuccess) return fail(device_error(e, "boundary allocation"));
   if ((e = hipMalloc((void**)&s->wall_cells, (s->n_wall ? s->n_wall : 1) * 4)) != hipSuccess) return fail(device_error(e, "wall cell allocation"));
+  if (full.rows) {
+    const std::vector<uint32_t> desc = wave_descriptors(L, geometry);
+    if ((e = hipMalloc((void**)&s->wave_desc, desc.size() * 4)) != hipSuccess) return fail(device_error(e, "descriptor allocation"));
+    if ((e = hipMemcpy(s->wave_desc, desc.data(), desc.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(device_error(e, "descriptor upload"));
+    full.wave_desc = s->wave_desc;
+    sp.wave_desc = s->wave_desc;
+    s->sp.wave_desc = s->wave_desc;
+  }
   if (learned) {
     std::vector<uint32_t> slot(N, 0xffffffffu);
     for (size_t k = 0; k < wall_list.size(); ++k) slot[wall_list[k]] = (uint32_t)k;

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "pw.h"
 
@@ -66,11 +67,22 @@ struct StepParams {
   FastDiv div_nx, div_ny;
   uint32_t bc;  // kBcFreeSlip | kBcZeroGradient, or 0 for the reference's periodic y/z sides
   uint32_t rows;  // 1: one wavefront per 64-cell x-row (nx a multiple of 64); 0: one cell per lane
+  // Row path only: one descriptor word per 64-cell run (wave_descriptors), or null to read
+  // every cell's flag.
+  const uint32_t* wave_desc;
   // Learned wall closure (null: the wall law): shear stress per wall cell (xyz, w > 0 when
   // valid) and the wall-cell index of every cell.
   const float4* closure_tau;
   const uint32_t* closure_slot;
 };
+
+// Descriptor of a 64-cell run on the row path. 0: every cell is a fluid cell without wall
+// links (flag 0). kWaveEqFirst / kWaveEqLast: the same, except that the first / last cell is an
+// EQ cell (flag kEq, the inlet and outlet planes). kWaveGeneral: anything else; the step reads
+// each cell's flag.
+enum : uint32_t { kWaveGeneral = 1u, kWaveEqFirst = 2u, kWaveEqLast = 4u };
+// The descriptors of a lattice whose nx is a multiple of 64, from its cell flags.
+std::vector<uint32_t> wave_descriptors(const pw_layout& layout, const uint32_t* flags);
 
 void set_error(const std::string& message);
 pw_status device_error(hipError_t e, const char* what);
@@ -135,6 +147,7 @@ struct pw_solver {
   int ramp_settled = 0;
   long long first_fluid = -1;
   uint32_t* wall_cells = nullptr;  // fluid cells with wall links, in cell order
+  uint32_t* wave_desc = nullptr;   // row path: descriptor of each 64-cell run
   size_t n_wall = 0;
   uint32_t* closure_slot = nullptr;  // wall-cell index of every cell (learned model only)
   float4* closure_tau = nullptr;     // closure output per wall cell

@@ -50,10 +50,41 @@ struct Stress {
   float xx, yy, zz, xy, xz, yz;
 };
 
+// A sum whose terms are known at compile time to be present or absent (products with the
+// D3Q19 velocity components, which are -1, 0 or 1). The first present term starts it, so no
+// product with a zero component and no addition of an initial zero is ever evaluated: IEEE
+// arithmetic cannot fold either away (0 * x is not 0 for every x, nor is 0 + x always x).
+struct Sum {
+  float v = 0.0f;
+  bool any = false;
+  __host__ __device__ __forceinline__ void add(float t) {
+    v = any ? v + t : t;
+    any = true;
+  }
+  __host__ __device__ __forceinline__ void sub(float t) {
+    v = any ? v - t : -t;
+    any = true;
+  }
+  // c t for a velocity component c in {-1, 0, 1}.
+  __host__ __device__ __forceinline__ void add(int c, float t) {
+    if (c > 0) add(t);
+    else if (c < 0) sub(t);
+  }
+};
+
+// c_i . (x, y, z) from the nonzero components of c_i only.
+__host__ __device__ __forceinline__ float cdot(int i, float x, float y, float z) {
+  Sum r;
+  r.add(cx(i), x);
+  r.add(cy(i), y);
+  r.add(cz(i), z);
+  return r.v;
+}
+
 // common.wgsl:84-88: equilibrium with the density deviation kept apart from unit density.
 __host__ __device__ __forceinline__ float geq(int i, float rho_delta, float ux, float uy, float uz,
                                               float uu) {
-  const float cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+  const float cu = cdot(i, ux, uy, uz);
   return wf(i) * (rho_delta + (1.0f + rho_delta) * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu));
 }
 
@@ -67,37 +98,59 @@ struct CellState {
   float sum[kPairs], dif[kPairs];  // g_{2p+1} + g_{2p+2}, g_{2p+1} - g_{2p+2}
 };
 
-__host__ __device__ __forceinline__ void cell_state(const float* g, CellState& c) {
+__host__ __device__ __forceinline__ void cell_moments(const float* g, CellState& c) {
 #pragma unroll
   for (int p = 0; p < kPairs; ++p) {
     c.sum[p] = g[2 * p + 1] + g[2 * p + 2];
     c.dif[p] = g[2 * p + 1] - g[2 * p + 2];
   }
-  float rd = g[0], mx = 0.0f, my = 0.0f, mz = 0.0f;
+  float rd = g[0];
+  Sum mx, my, mz;
 #pragma unroll
   for (int p = 0; p < kPairs; ++p) {
     const int i = 2 * p + 1;
     rd += c.sum[p];
-    if (cx(i)) mx += cx(i) * c.dif[p];
-    if (cy(i)) my += cy(i) * c.dif[p];
-    if (cz(i)) mz += cz(i) * c.dif[p];
+    mx.add(cx(i), c.dif[p]);
+    my.add(cy(i), c.dif[p]);
+    mz.add(cz(i), c.dif[p]);
   }
   c.rd = rd;
   c.rho = 1.0f + rd;
   c.inv_rho = fast_rcp(c.rho);
-  c.ux = mx * c.inv_rho;
-  c.uy = my * c.inv_rho;
-  c.uz = mz * c.inv_rho;
+  c.ux = mx.v * c.inv_rho;
+  c.uy = my.v * c.inv_rho;
+  c.uz = mz.v * c.inv_rho;
+}
+
+// A cell whose state is prescribed (an EQ cell, or a wall at rest) keeps its gathered sums only
+// as placeholders: unit density and the velocity (ux, uy, uz) replace its moments.
+__host__ __device__ __forceinline__ void prescribe_moments(CellState& c, float ux, float uy, float uz) {
+  c.rd = 0.0f;
+  c.rho = 1.0f;
+  c.inv_rho = 1.0f;
+  c.ux = ux;
+  c.uy = uy;
+  c.uz = uz;
+}
+
+// The pair-form equilibrium of the cell's moments.
+__host__ __device__ __forceinline__ void cell_equilibrium(CellState& c) {
+  const float rd = c.rd;
   const float h = 1.5f * (c.ux * c.ux + c.uy * c.uy + c.uz * c.uz);
   c.eq0 = wf(0) * (rd - c.rho * h);
 #pragma unroll
   for (int p = 0; p < kPairs; ++p) {
     const int i = 2 * p + 1;
-    const float cu = cx(i) * c.ux + cy(i) * c.uy + cz(i) * c.uz;
+    const float cu = cdot(i, c.ux, c.uy, c.uz);
     const float rw = wf(i) * c.rho;
     c.a[p] = rw * (4.5f * cu * cu - h) + wf(i) * rd;
     c.b[p] = (3.0f * rw) * cu;
   }
+}
+
+__host__ __device__ __forceinline__ void cell_state(const float* g, CellState& c) {
+  cell_moments(g, c);
+  cell_equilibrium(c);
 }
 
 __host__ __device__ __forceinline__ float eq_of(const CellState& c, int i) {
@@ -109,18 +162,19 @@ __host__ __device__ __forceinline__ float eq_of(const CellState& c, int i) {
 // common.wgsl:100-113: non-equilibrium stress, here as the second moment of g minus its
 // equilibrium value (rho - 1)/3 I + rho u u.
 __host__ __device__ __forceinline__ Stress neq_stress(const CellState& c) {
-  Stress s{0, 0, 0, 0, 0, 0};
+  Sum xx, yy, zz, xy, xz, yz;
 #pragma unroll
   for (int p = 0; p < kPairs; ++p) {
     const int i = 2 * p + 1;
     const float v = c.sum[p];
-    if (cx(i)) s.xx += v;
-    if (cy(i)) s.yy += v;
-    if (cz(i)) s.zz += v;
-    if (cx(i) * cy(i)) s.xy += cx(i) * cy(i) * v;
-    if (cx(i) * cz(i)) s.xz += cx(i) * cz(i) * v;
-    if (cy(i) * cz(i)) s.yz += cy(i) * cz(i) * v;
+    xx.add(cx(i) * cx(i), v);
+    yy.add(cy(i) * cy(i), v);
+    zz.add(cz(i) * cz(i), v);
+    xy.add(cx(i) * cy(i), v);
+    xz.add(cx(i) * cz(i), v);
+    yz.add(cy(i) * cz(i), v);
   }
+  Stress s{xx.v, yy.v, zz.v, xy.v, xz.v, yz.v};
   const float third = c.rd * (1.0f / 3.0f);
   const float rx = c.rho * c.ux, ry = c.rho * c.uy, rz = c.rho * c.uz;
   s.xx -= rx * c.ux + third;
@@ -138,17 +192,27 @@ __host__ __device__ __forceinline__ float les_tau(const StepParams& p, const Str
   return 0.5f * (p.tau0 + fast_sqrt(p.tau0 * p.tau0 + p.cs_term * fast_sqrt(q) * inv_rho));
 }
 
+// c_i c_i : s - trace(s) / 3 (third), from the components c_i has.
 __host__ __device__ __forceinline__ float projected(int i, const Stress& s, float third) {
-  const float x = cx(i), y = cy(i), z = cz(i);
-  return x * x * s.xx + y * y * s.yy + z * z * s.zz + 2.0f * (x * y * s.xy + x * z * s.xz + y * z * s.yz) -
-         third;
+  Sum diag, off;
+  diag.add(cx(i) * cx(i), s.xx);
+  diag.add(cy(i) * cy(i), s.yy);
+  diag.add(cz(i) * cz(i), s.zz);
+  off.add(cx(i) * cy(i), s.xy);
+  off.add(cx(i) * cz(i), s.xz);
+  off.add(cy(i) * cz(i), s.yz);
+  return (off.any ? diag.v + 2.0f * off.v : diag.v) - third;
 }
 
 // collide-trt.wgsl:1-15 and collide-reg.wgsl:3-18; writes the post-collision state into g.
+// With `prescribed` (REG only) the non-equilibrium stress is zero, so the cell leaves with its
+// equilibrium: the state of an EQ cell, or the rest state of a wall.
 template <int OP, bool LES>
-__host__ __device__ __forceinline__ void collide(const StepParams& p, float* g, const CellState& c) {
+__host__ __device__ __forceinline__ void collide(const StepParams& p, float* g, const CellState& c,
+                                                 bool prescribed = false) {
   Stress s{0, 0, 0, 0, 0, 0};
   if (LES || OP == kOpReg) s = neq_stress(c);
+  if (OP == kOpReg && prescribed) s = Stress{0, 0, 0, 0, 0, 0};
   const float tau = LES ? les_tau(p, s, c.inv_rho) : p.tau0;
   const float wp = LES ? fast_rcp(tau) : p.inv_tau0;
   if (OP == kOpReg) {
@@ -234,7 +298,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
     if (!has_wall(flag, i)) continue;
     const int j = opp(i);
     const float x = cx(j), y = cy(j), z = cz(j);
-    const float along = x * tx + y * ty + z * tz;
+    const float along = cdot(j, tx, ty, tz);
     // The weight part of the flux, sum 2 along_j w_j = -2 n.t, is exactly zero once the slip
     // has had its normal component removed; evaluated in f32 it would only add the rounding
     // residual of t's normal component, amplified by 1 / |slip| when the slip is small.
@@ -409,7 +473,7 @@ __host__ __device__ __forceinline__ void equilibrium_values(float ux, float uy, 
 #pragma unroll
   for (int q = 0; q < kPairs; ++q) {
     const int i = 2 * q + 1;
-    const float cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+    const float cu = cdot(i, ux, uy, uz);
     const float a = wf(i) * (4.5f * cu * cu - h15);
     const float b = (3.0f * wf(i)) * cu;
     g[i] = a + b;
@@ -529,6 +593,24 @@ __host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, 
   return make_float4(c.rho, c.ux, c.uy, c.uz);
 }
 
+// relax for any lane of a REG step (TRT steps call it for fluid lanes only): fluid lanes relax their gathered populations; a prescribed
+// lane (EQ or wall) takes unit density and the velocity (ux, uy, uz) as its moments and zero
+// non-equilibrium stress, so it leaves with that equilibrium (a wall with the rest state, exactly
+// zero) through the same arithmetic as every fluid lane instead of a separate branch.
+template <int OP, bool LES>
+__host__ __device__ __forceinline__ float4 relax_prescribed(const StepParams& p, float* g, uint32_t flag, uint32_t n,
+                                                            bool prescribed, float ux, float uy, float uz) {
+  CellState c;
+  cell_moments(g, c);
+  if (prescribed) prescribe_moments(c, ux, uy, uz);
+  cell_equilibrium(c);
+  if (LES && (flag >> 3) != 0)
+    collide_wall<OP>(p, g, c, flag, n);
+  else
+    collide<OP, LES>(p, g, c, prescribed);
+  return make_float4(c.rho, c.ux, c.uy, c.uz);
+}
+
 // The outlet plane x = nx - 1 takes its velocity from the stored state of the cell upstream,
 // x = nx - 2 (stream_collide.wgsl:102-107): (1 + sum g, sum c g) of its 19 stored values, in
 // direction order.
@@ -584,14 +666,22 @@ __host__ __device__ __forceinline__ RowMoments encoded_moments(const float* g, u
 // stream_collide.wgsl:83-107: an EQ cell takes the equilibrium at unit density and the
 // inlet speed or, on the outlet plane x = nx - 1 (nx > 1), the velocity of the upstream
 // moments `up`. Writes the post-step values into g and returns (1, u).
-__host__ __device__ __forceinline__ float4 eq_from(const StepParams& p, uint32_t x, const RowMoments& up, float* g) {
-  float ux = p.u_in, uy = 0.0f, uz = 0.0f;
+__host__ __device__ __forceinline__ void eq_velocity(const StepParams& p, uint32_t x, const RowMoments& up, float& ux,
+                                                    float& uy, float& uz) {
+  ux = p.u_in;
+  uy = 0.0f;
+  uz = 0.0f;
   if (x == p.nx - 1 && x > 0) {
     const float inv = fast_rcp(up.mass);
     ux = up.mx * inv;
     uy = up.my * inv;
     uz = up.mz * inv;
   }
+}
+
+__host__ __device__ __forceinline__ float4 eq_from(const StepParams& p, uint32_t x, const RowMoments& up, float* g) {
+  float ux, uy, uz;
+  eq_velocity(p, x, up, ux, uy, uz);
   equilibrium_values(ux, uy, uz, g);
   return make_float4(1.0f, ux, uy, uz);
 }
@@ -704,7 +794,23 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
                                  : S::load(byte_offset(in + (size_t)i * P, pnb.at(-cx(i), -cy(i), -cz(i))));
     }
     if (BC && (p.bc & kBcFreeSlip) && on_side(y, z, p)) slip_cell<S, POL == kNtAll>(in, p, pnb, y, z, g);
-    m = relax<OP, LES>(p, g, flag, n);
+    if constexpr (OP != kOpReg) m = relax<OP, LES>(p, g, flag, n);
+  } else if (OP == kOpReg) {
+    rest_state(g);  // placeholders: an EQ or wall cell's gathered values do not enter its state
+  }
+  if constexpr (OP == kOpReg) {
+    // EQ and wall cells take their equilibrium through the fluid cells' collision, from the one
+    // call site the row path has too, so that both compile the collision alike.
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (kind == kEq) {
+      RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
+      if (x == p.nx - 1 && x > 0) {
+        if constexpr (EDGE) up = row_moments(edge_rows(in, p)[y + p.ny * z]);
+        else up = stored_moments<S>(in, p, n - 1);
+      }
+      eq_velocity(p, x, up, ux, uy, uz);
+    }
+    m = relax_prescribed<OP, LES>(p, g, flag, n, kind != kFluid, ux, uy, uz);
   } else if (kind == kEq) {
     if constexpr (EDGE) {
       RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
@@ -713,7 +819,7 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
     } else {
       m = eq_state<S>(in, p, n, x, g);
     }
-  } else {
+  } else if (kind != kFluid) {
     rest_state(g);
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
@@ -783,11 +889,15 @@ __device__ __forceinline__ void row_columns(const StepParams& p, uint32_t bc, ui
   xs = (pop_x(p, x0) + lane) << SH;
 }
 
+// On the row path every run holds one wavefront's 64 cells, so direction i's run starts i * 64
+// elements after the row's first run: a compile-time offset from one scalar base per source
+// row (nine rows feed a D3Q19 pull), which the loads and stores carry as immediates.
+constexpr uint32_t kRowRun = 64;
+
 template <class S, bool NT>
 __device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in, const StepParams& p, uint32_t bc,
                                            const RowWave& w, uint32_t lane, float* g) {
   constexpr int SH = S::kShift;
-  const size_t P = p.run;  // 64: one wavefront's run
   const uint32_t row = kQ * p.nx, plane = row * p.ny;
   const uint32_t y = w.y, z = w.z;
   const uint32_t yo[3] = {((y == 0 ? p.ny - 1 : y - 1) * row) << SH, (y * row) << SH,
@@ -798,7 +908,7 @@ __device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in,
   row_columns<SH>(p, bc, w.x0, lane, left, right, xs);
   static_for<0, kQ>([&](auto c) {
     constexpr int i = decltype(c)::value;
-    g[i] = pull_row<S, NT, i>(byte_offset(in + (size_t)i * P, yo[1 - cy(i)] + zo[1 - cz(i)]), xs, left, right);
+    g[i] = pull_row<S, NT, i>(byte_offset(in, yo[1 - cy(i)] + zo[1 - cz(i)]) + i * kRowRun, xs, left, right);
   });
 }
 
@@ -828,38 +938,25 @@ __device__ __forceinline__ void slip_row(const typename S::T* __restrict__ in, c
   });
 }
 
-// Base of plane i at cell n0 as an opaque wavefront-uniform global pointer. Kept opaque so
-// the compiler addresses each store as scalar base + lane offset rather than chaining 64-bit
-// vector adds from one plane to the next.
-template <class T>
-__device__ __forceinline__ __attribute__((address_space(1))) T* plane_base(T* out, size_t P, int i, uint32_t e0) {
-  T* p = out + (size_t)i * P + e0;
-  asm("" : "+s"(p));
-  return (__attribute__((address_space(1))) T*)p;
-}
-
-// Stores of a row wavefront: plane i of cell n0 + lane. 16-bit formats round two directions
-// into one register and store its halves.
+// Stores of a row wavefront: run i of the row at e0 (the wavefront's first element), lane
+// `lane`. 16-bit formats round two directions into one register and store its halves.
 // With `moments` (wavefront-uniform) the stored values' moments accumulate into m.
 template <class S, bool NT>
-__device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_t P, uint32_t e0, uint32_t lane,
+__device__ __forceinline__ void store_row(typename S::T* __restrict__ out, uint32_t e0, uint32_t lane,
                                           const float* g, uint32_t h, bool moments, RowMoments& m) {
   using T = typename S::T;
   if constexpr (sizeof(T) == 4) {
+    T* b = out + e0 + lane;
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
       const float v = S::encode(g[i], direction_random(h, i));
-      auto* b = plane_base(out, P, i, e0);
-      if constexpr (NT) __builtin_nontemporal_store(v, b + lane);
-      else b[lane] = v;
+      st<NT>(b + i * kRowRun, v);
       if (moments) add_moment(m, i, v);
     }
   } else {
-    uint16_t* o = reinterpret_cast<uint16_t*>(out);
+    uint16_t* b = reinterpret_cast<uint16_t*>(out) + e0 + lane;
     const uint16_t v0 = __builtin_bit_cast(uint16_t, S::encode(g[0], direction_random(h, 0)));
-    auto* b0 = plane_base(o, P, 0, e0);
-    if constexpr (NT) __builtin_nontemporal_store(v0, b0 + lane);
-    else b0[lane] = v0;
+    st<NT>(b, v0);
     if (moments) add_moment(m, 0, decode_bits<S>(v0));
 #pragma unroll
     for (int q = 0; q < kPairs; ++q) {
@@ -869,55 +966,62 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, size_
         add_moment(m, i, decode_bits<S>(v & 0xffffu));
         add_moment(m, i + 1, decode_bits<S>(v >> 16));
       }
-      auto* bi = plane_base(o, P, i, e0);
-      auto* bj = plane_base(o, P, i + 1, e0);
-      if constexpr (NT) {
-        __builtin_nontemporal_store((uint16_t)(v & 0xffffu), bi + lane);
-        __builtin_nontemporal_store((uint16_t)(v >> 16), bj + lane);
-      } else {
-        bi[lane] = (uint16_t)(v & 0xffffu);
-        bj[lane] = (uint16_t)(v >> 16);
-      }
+      st<NT>(b + i * kRowRun, (uint16_t)(v & 0xffffu));
+      st<NT>(b + (i + 1) * kRowRun, (uint16_t)(v >> 16));
     }
   }
 }
 
 // stream_collide_cell for a row wavefront: the whole wavefront gathers (gather_row), then fluid
 // lanes replace wall-link directions and collide, EQ and wall lanes take their fixed states,
-// and every lane stores. A wavefront of walls only skips the step.
+// and every lane stores. Every load that does not depend on another (the pulls, the flags, the
+// outlet's upstream moments) is issued before any of them is waited for, so a wavefront waits
+// for memory once; a wavefront of walls only stores their rest state again like any other.
 template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool BC>
 __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __restrict__ in,
                                                         typename S::T* __restrict__ out,
                                                         const uint32_t* __restrict__ geo,
                                                         float4* __restrict__ macro, const StepParams& p,
                                                         uint32_t n, uint32_t seed) {
-  const uint32_t flag = geo[n];
-  const uint32_t kind = flag & kTypeMask;
-  if (__all(is_wall(kind))) return;
   const uint32_t lane = threadIdx.x & 63u;
   RowWave w;
   w.n0 = __builtin_amdgcn_readfirstlane(n - lane);
   cell_coords(w.n0, p, w.x0, w.y, w.z);
   w.e0 = pop_x(p, w.x0) + pop_row(p, w.y + p.ny * w.z);
-  // The wavefront holding an EQ outlet cell (lane 63) and its upstream cell (lane 62).
-  const bool outlet_wave =
-      w.x0 + 64 == p.nx && (__builtin_amdgcn_readlane(kind, 63) == kEq);
   const uint32_t row = w.y + p.ny * w.z;
+  // The last wavefront of a row holds the outlet plane (lane 63) and its upstream cell (lane 62).
+  const bool last_wave = w.x0 + 64 == p.nx;
   RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
-  if (outlet_wave) up = row_moments(edge_rows(in, p)[row]);
+  if (last_wave) up = row_moments(edge_rows(in, p)[row]);
+  // The run's descriptor (a scalar load) tells plain runs, whose flags follow from the lane,
+  // from the few that touch a body and read every cell's flag.
+  const uint32_t desc = p.wave_desc ? p.wave_desc[w.n0 >> 6] : kWaveGeneral;
   float g[kQ];
   const uint32_t bc = BC ? p.bc : 0u;
   gather_row<S, POL == kNtAll>(in, p, bc, w, lane, g);
+  uint32_t flag;
+  if (desc & kWaveGeneral)
+    flag = geo[n];
+  else
+    flag = ((lane == 0 && (desc & kWaveEqFirst)) || (lane == 63 && (desc & kWaveEqLast))) ? (uint32_t)kEq : 0u;
+  const uint32_t kind = flag & kTypeMask;
+  const bool outlet_wave = last_wave && (__builtin_amdgcn_readlane(kind, 63) == kEq);
   if (BC && (p.bc & kBcFreeSlip) && on_side(w.y, w.z, p)) slip_row<S, POL == kNtAll>(in, p, w, lane, g);
   const uint32_t x = w.x0 + lane;
-  float4 m;
-  if (kind == kFluid) {
-    if ((flag >> 3) != 0) {
-      const Neighbours nb(x, w.y, w.z, p, bc);
+  if (kind == kFluid && (flag >> 3) != 0) {
+    const Neighbours nb(x, w.y, w.z, p, bc);
 #pragma unroll
-      for (int i = 1; i < kQ; ++i)
-        if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, w.e0 + lane, flag, i);
-    }
+    for (int i = 1; i < kQ; ++i)
+      if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, w.e0 + lane, flag, i);
+  }
+  float4 m;
+  if constexpr (OP == kOpReg) {
+    // Every lane through one collision: EQ and wall lanes with prescribed moments.
+    const bool prescribed = kind != kFluid;
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (kind == kEq) eq_velocity(p, x, up, ux, uy, uz);
+    m = relax_prescribed<OP, LES>(p, g, flag, n, prescribed, ux, uy, uz);
+  } else if (kind == kFluid) {
     m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
     m = eq_from(p, x, up, g);
@@ -926,7 +1030,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
   RowMoments stored{1.0f, 0.0f, 0.0f, 0.0f};
-  store_row<S, POL != kCached>(out, p.run, w.e0, lane, g, cell_random(n, seed), outlet_wave, stored);
+  store_row<S, POL != kCached>(out, w.e0, lane, g, cell_random(n, seed), outlet_wave, stored);
   if (outlet_wave && lane == 62) edge_rows(out, p)[row] = make_float4(stored.mass, stored.mx, stored.my, stored.mz);
   if (MACRO && !is_wall(kind)) {
     macro[n] = m;
