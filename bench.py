@@ -55,7 +55,30 @@ TAU0 = 3 * (U_IN * BODY_LENGTH / 320) + 0.5  # Re 320 on L: nu = 0.01, tau0 = 0.
 
 
 def bytes_per_cell(precision: str) -> int:
+    """The reference's convention (src/app/bench-page.ts:44-45): 19 populations read and
+    written plus a 4-byte flag per cell."""
     return 156 if precision == "f32" else 80
+
+
+def general_runs(flags: np.ndarray) -> int:
+    """64-cell runs of a row-path lattice whose flags the step must read: any run except those
+    of fluid cells without wall links, with at most an EQ cell first and last."""
+    runs = np.asarray(flags, np.uint32).reshape(-1, 64)
+    inner = runs[:, 1:63]
+    edge_ok = ((runs[:, 0] == 0) | (runs[:, 0] == 4)) & ((runs[:, 63] == 0) | (runs[:, 63] == 4))
+    return int(np.count_nonzero(~(edge_ok & np.all(inner == 0, axis=1))))
+
+
+def algorithmic_bytes(grid, precision: str) -> int:
+    """Bytes one step has to move on this grid: 19 populations read and 19 written per cell;
+    on a row-path lattice (nx a multiple of 64) one 4-byte descriptor per 64-cell run plus the
+    4-byte flags of the runs that touch a body, else every cell's flag."""
+    elem = 4 if precision == "f32" else 2
+    cells = grid.cells
+    pops = 2 * 19 * elem * cells
+    if grid.nx % 64:
+        return pops + 4 * cells
+    return pops + 4 * (cells // 64) + 4 * 64 * general_runs(grid.flags)
 
 
 def empty_grid(layout):
@@ -118,7 +141,9 @@ def time_solver(grid, params, steps, warmup, repeats=3, closure=None):
     cells = grid.cells
     return {"ms": ms, "wall_ms": wall, "launch_us": launch_ms * 1e3,
             "mlups": cells * steps / (ms * 1e3), "finite": finite, "variant": variant,
-            "gbps": bytes_per_cell(params.precision) * cells / (launch_ms * 1e6)}
+            "gbps": algorithmic_bytes(grid, params.precision) / (launch_ms * 1e6),
+            "bytes": algorithmic_bytes(grid, params.precision),
+            "gbps_reference_convention": bytes_per_cell(params.precision) * cells / (launch_ms * 1e6)}
 
 
 def measured_traffic(precision: str, variant: str):
@@ -304,6 +329,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(step16["gbps"], 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(step16["gbps"] / HBM_PEAK_GBPS, 4),
                      "traffic": measured_traffic("f16", step16["variant"]),
+                     "algorithmic_bytes": step16["bytes"],
+                     "frac_at_80_bytes_per_cell": round(step16["gbps_reference_convention"] / HBM_PEAK_GBPS, 4),
                      "kernel": "stream_collide_kernel, f16 storage", "kernel_us": round(step16["launch_us"], 2)},
         "f32": {"value": round(body["f32"]["mlups"], 1), "ms_per_step": round(body["f32"]["launch_us"] / 1e3, 5),
                 "achieved_gbps": round(body["f32"]["gbps"], 1),
