@@ -115,9 +115,12 @@ struct StoreF32 {
   using T = float;
   static constexpr int kBytes = 4;
   static constexpr int kShift = 2;
+  static constexpr int kScaleLog2 = 0;  // stored value = g * 2^kScaleLog2
   __host__ __device__ __forceinline__ static float decode(T v) { return v; }
   __host__ __device__ __forceinline__ static float load(const T* p) { return *p; }
   __host__ __device__ __forceinline__ static T encode(float g, uint32_t) { return g; }
+  // The same from a value already multiplied by 2^kScaleLog2.
+  __host__ __device__ __forceinline__ static T encode_scaled(float gs, uint32_t r) { return encode(gs, r); }
 };
 
 // f16 deviations scaled by 2^15 (prelude-f16.wgsl:5-22 fixes the format; the rounding's
@@ -126,22 +129,34 @@ struct StoreF16 {
   using T = _Float16;
   static constexpr int kBytes = 2;
   static constexpr int kShift = 1;
+  static constexpr int kScaleLog2 = 15;
   static constexpr float kScale = 32768.0f;
-  __host__ __device__ __forceinline__ static float decode(T v) { return static_cast<float>(v) * (1.0f / kScale); }
+  // An fma with a zero addend (one v_fma_mix_f32 from the half operand) rather than a conversion
+  // and a multiply; the scale is a power of two, so the result is the same exact product.
+  __host__ __device__ __forceinline__ static float decode(T v) {
+    return __builtin_fmaf(static_cast<float>(v), 1.0f / kScale, 0.0f);
+  }
   __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
   __host__ __device__ __forceinline__ static T encode(float g, uint32_t r) { return round_f16_sr(g * kScale, r); }
-  // Two values rounded into the low and high halves of one register.
-  __host__ __device__ __forceinline__ static uint32_t encode_pair(float a, uint32_t ra, float b, uint32_t rb) {
+  __host__ __device__ __forceinline__ static T encode_scaled(float gs, uint32_t r) { return round_f16_sr(gs, r); }
+  // Two already-scaled values rounded into the low and high halves of one register; `old`
+  // supplies the register (both halves are overwritten).
+  __host__ __device__ __forceinline__ static uint32_t encode_pair_scaled(uint32_t old, float as, uint32_t ra, float bs,
+                                                                         uint32_t rb) {
 #if defined(__HIP_DEVICE_COMPILE__)
     typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-    half2_t h = __builtin_nondeterministic_value(h);
-    h = __builtin_amdgcn_cvt_sr_f16_f32(h, a * kScale, ra, false);
-    h = __builtin_amdgcn_cvt_sr_f16_f32(h, b * kScale, rb, true);
+    half2_t h = __builtin_bit_cast(half2_t, old);
+    h = __builtin_amdgcn_cvt_sr_f16_f32(h, as, ra, false);
+    h = __builtin_amdgcn_cvt_sr_f16_f32(h, bs, rb, true);
     return __builtin_bit_cast(uint32_t, h);
 #else
-    return (uint32_t)__builtin_bit_cast(uint16_t, encode(a, ra)) |
-           ((uint32_t)__builtin_bit_cast(uint16_t, encode(b, rb)) << 16);
+    (void)old;
+    return (uint32_t)__builtin_bit_cast(uint16_t, encode_scaled(as, ra)) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, encode_scaled(bs, rb)) << 16);
 #endif
+  }
+  __host__ __device__ __forceinline__ static uint32_t encode_pair(float a, uint32_t ra, float b, uint32_t rb) {
+    return encode_pair_scaled(0u, a * kScale, ra, b * kScale, rb);
   }
 };
 
@@ -150,9 +165,24 @@ struct StoreBF16 {
   using T = uint16_t;
   static constexpr int kBytes = 2;
   static constexpr int kShift = 1;
+  static constexpr int kScaleLog2 = 0;
   __host__ __device__ __forceinline__ static float decode(T v) { return as_float(static_cast<uint32_t>(v) << 16); }
   __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
   __host__ __device__ __forceinline__ static T encode(float g, uint32_t r) { return round_bf16_sr(g, r); }
+  __host__ __device__ __forceinline__ static T encode_scaled(float gs, uint32_t r) { return encode(gs, r); }
+  __host__ __device__ __forceinline__ static uint32_t encode_pair_scaled(uint32_t old, float a, uint32_t ra, float b,
+                                                                         uint32_t rb) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef short short2_t __attribute__((ext_vector_type(2)));
+    short2_t h = __builtin_bit_cast(short2_t, old);
+    h = __builtin_amdgcn_cvt_sr_bf16_f32(h, a, ra, false);
+    h = __builtin_amdgcn_cvt_sr_bf16_f32(h, b, rb, true);
+    return __builtin_bit_cast(uint32_t, h);
+#else
+    (void)old;
+    return (uint32_t)encode(a, ra) | ((uint32_t)encode(b, rb) << 16);
+#endif
+  }
   __host__ __device__ __forceinline__ static uint32_t encode_pair(float a, uint32_t ra, float b, uint32_t rb) {
 #if defined(__HIP_DEVICE_COMPILE__)
     typedef short short2_t __attribute__((ext_vector_type(2)));

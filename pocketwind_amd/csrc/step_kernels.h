@@ -133,17 +133,24 @@ __host__ __device__ __forceinline__ void prescribe_moments(CellState& c, float u
   c.uz = uz;
 }
 
-// The pair-form equilibrium of the cell's moments.
+// The output scale 2^SC of a collision: its post-collision values leave multiplied by the
+// storage format's scale (2^15 for f16) so the stores convert them directly. A power of two
+// commutes exactly with every rounding here, so the stored bits are those of scaling after.
+template <int SC>
+constexpr float kOut = (float)(1u << SC);
+
+// The pair-form equilibrium of the cell's moments, scaled by 2^SC.
+template <int SC = 0>
 __host__ __device__ __forceinline__ void cell_equilibrium(CellState& c) {
   const float rd = c.rd;
   const float h = 1.5f * (c.ux * c.ux + c.uy * c.uy + c.uz * c.uz);
-  c.eq0 = wf(0) * (rd - c.rho * h);
+  c.eq0 = (wf(0) * kOut<SC>) * (rd - c.rho * h);
 #pragma unroll
   for (int p = 0; p < kPairs; ++p) {
     const int i = 2 * p + 1;
     const float cu = cdot(i, c.ux, c.uy, c.uz);
-    const float rw = wf(i) * c.rho;
-    c.a[p] = rw * (4.5f * cu * cu - h) + wf(i) * rd;
+    const float rw = (wf(i) * kOut<SC>) * c.rho;
+    c.a[p] = rw * (4.5f * cu * cu - h) + (wf(i) * kOut<SC>) * rd;
     c.b[p] = (3.0f * rw) * cu;
   }
 }
@@ -207,9 +214,12 @@ __host__ __device__ __forceinline__ float projected(int i, const Stress& s, floa
 // collide-trt.wgsl:1-15 and collide-reg.wgsl:3-18; writes the post-collision state into g.
 // With `prescribed` (REG only) the non-equilibrium stress is zero, so the cell leaves with its
 // equilibrium: the state of an EQ cell, or the rest state of a wall.
-template <int OP, bool LES>
+// c holds the equilibrium scaled by 2^SC (cell_equilibrium<SC>), and g receives the state at the
+// same scale; the TRT relaxation works on the gathered values and takes SC = 0.
+template <int OP, bool LES, int SC = 0>
 __host__ __device__ __forceinline__ void collide(const StepParams& p, float* g, const CellState& c,
                                                  bool prescribed = false) {
+  static_assert(OP == kOpReg || SC == 0, "the TRT relaxation works unscaled");
   Stress s{0, 0, 0, 0, 0, 0};
   if (LES || OP == kOpReg) s = neq_stress(c);
   if (OP == kOpReg && prescribed) s = Stress{0, 0, 0, 0, 0, 0};
@@ -218,11 +228,11 @@ __host__ __device__ __forceinline__ void collide(const StepParams& p, float* g, 
   if (OP == kOpReg) {
     const float third = (s.xx + s.yy + s.zz) * (1.0f / 3.0f);
     const float k = (1.0f - wp) * 4.5f;
-    g[0] = c.eq0 - k * wf(0) * third;
+    g[0] = c.eq0 - k * (wf(0) * kOut<SC>) * third;
 #pragma unroll
     for (int q = 0; q < kPairs; ++q) {
       const int i = 2 * q + 1;
-      const float t = c.a[q] + k * wf(i) * projected(i, s, third);
+      const float t = c.a[q] + k * (wf(i) * kOut<SC>) * projected(i, s, third);
       g[i] = t + c.b[q];
       g[i + 1] = t - c.b[q];
     }
@@ -251,7 +261,7 @@ __host__ __device__ __forceinline__ float friction_velocity(float nu, float spee
 
 // stream_collide.wgsl:18-80: prescribe the smallest post-collision stress whose bounce-back
 // flux along the slip is the wall-law shear; falls back to the bulk collision.
-template <int OP>
+template <int OP, int SC = 0>
 __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float* g, const CellState& c,
                                                      uint32_t flag, uint32_t n) {
   float nx = 0, ny = 0, nz = 0, exposed = 0;
@@ -275,7 +285,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
   }
   const float speed = sqrtf(sx * sx + sy * sy + sz * sz);
   if (speed < 1e-7f) {
-    collide<OP, true>(p, g, c);
+    collide<OP, true, SC>(p, g, c);
     return;
   }
   const float tx = sx / speed, ty = sy / speed, tz = sz / speed;
@@ -302,7 +312,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
     // The weight part of the flux, sum 2 along_j w_j = -2 n.t, is exactly zero once the slip
     // has had its normal component removed; evaluated in f32 it would only add the rounding
     // residual of t's normal component, amplified by 1 / |slip| when the slip is small.
-    const float eqj = eq_of(c, j);
+    const float eqj = eq_of(c, j) * (1.0f / kOut<SC>);
     flux += 2.0f * along * (projected_slip ? eqj : eqj + wf(j));
     const float s = 9.0f * wf(j) * along;
     b.xx += s * (x * x - 1.0f / 3.0f);
@@ -314,7 +324,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
   }
   const float norm = b.xx * b.xx + b.yy * b.yy + b.zz * b.zz + 2.0f * (b.xy * b.xy + b.xz * b.xz + b.yz * b.yz);
   if (norm < 0.05f) {
-    collide<OP, true>(p, g, c);
+    collide<OP, true, SC>(p, g, c);
     return;
   }
   const float scale = 4.5f * (shear - flux) / norm;
@@ -323,11 +333,11 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
   bool admissible = true;
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
-    post[i] = eq_of(c, i) + scale * wf(i) * projected(i, b, third);
-    admissible = admissible && (post[i] + wf(i) >= 0.0f);
+    post[i] = eq_of(c, i) + scale * (wf(i) * kOut<SC>) * projected(i, b, third);
+    admissible = admissible && (post[i] + wf(i) * kOut<SC> >= 0.0f);
   }
   if (!admissible) {
-    collide<OP, true>(p, g, c);
+    collide<OP, true, SC>(p, g, c);
     return;
   }
 #pragma unroll
@@ -453,14 +463,26 @@ __host__ __device__ __forceinline__ uint32_t pop_cell(const StepParams& p, uint3
   return pop_x(p, n - r * p.nx) + pop_row(p, r);
 }
 
+// The output scale exponent a step uses: the format's for the regularised collision (whose
+// equilibrium and reconstruction carry the scale in their weights), 0 for TRT.
+template <class S, int OP>
+constexpr int kOutLog2 = OP == kOpReg ? S::kScaleLog2 : 0;
+
 // Stores of one cell's 19 values; P is the direction stride (StepParams::run), `self` the
 // cell's byte offset within direction 0 and h its random word for this step.
-template <class S, bool NT = false>
+// SCALED: g holds the values already multiplied by the format's scale 2^S::kScaleLog2.
+template <class S, bool SCALED>
+__host__ __device__ __forceinline__ typename S::T encode_out(float v, uint32_t r) {
+  if constexpr (SCALED) return S::encode_scaled(v, r);
+  else return S::encode(v, r);
+}
+
+template <class S, bool NT = false, bool SCALED = false>
 __host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__ out, size_t P, uint32_t self,
                                                      const float* g, uint32_t h) {
 #pragma unroll
   for (int i = 0; i < kQ; ++i)
-    st<NT>(byte_offset(out + (size_t)i * P, self), S::encode(g[i], direction_random(h, i)));
+    st<NT>(byte_offset(out + (size_t)i * P, self), encode_out<S, SCALED>(g[i], direction_random(h, i)));
 }
 
 // common.wgsl:145-150 with rho - 1 = 0, in pair form: eq_i = w (4.5 cu^2 - 1.5 u^2) + 3 w cu.
@@ -597,17 +619,17 @@ __host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, 
 // lane (EQ or wall) takes unit density and the velocity (ux, uy, uz) as its moments and zero
 // non-equilibrium stress, so it leaves with that equilibrium (a wall with the rest state, exactly
 // zero) through the same arithmetic as every fluid lane instead of a separate branch.
-template <int OP, bool LES>
+template <int OP, bool LES, int SC = 0>
 __host__ __device__ __forceinline__ float4 relax_prescribed(const StepParams& p, float* g, uint32_t flag, uint32_t n,
                                                             bool prescribed, float ux, float uy, float uz) {
   CellState c;
   cell_moments(g, c);
   if (prescribed) prescribe_moments(c, ux, uy, uz);
-  cell_equilibrium(c);
+  cell_equilibrium<SC>(c);
   if (LES && (flag >> 3) != 0)
-    collide_wall<OP>(p, g, c, flag, n);
+    collide_wall<OP, SC>(p, g, c, flag, n);
   else
-    collide<OP, LES>(p, g, c, prescribed);
+    collide<OP, LES, SC>(p, g, c, prescribed);
   return make_float4(c.rho, c.ux, c.uy, c.uz);
 }
 
@@ -655,11 +677,11 @@ __host__ __device__ __forceinline__ const float4* edge_rows(const T* set, const 
 __host__ __device__ __forceinline__ RowMoments row_moments(const float4& e) { return RowMoments{e.x, e.y, e.z, e.w}; }
 
 // Moments of the values a store of g with the random word h writes.
-template <class S>
+template <class S, bool SCALED = false>
 __host__ __device__ __forceinline__ RowMoments encoded_moments(const float* g, uint32_t h) {
   RowMoments m{1.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) add_moment(m, i, S::decode(S::encode(g[i], direction_random(h, i))));
+  for (int i = 0; i < kQ; ++i) add_moment(m, i, S::decode(encode_out<S, SCALED>(g[i], direction_random(h, i))));
   return m;
 }
 
@@ -810,7 +832,7 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
       }
       eq_velocity(p, x, up, ux, uy, uz);
     }
-    m = relax_prescribed<OP, LES>(p, g, flag, n, kind != kFluid, ux, uy, uz);
+    m = relax_prescribed<OP, LES, kOutLog2<S, OP>>(p, g, flag, n, kind != kFluid, ux, uy, uz);
   } else if (kind == kEq) {
     if constexpr (EDGE) {
       RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
@@ -824,10 +846,10 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
   const uint32_t h = cell_random(n, seed);
-  store_state<S, POL != kCached>(out, P, self, g, h);
+  store_state<S, POL != kCached, (kOutLog2<S, OP> != 0)>(out, P, self, g, h);
   if constexpr (EDGE) {
     if (x + 2 == p.nx) {
-      const RowMoments e = encoded_moments<S>(g, h);
+      const RowMoments e = encoded_moments<S, (kOutLog2<S, OP> != 0)>(g, h);
       edge_rows(out, p)[y + p.ny * z] = make_float4(e.mass, e.mx, e.my, e.mz);
     }
   }
@@ -941,7 +963,7 @@ __device__ __forceinline__ void slip_row(const typename S::T* __restrict__ in, c
 // Stores of a row wavefront: run i of the row at e0 (the wavefront's first element), lane
 // `lane`. 16-bit formats round two directions into one register and store its halves.
 // With `moments` (wavefront-uniform) the stored values' moments accumulate into m.
-template <class S, bool NT>
+template <class S, bool NT, bool SCALED = false>
 __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, uint32_t e0, uint32_t lane,
                                           const float* g, uint32_t h, bool moments, RowMoments& m) {
   using T = typename S::T;
@@ -949,25 +971,33 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, uint3
     T* b = out + e0 + lane;
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
-      const float v = S::encode(g[i], direction_random(h, i));
+      const float v = encode_out<S, SCALED>(g[i], direction_random(h, i));
       st<NT>(b + i * kRowRun, v);
       if (moments) add_moment(m, i, v);
     }
   } else {
+    // Every store first; the outlet wavefront's moments afterwards from the packed words.
     uint16_t* b = reinterpret_cast<uint16_t*>(out) + e0 + lane;
-    const uint16_t v0 = __builtin_bit_cast(uint16_t, S::encode(g[0], direction_random(h, 0)));
+    const uint16_t v0 = __builtin_bit_cast(uint16_t, encode_out<S, SCALED>(g[0], direction_random(h, 0)));
     st<NT>(b, v0);
-    if (moments) add_moment(m, 0, decode_bits<S>(v0));
+    uint32_t words[kPairs];
 #pragma unroll
     for (int q = 0; q < kPairs; ++q) {
       const int i = 2 * q + 1;
-      const uint32_t v = S::encode_pair(g[i], direction_random(h, i), g[i + 1], direction_random(h, i + 1));
-      if (moments) {
-        add_moment(m, i, decode_bits<S>(v & 0xffffu));
-        add_moment(m, i + 1, decode_bits<S>(v >> 16));
+      const float scale = SCALED ? 1.0f : kOut<S::kScaleLog2>;
+      // The pair's first random word is dead after its conversion, so its register takes the pair.
+      const uint32_t ra = direction_random(h, i);
+      words[q] = S::encode_pair_scaled(ra, g[i] * scale, ra, g[i + 1] * scale, direction_random(h, i + 1));
+      st<NT>(b + i * kRowRun, (uint16_t)(words[q] & 0xffffu));
+      st<NT>(b + (i + 1) * kRowRun, (uint16_t)(words[q] >> 16));
+    }
+    if (moments) {
+      add_moment(m, 0, decode_bits<S>(v0));
+#pragma unroll
+      for (int q = 0; q < kPairs; ++q) {
+        add_moment(m, 2 * q + 1, decode_bits<S>(words[q] & 0xffffu));
+        add_moment(m, 2 * q + 2, decode_bits<S>(words[q] >> 16));
       }
-      st<NT>(b + i * kRowRun, (uint16_t)(v & 0xffffu));
-      st<NT>(b + (i + 1) * kRowRun, (uint16_t)(v >> 16));
     }
   }
 }
@@ -1020,7 +1050,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     const bool prescribed = kind != kFluid;
     float ux = 0.0f, uy = 0.0f, uz = 0.0f;
     if (kind == kEq) eq_velocity(p, x, up, ux, uy, uz);
-    m = relax_prescribed<OP, LES>(p, g, flag, n, prescribed, ux, uy, uz);
+    m = relax_prescribed<OP, LES, kOutLog2<S, OP>>(p, g, flag, n, prescribed, ux, uy, uz);
   } else if (kind == kFluid) {
     m = relax<OP, LES>(p, g, flag, n);
   } else if (kind == kEq) {
@@ -1030,7 +1060,8 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
   RowMoments stored{1.0f, 0.0f, 0.0f, 0.0f};
-  store_row<S, POL != kCached>(out, w.e0, lane, g, cell_random(n, seed), outlet_wave, stored);
+  store_row<S, POL != kCached, (kOutLog2<S, OP> != 0)>(out, w.e0, lane, g, cell_random(n, seed), outlet_wave,
+                                                          stored);
   if (outlet_wave && lane == 62) edge_rows(out, p)[row] = make_float4(stored.mass, stored.mx, stored.my, stored.mz);
   if (MACRO && !is_wall(kind)) {
     macro[n] = m;
