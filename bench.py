@@ -60,25 +60,10 @@ def bytes_per_cell(precision: str) -> int:
     return 156 if precision == "f32" else 80
 
 
-def general_runs(flags: np.ndarray) -> int:
-    """64-cell runs of a row-path lattice whose flags the step must read: any run except those
-    of fluid cells without wall links, with at most an EQ cell first and last."""
-    runs = np.asarray(flags, np.uint32).reshape(-1, 64)
-    inner = runs[:, 1:63]
-    edge_ok = ((runs[:, 0] == 0) | (runs[:, 0] == 4)) & ((runs[:, 63] == 0) | (runs[:, 63] == 4))
-    return int(np.count_nonzero(~(edge_ok & np.all(inner == 0, axis=1))))
-
-
 def algorithmic_bytes(grid, precision: str) -> int:
-    """Bytes one step has to move on this grid: 19 populations read and 19 written per cell;
-    on a row-path lattice (nx a multiple of 64) one 4-byte descriptor per 64-cell run plus the
-    4-byte flags of the runs that touch a body, else every cell's flag."""
-    elem = 4 if precision == "f32" else 2
-    cells = grid.cells
-    pops = 2 * 19 * elem * cells
-    if grid.nx % 64:
-        return pops + 4 * cells
-    return pops + 4 * (cells // 64) + 4 * 64 * general_runs(grid.flags)
+    """Bytes one step has to move on this grid (pocketwind_amd.lattice.step_bytes)."""
+    from pocketwind_amd.lattice import step_bytes
+    return step_bytes(grid, precision)
 
 
 def empty_grid(layout):

@@ -312,6 +312,10 @@ pw_status pw_batch_create(uint32_t count, const pw_layout *layouts, const uint32
                           const size_t *n_boundary, const double *tau0, const double *u_in,
                           const pw_params *shared, pw_batch **out);
 pw_status pw_batch_step(pw_batch *b, uint32_t count, int write_macro_on_last);
+/* The step variant the batch chose at create (timing each cache policy with 1 and 16 chunks
+ * per XCD, or PW_BATCH_KERNEL=<cell|cell-nt|cell-nt-stores>/<chunks>): a pw_kernel value, and
+ * the chunks per XCD in *xcd_chunks (may be NULL). */
+uint32_t pw_batch_step_variant(const pw_batch *b, uint32_t *xcd_chunks);
 /* As pw_solver_step_sampled for every tunnel: `out` receives [tunnel][sample] x
  * (step, fx, fy, fz), 4 doubles each. */
 pw_status pw_batch_step_sampled(pw_batch *b, uint32_t count, uint32_t every, double *out,

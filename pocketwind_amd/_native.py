@@ -103,6 +103,7 @@ SIGNATURES = {
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(Params),
                                   C.POINTER(C.c_void_p)]),
     "pw_batch_step": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int]),
+    "pw_batch_step_variant": (C.c_uint32, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pw_batch_step_sampled": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t]),
     "pw_batch_read_macro": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t]),
     "pw_batch_read_populations": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t]),

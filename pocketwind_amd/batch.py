@@ -66,6 +66,13 @@ class TunnelBatch:
         N.check(N.lib().pw_batch_read_macro(self._h, int(tunnel), out.ctypes.data, out.size))
         return out
 
+    @property
+    def variant(self) -> str:
+        """The step variant chosen at create: cache policy / chunks per XCD."""
+        chunks = C.c_uint32()
+        code = int(N.lib().pw_batch_step_variant(self._h, C.byref(chunks)))
+        return f"{ {1: 'cell', 2: 'cell-nt', 3: 'cell-nt-stores'}.get(code, '?') }/xcd-chunks-{chunks.value}"
+
     def read_populations(self, tunnel: int) -> np.ndarray:
         g = self.grids[tunnel]
         dt = {"f32": np.float32, "f16": np.float16, "bf16": np.uint16}[self.params.precision]

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -117,6 +118,10 @@ struct pw_batch {
   uint32_t* tile_tunnel = nullptr;
   uint32_t* tile_first = nullptr;
   uint32_t n_tiles = 0;
+  uint32_t policy = kNtStores;  // step variant (tuned at create)
+  uint32_t xcd_chunks = 16;
+  uint32_t chunk = 1;           // tiles per XCD chunk of the step grid
+  unsigned step_grid = 0;
   uint32_t* part_tunnel = nullptr;
   uint32_t n_partials = 0;
   double* partials = nullptr;
@@ -140,7 +145,8 @@ struct pw_batch {
 namespace {
 
 void launch_step(pw_batch* b, int parity, bool macro) {
-  const BatchLaunch L{b->grid, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, b->seeds, b->params};
+  const BatchLaunch L{b->step_grid, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, b->seeds,
+                      b->params, b->policy, b->chunk};
   switch (b->params.storage) {
     case PW_STORAGE_F16: batch_step_s<StoreF16>(L, parity, macro); break;
     case PW_STORAGE_BF16: batch_step_s<StoreBF16>(L, parity, macro); break;
@@ -221,6 +227,71 @@ hipError_t capture_pair(pw_batch* b, int first_parity, hipGraphExec_t* exec) {
 }
 
 pw_status step_plain(pw_batch* b, uint32_t count, int write_macro_on_last);
+
+bool forced_batch_variant(pw_batch* b, const char* text) {
+  if (!text) return false;
+  const std::string v(text);
+  const size_t a = v.find('/');
+  const std::string name = v.substr(0, a);
+  uint32_t policy;
+  if (name == "cell") policy = kCached;
+  else if (name == "cell-nt") policy = kNtAll;
+  else if (name == "cell-nt-stores") policy = kNtStores;
+  else return false;
+  uint32_t chunks = b->xcd_chunks;
+  if (a != std::string::npos) chunks = (uint32_t)std::strtoul(v.c_str() + a + 1, nullptr, 10);
+  if (chunks != 1 && chunks != 16) return false;
+  b->policy = policy;
+  b->xcd_chunks = chunks;
+  return true;
+}
+
+// Times the three cache policies with 1 and 16 chunks per XCD (two runs of a few steps each,
+// the faster counting) and keeps the fastest.
+hipError_t tune_batch(pw_batch* b) {
+  const uint32_t policies[3] = {kCached, kNtAll, kNtStores};
+  const uint32_t chunkings[2] = {1, 16};
+  const uint64_t cells = [&] {
+    uint64_t n = 0;
+    for (const auto& t : b->host) n += t.p.cells;
+    return n;
+  }();
+  const int reps = cells >= (1u << 22) ? 4 : 16;
+  float best = 0;
+  uint32_t best_policy = kNtStores, best_chunks = 16;
+  for (uint32_t chunks : chunkings)
+    for (uint32_t pol : policies) {
+      b->policy = pol;
+      b->xcd_chunks = chunks;
+      b->step_grid = batch_grid(b->n_tiles, chunks, &b->chunk);
+      for (int w = 0; w < 2; ++w) {
+        launch_step(b, b->parity, false);
+        b->parity ^= 1;
+      }
+      float fastest = 0;
+      for (int round = 0; round < 2; ++round) {
+        hipError_t e = hipEventRecord(b->t0, b->stream);
+        if (e != hipSuccess) return e;
+        for (int r = 0; r < reps; ++r) {
+          launch_step(b, b->parity, false);
+          b->parity ^= 1;
+        }
+        if ((e = hipEventRecord(b->t1, b->stream)) != hipSuccess) return e;
+        if ((e = hipEventSynchronize(b->t1)) != hipSuccess) return e;
+        float ms = 0;
+        if ((e = hipEventElapsedTime(&ms, b->t0, b->t1)) != hipSuccess) return e;
+        if (round == 0 || ms < fastest) fastest = ms;
+      }
+      if (best == 0 || fastest < best) {
+        best = fastest;
+        best_policy = pol;
+        best_chunks = chunks;
+      }
+    }
+  b->policy = best_policy;
+  b->xcd_chunks = best_chunks;
+  return hipGetLastError();
+}
 
 }  // namespace
 
@@ -366,10 +437,26 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
       (e = dev_alloc(b, reinterpret_cast<char**>(&b->closure_net), mlp_packed_bytes())) != hipSuccess)
     return fail(e, "batch closure network");
   launch_init(b);
+  // PW_BATCH_KERNEL=<cell|cell-nt|cell-nt-stores>/<chunks per XCD> fixes the step variant;
+  // otherwise each is timed on these tunnels and the fastest kept. Timing leaves the state
+  // changed, so the tunnels are initialised again afterwards.
+  if (!forced_batch_variant(b, std::getenv("PW_BATCH_KERNEL")) && (e = tune_batch(b)) != hipSuccess)
+    return fail(e, "batch tuning");
+  b->step_grid = batch_grid(b->n_tiles, b->xcd_chunks, &b->chunk);
+  launch_init(b);
+  if ((e = hipMemcpyAsync(b->seeds, seeds0, sizeof seeds0, hipMemcpyHostToDevice, b->stream)) != hipSuccess)
+    return fail(e, "batch seeds");
+  b->parity = 0;
   if ((e = hipGetLastError()) != hipSuccess) return fail(e, "batch init");
   if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail(e, "batch init");
   *out = b;
   return PW_OK;
+}
+
+uint32_t pw_batch_step_variant(const pw_batch* b, uint32_t* xcd_chunks) {
+  if (!b) return 0;
+  if (xcd_chunks) *xcd_chunks = b->xcd_chunks;
+  return b->policy == kNtAll ? PW_KERNEL_CELL_NT : b->policy == kNtStores ? PW_KERNEL_CELL_NT_STORES : PW_KERNEL_CELL;
 }
 
 pw_status pw_batch_step(pw_batch* b, uint32_t count, int write_macro_on_last) {

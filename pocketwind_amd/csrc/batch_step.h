@@ -31,6 +31,8 @@ struct BatchLaunch {
   uint32_t n_tiles;
   uint32_t* seeds;  // device: seeds[parity] is the step count a launch of that parity runs
   pw_params params;
+  uint32_t policy;  // population cache policy (kCached, kNtAll, kNtStores)
+  uint32_t chunk;   // consecutive tiles one XCD takes (the grid holds a multiple of 8 chunks)
 };
 
 // One step of every tunnel. The rounding seed is the step count, read on the device so a
@@ -39,31 +41,45 @@ struct BatchLaunch {
 // A tunnel whose nx is a multiple of 64 steps its cells as row wavefronts (the single solver's
 // fast path; its sets carry the upstream-outlet moments after the 19 N elements), others per
 // cell. Either way a tile is one block of one tunnel.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO>
+// One block per tile. Blocks are renumbered as in the single solver's step so that each XCD
+// takes runs of `chunk` consecutive tiles (neighbouring rows of one tunnel share its L2).
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWaves<OP, LES>))) void batch_step_kernel(const Tunnel* __restrict__ tunnels,
                                                             const uint32_t* __restrict__ tile_tunnel,
                                                             const uint32_t* __restrict__ tile_first,
-                                                            uint32_t n_tiles, int parity, uint32_t* seeds) {
+                                                            uint32_t n_tiles, int parity, uint32_t* seeds,
+                                                            uint32_t chunk) {
   const uint32_t seed = seeds[parity];
   if (blockIdx.x == 0 && threadIdx.x == 0) seeds[parity ^ 1] = seed + 1;
-  for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-    const uint32_t t = tile_tunnel[tile];
-    const Tunnel& tu = tunnels[t];
-    const uint32_t n = (tile - tile_first[t]) * kBlock + threadIdx.x;
-    if (n >= tu.p.cells) continue;
-    const auto* in = (const typename S::T*)tu.pops[parity];
-    auto* out = (typename S::T*)tu.pops[parity ^ 1];
-    if (tu.p.nx % 64 == 0)
-      stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, kNtAll, false>(in, out, tu.geo, tu.macro, tu.p, n, seed);
-    else
-      stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, kNtAll>(in, out, tu.geo, tu.macro, tu.p, n, seed);
-  }
+  const uint32_t b = blockIdx.x, local = b / kXcds;
+  const uint32_t tile = ((local / chunk) * kXcds + b % kXcds) * chunk + local % chunk;
+  if (tile >= n_tiles) return;
+  const uint32_t t = tile_tunnel[tile];
+  const Tunnel& tu = tunnels[t];
+  const uint32_t n = (tile - tile_first[t]) * kBlock + threadIdx.x;
+  if (n >= tu.p.cells) return;
+  const auto* in = (const typename S::T*)tu.pops[parity];
+  auto* out = (typename S::T*)tu.pops[parity ^ 1];
+  if (tu.p.rows)
+    stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL, false>(in, out, tu.geo, tu.macro, tu.p, n, seed);
+  else
+    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL>(in, out, tu.geo, tu.macro, tu.p, n, seed);
+}
+
+// Grid of a batched step: the tiles rounded up to whole chunks on each of the 8 XCDs.
+inline unsigned batch_grid(uint32_t n_tiles, uint32_t chunks_per_xcd, uint32_t* chunk) {
+  const uint32_t per_xcd = ((n_tiles + kXcds - 1) / kXcds + chunks_per_xcd - 1) / chunks_per_xcd * chunks_per_xcd;
+  *chunk = per_xcd / chunks_per_xcd;
+  return kXcds * per_xcd;
 }
 
 template <class S, int OP, bool LES, bool BZ, bool MACRO>
 void batch_step_t(const BatchLaunch& L, int parity) {
-  hipLaunchKernelGGL((batch_step_kernel<S, OP, LES, BZ, MACRO>), dim3(L.grid), dim3(kBlock), 0, L.stream,
-                     L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles, parity, L.seeds);
+  auto k = L.policy == kNtAll ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtAll>
+           : L.policy == kNtStores ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtStores>
+                                   : batch_step_kernel<S, OP, LES, BZ, MACRO, kCached>;
+  hipLaunchKernelGGL(k, dim3(L.grid), dim3(kBlock), 0, L.stream, L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles,
+                     parity, L.seeds, L.chunk);
 }
 
 template <class S, bool MACRO>

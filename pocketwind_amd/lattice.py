@@ -77,3 +77,25 @@ class VoxelGrid:
     @property
     def fractions(self) -> np.ndarray:
         return self.geometry[2 * self.cells:].view(np.float32)
+
+
+def general_runs(flags) -> int:
+    """64-cell runs of a row-path lattice (nx a multiple of 64) whose cell flags a step reads:
+    every run except those of fluid cells without wall links, with at most an EQ cell first and
+    last (the step's run descriptors cover those)."""
+    runs = np.asarray(flags, np.uint32).reshape(-1, 64)
+    edge_ok = ((runs[:, 0] == 0) | (runs[:, 0] == EQ)) & ((runs[:, 63] == 0) | (runs[:, 63] == EQ))
+    return int(np.count_nonzero(~(edge_ok & np.all(runs[:, 1:63] == 0, axis=1))))
+
+
+def step_bytes(grid, precision: str) -> int:
+    """Bytes one step has to move on `grid`: 19 populations read and 19 written per cell; on a
+    row-path lattice one 4-byte descriptor per 64-cell run plus the 4-byte flags of the runs that
+    touch a body, else every cell's flag. The reference counts a flag for every cell
+    (src/app/bench-page.ts:44-45: 156 B per cell at f32, 80 B at f16)."""
+    elem = 4 if precision == "f32" else 2
+    cells = grid.nx * grid.ny * grid.nz
+    pops = 2 * 19 * elem * cells
+    if grid.nx % 64:
+        return pops + 4 * cells
+    return pops + 4 * (cells // 64) + 4 * 64 * general_runs(grid.flags)

@@ -27,12 +27,12 @@ import numpy as np
 from . import bodies
 from .batch import TunnelBatch
 from .gpu_solver import FORCE_EVERY_STEPS, SolverParams
+from .lattice import step_bytes
 from .particles import Particles
 from .voxelize import voxelize
 
 DEFAULT_YAWS = (0, 2, 4, 6, 8, 10, 12, 14)  # degrees
 DEFAULT_CLOSURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "closure_c2.npz")
-BYTES_PER_CELL_F16 = 80  # 19 x 2 B read + 19 x 2 B written + the 4 B flag
 
 
 def yawed(body: np.ndarray, yaw_deg: float) -> np.ndarray:
@@ -99,6 +99,7 @@ def run(layout=(512, 256, 256), yaws: Sequence[float] = DEFAULT_YAWS, length_cel
                 parts.append(Particles.create_in_batch(batch, k, (lo, hi), per_tunnel[k], particle_seed + k,
                                                        max_count=1 << 24))
         cells = batch.cells
+        step_bytes_all = sum(step_bytes(g, "f16") for g in grids)
         # Step throughput of the batched launch alone (graph-replayed pairs, no macro writes).
         batch.step(20)
         step_ms = min(batch.time_steps(100) for _ in range(3)) / 100
@@ -150,8 +151,10 @@ def run(layout=(512, 256, 256), yaws: Sequence[float] = DEFAULT_YAWS, length_cel
             "steps": total, "average_steps": window, "force_samples": len(samples),
             "voxelise_s": round(t_vox, 2),
             "batched_step_us": round(step_ms * 1e3, 2), "batched_step_mlups": round(mlups_step, 1),
-            "batched_step_gbps": round(BYTES_PER_CELL_F16 * cells / step_ms / 1e6, 1),
-            "hbm_roofline_frac": round(BYTES_PER_CELL_F16 * cells / step_ms / 1e6 / 8000.0, 4),
+            "batched_step_variant": batch.variant,
+            "batched_step_bytes": step_bytes_all,
+            "batched_step_gbps": round(step_bytes_all / step_ms / 1e6, 1),
+            "hbm_roofline_frac": round(step_bytes_all / step_ms / 1e6 / 8000.0, 4),
             "pipeline_s": round(loop_s, 2), "pipeline_mlups": round(cells * total / loop_s / 1e6, 1),
             "particles": int(sum(per_tunnel)),
             "particle_updates_per_s": round(sum(per_tunnel) * 100 / particle_s, 1),
