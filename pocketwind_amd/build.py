@@ -33,6 +33,32 @@ def _compile(src: str, obj: str) -> tuple[str, subprocess.CompletedProcess]:
     return src, subprocess.run(cmd, capture_output=True, text=True)
 
 
+NAPI_DIR = os.path.join(HERE, "napi")
+NAPI_ADDON = os.path.join(NAPI_DIR, "pw_napi.node")
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_napi(verbose: bool = False) -> str | None:
+    """Compile the Node-API addon (plain C, gcc) against the built library, with an rpath to
+    it; skipped, returning None, where the Node headers are absent."""
+    if not os.path.isfile(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print("node_api.h not found; Node-API addon not built", flush=True)
+        return None
+    cmd = ["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-Werror", "-shared", "-fPIC",
+           "-DNAPI_VERSION=8", "-I", NODE_INCLUDE, "-I", os.path.join(HERE, "..", "include"),
+           "-o", NAPI_ADDON + ".tmp", os.path.join(NAPI_DIR, "pw_napi.c"),
+           "-L", HERE, "-lpocketwind", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("gcc failed to build pw_napi.node")
+    os.replace(NAPI_ADDON + ".tmp", NAPI_ADDON)
+    if verbose:
+        print(f"built {NAPI_ADDON}", flush=True)
+    return NAPI_ADDON
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> str:
     """Compile every csrc/*.hip to an object in parallel, then link the shared library."""
     srcs = sources()
@@ -58,6 +84,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     os.replace(LIB + ".tmp", LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources", flush=True)
+    build_napi(verbose)
     return LIB
 
 
