@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-#define PW_ABI_VERSION 6
+#define PW_ABI_VERSION 7
 
 /* Errors (src/lbm/gpu-resources.ts:3-27, gpu-force.ts:192-205). EINVAL is the reference's
  * RangeError, ENOMEM its AllocationFailure, EBUSY its 'busy' readback. */
@@ -206,6 +206,15 @@ pw_status pw_solver_set_step_variant(pw_solver *s, uint32_t kernel, uint32_t blo
  * two paths run the same collision code; comparing them pins the row path to the per-cell
  * path. No reference counterpart. */
 pw_status pw_debug_cell_path(pw_solver *s, int on);
+/* Rounding of f16 population stores. HARDWARE (the default): the gfx950 stochastic conversion
+ * with this library's per-cell, per-step random words. REFERENCE: encode_population of
+ * src/lbm/shaders/prelude-f16.wgsl:7-22, the random word lowbias32 of the value's f32 bits, the
+ * cell and the direction, so that stored bits can be compared with the reference's for equal f32
+ * values (about 20 integer operations per store instead of one). Setting it resets the solver
+ * (both sets to the initial state under the new rounding, step 0). PW_EINVAL for other storage
+ * formats. No reference counterpart as an option: the reference has only its own rule. */
+typedef enum { PW_ROUNDING_HARDWARE = 0, PW_ROUNDING_REFERENCE = 1 } pw_rounding;
+pw_status pw_solver_set_rounding(pw_solver *s, uint32_t rounding);
 /* Threads per block of the step launches. */
 uint32_t pw_solver_block_size(const pw_solver *s);
 /* Chunks of the lattice each of the 8 XCDs steps (1: one contiguous eighth each; 16: chunks

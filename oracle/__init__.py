@@ -13,7 +13,12 @@ Reference correspondence (file:line, relative to the pocketwind repository):
   preset_mesh       src/geometry/presets.ts:179-184
   place_mesh        src/geometry/transform.ts:118-143
   particle_*        src/render/particle-layout.ts:14-112, shaders/particles_update.wgsl
-  encode_f16        src/lbm/shaders/prelude-f16.wgsl:7-22
+  encode_f16        src/lbm/shaders/prelude-f16.wgsl:7-22, common.wgsl:134-135
+                    (pinned by tests/test_oracle_f16_encoding.py: exact halves pass, subnormals
+                    round to nearest even as numpy's float16 cast, normal values land on one of
+                    their two neighbours with the dropped fraction's probability; the reference's
+                    tests hold no known encodings, so the hash's bit mapping follows the cited
+                    constants without an external known answer)
 """
 from __future__ import annotations
 
@@ -131,6 +136,8 @@ def _declare(L):
                                        C.c_uint32, C.c_uint32, C.c_float, _f, _f]
     L.pwo_encode_f16.argtypes = [C.c_float, C.c_uint32, C.c_uint32]
     L.pwo_encode_f16.restype = C.c_uint16
+    L.pwo_encode_f16_planes.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
+    L.pwo_encode_f16_planes.restype = None
     L.pwo_decode_f16.argtypes = [C.c_uint16]
     L.pwo_decode_f16.restype = C.c_float
     L.pwo_encode_bf16.argtypes = [C.c_float, C.c_uint32, C.c_uint32]
@@ -396,6 +403,15 @@ def particles_update(particles, macro, flags, layout, seed, steps, cross_section
 # ---------------------------------------------------------------- storage encodings
 def encode_f16(value_scaled: float, i: int, n: int) -> int:
     return lib().pwo_encode_f16(value_scaled, i, n)
+
+
+def encode_f16_planes(g: np.ndarray) -> np.ndarray:
+    """Every deviation of a (19, cells) f32 state stored as the reference stores it
+    (common.wgsl:134-135): encode_f16(g * 2^15, i, n)."""
+    g = np.ascontiguousarray(g, np.float32).reshape(19, -1)
+    out = np.empty(g.shape, np.uint16)
+    lib().pwo_encode_f16_planes(g.ctypes.data, g.shape[1], out.ctypes.data)
+    return out
 
 
 def decode_f16(bits: int) -> float:

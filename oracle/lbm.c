@@ -605,6 +605,13 @@ uint16_t pwo_encode_f16(float value, uint32_t i, uint32_t n) {
   return f32_to_f16_rne(u2f((bits & 0x80000000u) | rounded));
 }
 
+/* common.wgsl:134-135 over a whole state: the planes of 19 x cells deviations g (direction
+ * major, the reference's cell order), each stored as encode_population(g * POP_SCALE, i, n). */
+void pwo_encode_f16_planes(const float *g, size_t cells, uint16_t *out) {
+  for (uint32_t i = 0; i < 19; ++i)
+    for (size_t n = 0; n < cells; ++n) out[i * cells + n] = pwo_encode_f16(g[i * cells + n] * 32768.0f, i, (uint32_t)n);
+}
+
 /* bfloat16 deviation storage: the same hash decides the 16 dropped bits (build-only format). */
 uint16_t pwo_encode_bf16(float value, uint32_t i, uint32_t n) {
   const uint32_t bits = f2u(value);

@@ -127,6 +127,7 @@ void pwo_particles_update(float *particles, uint32_t count, const float *macro,
 
 /* ---- lowbias32 stochastic f16 / bf16 storage (src/lbm/shaders/prelude-f16.wgsl:7-22) ---- */
 uint16_t pwo_encode_f16(float value_scaled, uint32_t i, uint32_t n);
+void pwo_encode_f16_planes(const float *g, size_t cells, uint16_t *out);
 float pwo_decode_f16(uint16_t bits);
 uint16_t pwo_encode_bf16(float value, uint32_t i, uint32_t n);
 float pwo_decode_bf16(uint16_t bits);

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(const Tunnel* __rest
     if (n >= tu.p.cells) continue;
     const uint32_t kind = tu.geo[n] & kTypeMask;
     const float ux = is_wall(kind) ? 0.0f : tu.p.u_in;
-    store_equilibrium<S>((typename S::T*)tu.pops[set], tu.p, n, ux, 0.0f, 0.0f, cell_random(n, kInitSeed));
+    store_equilibrium<S>((typename S::T*)tu.pops[set], tu.p, n, ux, 0.0f, 0.0f, S::cell_word(n, kInitSeed));
     tu.macro[n] = make_float4(1.0f, ux, 0.0f, 0.0f);
   }
 }

@@ -109,9 +109,16 @@ __host__ __device__ __forceinline__ uint16_t round_bf16_sr(float v, uint32_t r) 
 #endif
 }
 
+// The random words of a codec's stores: cell_word(n, seed) once per cell and step, then
+// dir_word(h, i) per direction.
+struct HashedWords {
+  __host__ __device__ __forceinline__ static uint32_t cell_word(uint32_t n, uint32_t seed) { return cell_random(n, seed); }
+  __host__ __device__ __forceinline__ static uint32_t dir_word(uint32_t h, int i) { return direction_random(h, i); }
+};
+
 // Storage codecs: decode() returns the f32 deviation g = f - w; encode() takes g and the
 // store's random word.
-struct StoreF32 {
+struct StoreF32 : HashedWords {
   using T = float;
   static constexpr int kBytes = 4;
   static constexpr int kShift = 2;
@@ -125,7 +132,7 @@ struct StoreF32 {
 
 // f16 deviations scaled by 2^15 (prelude-f16.wgsl:5-22 fixes the format; the rounding's
 // random source is this library's own).
-struct StoreF16 {
+struct StoreF16 : HashedWords {
   using T = _Float16;
   static constexpr int kBytes = 2;
   static constexpr int kShift = 1;
@@ -160,8 +167,38 @@ struct StoreF16 {
   }
 };
 
+// The f16 format with the reference's rounding (prelude-f16.wgsl:7-22): the random word of a
+// store is lowbias32 of the value's f32 bits, the cell index n and the direction i, and the 13
+// dropped bits round up when the word's low 13 bits are below them; values under the smallest
+// normal half convert to nearest. Written out in integer operations (about 20 per value, where
+// the hardware rounding takes one); selected per solver to compare stored bits with the reference.
+__host__ __device__ __forceinline__ _Float16 round_f16_reference(float v, uint32_t cell_dir) {
+  const uint32_t bits = as_uint(v), mag = bits & 0x7fffffffu;
+  if (mag < 0x38800000u) return static_cast<_Float16>(v);
+  const uint32_t random = hash32(bits ^ cell_dir);
+  const uint32_t rounded = (mag & 0xffffe000u) + ((random & 0x1fffu) < (mag & 0x1fffu) ? 0x2000u : 0u);
+  return static_cast<_Float16>(as_float((bits & 0x80000000u) | rounded));
+}
+
+struct StoreF16Reference : StoreF16 {
+  // The words carry the cell and direction only (prelude-f16.wgsl:10); the value's bits join at
+  // the rounding, so the seed (the step) takes no part.
+  __host__ __device__ __forceinline__ static uint32_t cell_word(uint32_t n, uint32_t) { return n * 0x9e3779b9u; }
+  __host__ __device__ __forceinline__ static uint32_t dir_word(uint32_t h, int i) { return h ^ ((uint32_t)i * 0x85ebca6bu); }
+  __host__ __device__ __forceinline__ static T encode(float g, uint32_t r) { return round_f16_reference(g * kScale, r); }
+  __host__ __device__ __forceinline__ static T encode_scaled(float gs, uint32_t r) { return round_f16_reference(gs, r); }
+  __host__ __device__ __forceinline__ static uint32_t encode_pair_scaled(uint32_t, float as, uint32_t ra, float bs,
+                                                                         uint32_t rb) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, encode_scaled(as, ra)) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, encode_scaled(bs, rb)) << 16);
+  }
+  __host__ __device__ __forceinline__ static uint32_t encode_pair(float a, uint32_t ra, float b, uint32_t rb) {
+    return encode_pair_scaled(0u, a * kScale, ra, b * kScale, rb);
+  }
+};
+
 // bf16 deviations, unscaled (this library's own format).
-struct StoreBF16 {
+struct StoreBF16 : HashedWords {
   using T = uint16_t;
   static constexpr int kBytes = 2;
   static constexpr int kShift = 1;

@@ -220,6 +220,8 @@ void launch_init(pw_solver* s, void* out, const StepParams& p) {
 extern template void launch_step_s<StoreF32>(pw_solver*, const void*, void*, bool, const StepParams&, uint32_t);
 extern template void launch_step_s<StoreF16>(pw_solver*, const void*, void*, bool, const StepParams&, uint32_t);
 extern template void launch_step_s<StoreBF16>(pw_solver*, const void*, void*, bool, const StepParams&, uint32_t);
+extern template void launch_step_s<StoreF16Reference>(pw_solver*, const void*, void*, bool, const StepParams&,
+                                                      uint32_t);
 
 namespace {
 
@@ -231,7 +233,10 @@ void launch_step(pw_solver* s, bool macro, const StepParams& params) {
   StepParams p = params;
   launch_closure(s, p);
   switch (s->params.storage) {
-    case PW_STORAGE_F16: launch_step_s<StoreF16>(s, in, out, macro, p, seed); break;
+    case PW_STORAGE_F16:
+      if (s->rounding == PW_ROUNDING_REFERENCE) launch_step_s<StoreF16Reference>(s, in, out, macro, p, seed);
+      else launch_step_s<StoreF16>(s, in, out, macro, p, seed);
+      break;
     case PW_STORAGE_BF16: launch_step_s<StoreBF16>(s, in, out, macro, p, seed); break;
     default: launch_step_s<StoreF32>(s, in, out, macro, p, seed); break;
   }
@@ -256,7 +261,10 @@ void launch_edge_any(pw_solver* s, void* set, const StepParams& p) {
 void launch_init_both(pw_solver* s, const StepParams& p) {
   for (int k = 0; k < 2; ++k) {
     switch (s->params.storage) {
-      case PW_STORAGE_F16: launch_init<StoreF16>(s, s->pops[k], p); break;
+      case PW_STORAGE_F16:
+        if (s->rounding == PW_ROUNDING_REFERENCE) launch_init<StoreF16Reference>(s, s->pops[k], p);
+        else launch_init<StoreF16>(s, s->pops[k], p);
+        break;
       case PW_STORAGE_BF16: launch_init<StoreBF16>(s, s->pops[k], p); break;
       default: launch_init<StoreF32>(s, s->pops[k], p); break;
     }
@@ -695,6 +703,20 @@ pw_status pw_solver_reset(pw_solver* s) {
   s->ramp_settled = 0;
   s->force_encoded = 0;  // gpu-force.ts:208-210: an unread sample is discarded
   return PW_OK;
+}
+
+pw_status pw_solver_set_rounding(pw_solver* s, uint32_t rounding) {
+  if (pw_status st = check_handle(s)) return st;
+  if (rounding > PW_ROUNDING_REFERENCE) {
+    set_error("Unknown f16 rounding.");
+    return PW_EINVAL;
+  }
+  if (s->params.storage != PW_STORAGE_F16) {
+    set_error("The rounding option applies to f16 storage.");
+    return PW_EINVAL;
+  }
+  s->rounding = rounding;
+  return pw_solver_reset(s);
 }
 
 uint64_t pw_solver_step_count(const pw_solver* s) { return solver_live(s) ? s->steps : 0; }

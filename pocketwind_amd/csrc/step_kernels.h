@@ -482,7 +482,7 @@ __host__ __device__ __forceinline__ void store_state(typename S::T* __restrict__
                                                      const float* g, uint32_t h) {
 #pragma unroll
   for (int i = 0; i < kQ; ++i)
-    st<NT>(byte_offset(out + (size_t)i * P, self), encode_out<S, SCALED>(g[i], direction_random(h, i)));
+    st<NT>(byte_offset(out + (size_t)i * P, self), encode_out<S, SCALED>(g[i], S::dir_word(h, i)));
 }
 
 // common.wgsl:145-150 with rho - 1 = 0, in pair form: eq_i = w (4.5 cu^2 - 1.5 u^2) + 3 w cu.
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ o
   // whose direction-0 term could come out as -0 depending on how the compiler folds it.
   if (wall) rest_state(g);
   else equilibrium_values(p.u_in, 0.0f, 0.0f, g);
-  store_state<S, false>(out, p.run, pop_cell(p, n) << S::kShift, g, cell_random(n, kInitSeed));
+  store_state<S, false>(out, p.run, pop_cell(p, n) << S::kShift, g, S::cell_word(n, kInitSeed));
   macro[n] = make_float4(1.0f, wall ? 0.0f : p.u_in, 0.0f, 0.0f);
 }
 
@@ -681,7 +681,7 @@ template <class S, bool SCALED = false>
 __host__ __device__ __forceinline__ RowMoments encoded_moments(const float* g, uint32_t h) {
   RowMoments m{1.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) add_moment(m, i, S::decode(encode_out<S, SCALED>(g[i], direction_random(h, i))));
+  for (int i = 0; i < kQ; ++i) add_moment(m, i, S::decode(encode_out<S, SCALED>(g[i], S::dir_word(h, i))));
   return m;
 }
 
@@ -845,7 +845,7 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
     rest_state(g);
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
-  const uint32_t h = cell_random(n, seed);
+  const uint32_t h = S::cell_word(n, seed);
   store_state<S, POL != kCached, (kOutLog2<S, OP> != 0)>(out, P, self, g, h);
   if constexpr (EDGE) {
     if (x + 2 == p.nx) {
@@ -971,14 +971,14 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, uint3
     T* b = out + e0 + lane;
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
-      const float v = encode_out<S, SCALED>(g[i], direction_random(h, i));
+      const float v = encode_out<S, SCALED>(g[i], S::dir_word(h, i));
       st<NT>(b + i * kRowRun, v);
       if (moments) add_moment(m, i, v);
     }
   } else {
     // Every store first; the outlet wavefront's moments afterwards from the packed words.
     uint16_t* b = reinterpret_cast<uint16_t*>(out) + e0 + lane;
-    const uint16_t v0 = __builtin_bit_cast(uint16_t, encode_out<S, SCALED>(g[0], direction_random(h, 0)));
+    const uint16_t v0 = __builtin_bit_cast(uint16_t, encode_out<S, SCALED>(g[0], S::dir_word(h, 0)));
     st<NT>(b, v0);
     uint32_t words[kPairs];
 #pragma unroll
@@ -986,8 +986,8 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, uint3
       const int i = 2 * q + 1;
       const float scale = SCALED ? 1.0f : kOut<S::kScaleLog2>;
       // The pair's first random word is dead after its conversion, so its register takes the pair.
-      const uint32_t ra = direction_random(h, i);
-      words[q] = S::encode_pair_scaled(ra, g[i] * scale, ra, g[i + 1] * scale, direction_random(h, i + 1));
+      const uint32_t ra = S::dir_word(h, i);
+      words[q] = S::encode_pair_scaled(ra, g[i] * scale, ra, g[i + 1] * scale, S::dir_word(h, i + 1));
       st<NT>(b + i * kRowRun, (uint16_t)(words[q] & 0xffffu));
       st<NT>(b + (i + 1) * kRowRun, (uint16_t)(words[q] >> 16));
     }
@@ -1060,7 +1060,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
   RowMoments stored{1.0f, 0.0f, 0.0f, 0.0f};
-  store_row<S, POL != kCached, (kOutLog2<S, OP> != 0)>(out, w.e0, lane, g, cell_random(n, seed), outlet_wave,
+  store_row<S, POL != kCached, (kOutLog2<S, OP> != 0)>(out, w.e0, lane, g, S::cell_word(n, seed), outlet_wave,
                                                           stored);
   if (outlet_wave && lane == 62) edge_rows(out, p)[row] = make_float4(stored.mass, stored.mx, stored.my, stored.mz);
   if (MACRO && !is_wall(kind)) {

@@ -242,6 +242,16 @@ class GpuSolver:
         """Test hook: one cell per lane even on row-wavefront grids (nx a multiple of 64)."""
         N.check(N.lib().pw_debug_cell_path(self._live(), int(bool(on))))
 
+    ROUNDINGS = ("hardware", "reference")
+
+    def set_rounding(self, rounding: str) -> None:
+        """Rounding of f16 stores: "hardware" (the default) or "reference", the rule of
+        prelude-f16.wgsl:7-22 (random word lowbias32 of the value's bits, cell and direction).
+        Resets the solver to its initial state under the new rounding."""
+        if rounding not in self.ROUNDINGS:
+            raise ValueError(f"Unknown f16 rounding {rounding!r}.")
+        N.check(N.lib().pw_solver_set_rounding(self._live(), self.ROUNDINGS.index(rounding)))
+
     def set_variant(self, kernel: str, block: int, xcd_chunks: int) -> None:
         """An explicit step variant: population cache policy, threads per block (64, 128,
         256) and chunks per XCD (1, 16), without timing."""
