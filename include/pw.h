@@ -389,6 +389,11 @@ pw_status pw_solver_set_closure(pw_solver *s, const pw_mlp *net, const float *no
 /* Evaluate the closure on the steps whose count is a multiple of `every` (default 1) and
  * reuse its stresses in between (the first step after attaching or reset always evaluates). */
 pw_status pw_solver_set_closure_interval(pw_solver *s, uint32_t every);
+/* Test hook: the wall collision takes the given stress vectors (x, y, z, w per wall cell, in
+ * the order of pw_solver_wall_cells; w > 0 uses the vector, else the wall law) in place of the
+ * network's, until pw_solver_set_closure attaches a network again. Used to measure what a
+ * perfect closure output would give. Learned wall model only. No reference counterpart. */
+pw_status pw_debug_set_wall_stress(pw_solver *s, const float *stress, size_t n_floats);
 /* The last closure evaluation: 4 floats per wall cell, stress xyz and 1 (used) or 0 (wall
  * law). */
 pw_status pw_solver_closure_output(pw_solver *s, float *out, size_t n_floats);

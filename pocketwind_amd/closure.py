@@ -306,6 +306,78 @@ def evaluate_cd(reynolds: float = 2000.0, layout=(256, 128, 128), length: int = 
             "wall_cells": walls, "wall_cells_on_network": used}
 
 
+def drag_split(forces, normals, frontal_cells: float, u_in: float) -> dict:
+    """Cd of per-wall-cell forces (n x 3, lattice units) split into the part along each cell's
+    unit wall normal (pressure) and the rest (friction)."""
+    f = np.asarray(forces, np.float64)
+    n = np.asarray(normals, np.float64)
+    n = n / np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-12)
+    normal = np.sum(f * n, 1, keepdims=True) * n
+    scale = 1.0 / (0.5 * u_in * u_in * frontal_cells)
+    return {"cd": float(f[:, 0].sum() * scale), "pressure": float(normal[:, 0].sum() * scale),
+            "friction": float((f - normal)[:, 0].sum() * scale)}
+
+
+def shear_floor(reynolds: float = 320.0, layout=(256, 128, 128), length: int = 64, u_in: float = 0.05,
+                flow_throughs: int = 6, average: int = 2, snapshots: int = 40, log=print) -> dict:
+    """What a wall-shear closure can do for Cd at best: the slanted bluff body of evaluate_cd
+    on `layout` with the wall law; on the twice finer grid with the wall law; and on `layout`
+    with the fine grid's time-averaged wall shear restricted to each coarse wall cell
+    (restrict_stress, the closure's training target) prescribed as the closure output, i.e. a
+    perfect network. Each run's window-mean Cd and its time-averaged wall forces split into
+    pressure (wall-normal) and friction (tangential) parts."""
+    from . import bodies, drag
+    from .gpu_solver import GpuSolver, SolverParams
+    from .voxelize import voxelize
+
+    def run(grid_layout, grid_length, prescribed=None):
+        body, _ = bodies.place_mesh(bodies.bluff_body(), None, grid_length, grid_layout, False)
+        grid = voxelize(body, None, grid_layout, False, link_fractions=True)
+        nu = u_in * grid_length / reynolds
+        s = GpuSolver.create(grid_layout, grid, SolverParams(3 * nu + 0.5, u_in, True, operator="reg",
+                                                             wall_model="law" if prescribed is None else "learned"))
+        try:
+            if prescribed is not None:
+                s.prescribe_wall_stress(*prescribed)
+            flow = drag.flow_through_steps(grid.nx, u_in)
+            total, window = flow_throughs * flow, average * flow
+            samples = [s.step_sampled(total - window, 8)]
+            chunk = max(8, (window // snapshots) // 8 * 8)
+            forces, done = None, total - window
+            while done < total:
+                k = min(chunk, total - done)
+                samples.append(s.step_sampled(k, 8))
+                done += k
+                f = s.wall_forces()
+                forces = f if forces is None else forces + f
+            forces /= -(-window // chunk)
+            fx, _, _ = drag.window_mean_force(np.concatenate(samples), total, window)
+            rec = s.closure_features(u_in)
+            out = dict(cells=s.wall_cells(), normals=rec[:, 8:11].astype(np.float64),
+                       exposed=rec[:, 11].astype(np.float64), forces=forces,
+                       cd=fx * drag.cd_per_lattice_force(grid.frontal_cells, u_in),
+                       split=drag_split(forces, rec[:, 8:11], grid.frontal_cells, u_in))
+        finally:
+            s.destroy()
+        log(f"  {grid_layout} {'wall law' if prescribed is None else 'prescribed fine-grid shear'}: "
+            f"Cd {out['cd']:.6f} (time-averaged wall forces: {out['split']})")
+        return out
+
+    fine_layout = tuple(2 * v for v in layout)
+    coarse = run(layout, length)
+    fine = run(fine_layout, 2 * length)
+    normals = coarse["normals"] / np.maximum(np.linalg.norm(coarse["normals"], axis=1, keepdims=True), 1e-12)
+    target, support = restrict_stress(coarse["cells"], layout, normals, fine["cells"], fine_layout, fine["forces"],
+                                      fine["normals"], fine["exposed"])
+    perfect = run(layout, length, (target, support > 0))
+    pct = lambda a: 100 * (a / fine["cd"] - 1)
+    return {"reynolds": reynolds, "layout": list(layout), "length_cells": length,
+            "cd_law": coarse["cd"], "cd_prescribed_fine_shear": perfect["cd"], "cd_law_2x_grid": fine["cd"],
+            "law_vs_2x_grid_pct": pct(coarse["cd"]), "prescribed_vs_2x_grid_pct": pct(perfect["cd"]),
+            "split_law": coarse["split"], "split_prescribed": perfect["split"], "split_2x_grid": fine["split"],
+            "wall_cells": int(len(coarse["cells"])), "wall_cells_with_target": int(np.sum(support > 0))}
+
+
 def main(argv=None):
     """Train the closure on coarse/fine pairs of random bodies and save it.
 
@@ -325,10 +397,20 @@ def main(argv=None):
     ap.add_argument("--flow-throughs", type=float, default=4.0)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--shear-floor", type=float, default=None, metavar="RE",
+                    help="instead of training, measure Cd with the fine grid's wall shear prescribed "
+                         "on the coarse grid (a perfect closure output) beside the wall law and the "
+                         "twice finer grid at this Reynolds number")
     ap.add_argument("--evaluate", type=float, default=None, metavar="RE",
                     help="instead of training, compare Cd of the learned closure with the wall law "
                          "and the twice finer grid at this Reynolds number (the closure file --out)")
     args = ap.parse_args(argv)
+    if args.shear_floor is not None:
+        rec = shear_floor(args.shear_floor, tuple(args.coarse), args.length, log=lambda m: print(m, flush=True))
+        print(json.dumps(rec))
+        if args.report:
+            json.dump(rec, open(args.report, "w"), indent=1)
+        return 0
     if args.evaluate is not None:
         rec = evaluate_cd(args.evaluate, tuple(args.coarse), args.length, args.out,
                           log=lambda m: print(m, flush=True))

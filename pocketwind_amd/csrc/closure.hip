@@ -81,7 +81,8 @@ void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, c
 
 void launch_closure(pw_solver* s, StepParams& p) {
   if (!s->closure_on || s->n_wall == 0) return;
-  if (!s->closure_fresh || s->steps % s->closure_every == 0) {
+  // closure_on 2: the output was prescribed from the host (pw_debug_set_wall_stress).
+  if (s->closure_on == 1 && (!s->closure_fresh || s->steps % s->closure_every == 0)) {
     launch_closure_any(s, s->pops[s->current], s->closure_net, s->closure_tau, nullptr);
     s->closure_fresh = 1;
   }
@@ -189,6 +190,26 @@ pw_status pw_solver_set_closure(pw_solver* s, const pw_mlp* net, const float* no
   std::memcpy(s->closure_norm, norm, n_norm * sizeof(float));
   s->closure_on = 1;
   s->closure_fresh = 0;
+  return PW_OK;
+}
+
+pw_status pw_debug_set_wall_stress(pw_solver* s, const float* stress, size_t n_floats) {
+  if (pw_status st = check(s)) return st;
+  if (s->params.wall_model != PW_WALL_LEARNED || !s->closure_tau) {
+    set_error("The solver was not created with the learned wall model.");
+    return PW_EINVAL;
+  }
+  if (!stress || n_floats != 4 * s->n_wall) {
+    set_error("Wall stress input needs 4 floats per wall cell.");
+    return PW_EINVAL;
+  }
+  if (s->n_wall) {
+    hipError_t e = hipMemcpyAsync(s->closure_tau, stress, n_floats * sizeof(float), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return device_error(e, "wall stress upload");
+  }
+  s->closure_on = 2;
+  s->closure_fresh = 1;
   return PW_OK;
 }
 

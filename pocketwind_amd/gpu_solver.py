@@ -294,6 +294,15 @@ class GpuSolver:
         norm = np.ascontiguousarray(norm, np.float32)
         N.check(N.lib().pw_solver_set_closure(self._live(), net._h, norm.ctypes.data, norm.size))
 
+    def prescribe_wall_stress(self, stress, valid) -> None:
+        """Test hook: the wall collision uses these stress vectors (n_wall x 3, wall-cell order)
+        where `valid`, the wall law elsewhere, in place of a network's output."""
+        stress = np.asarray(stress, np.float32).reshape(-1, 3)
+        out = np.zeros((stress.shape[0], 4), np.float32)
+        out[:, :3] = stress
+        out[:, 3] = np.asarray(valid, bool)
+        N.check(N.lib().pw_debug_set_wall_stress(self._live(), out.ctypes.data, out.size))
+
     def set_closure_interval(self, every: int) -> None:
         N.check(N.lib().pw_solver_set_closure_interval(self._live(), int(every)))
 
