@@ -342,6 +342,8 @@ void pw_batch_destroy(pw_batch *b);
  * same closure. */
 pw_status pw_batch_set_closure(pw_batch *b, const pw_mlp *net, const float *norm, size_t n_norm);
 pw_status pw_batch_set_closure_interval(pw_batch *b, uint32_t every);
+/* pw_solver_set_closure_placement for every tunnel of the batch (row tunnels only). */
+pw_status pw_batch_set_closure_placement(pw_batch *b, uint32_t placement);
 /* The momentum-exchange force of every tunnel now (fx, fy, fz per tunnel, FP64; the mean over
  * both states as a force sample), waiting for the batch's queued work. */
 pw_status pw_batch_force(pw_batch *b, double *out, size_t out_doubles);
@@ -389,6 +391,16 @@ pw_status pw_solver_set_closure(pw_solver *s, const pw_mlp *net, const float *no
 /* Evaluate the closure on the steps whose count is a multiple of `every` (default 1) and
  * reuse its stresses in between (the first step after attaching or reset always evaluates). */
 pw_status pw_solver_set_closure_interval(pw_solver *s, uint32_t every);
+/* Where the closure is evaluated. KERNEL (the default): its own kernel over the wall-cell list,
+ * on the state the next step reads, every `every` steps. IN_STEP: inside the step kernel, by
+ * each wavefront that holds wall cells, every step (the interval is ignored), for the REG +
+ * LES step with the reference's boundaries on grids whose nx is a multiple of 64; other
+ * solvers keep the kernel. Both give the same stresses bit for bit. In-step evaluation saves
+ * the launch and the stress buffer but lengthens the wall wavefronts by the features' gathers
+ * and the network, and measures slower than the kernel every step on the 256x128x128 bluff
+ * body. No reference counterpart. */
+typedef enum { PW_CLOSURE_KERNEL = 0, PW_CLOSURE_IN_STEP = 1 } pw_closure_placement;
+pw_status pw_solver_set_closure_placement(pw_solver *s, uint32_t placement);
 /* Test hook: the wall collision takes the given stress vectors (x, y, z, w per wall cell, in
  * the order of pw_solver_wall_cells; w > 0 uses the vector, else the wall law) in place of the
  * network's, until pw_solver_set_closure attaches a network again. Used to measure what a

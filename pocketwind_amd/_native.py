@@ -74,6 +74,8 @@ SIGNATURES = {
     "pw_debug_cell_path": (C.c_int, [C.c_void_p, C.c_int]),
     "pw_solver_set_rounding": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pw_debug_set_wall_stress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "pw_solver_set_closure_placement": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "pw_batch_set_closure_placement": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pw_solver_set_step_variant": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     "pw_solver_xcd_chunks": (C.c_uint32, [C.c_void_p]),
     "pw_solver_kernel": (C.c_uint32, [C.c_void_p]),

@@ -94,6 +94,13 @@ class TunnelBatch:
     def set_closure_interval(self, every: int) -> None:
         N.check(N.lib().pw_batch_set_closure_interval(self._h, int(every)))
 
+    def set_closure_placement(self, placement: str) -> None:
+        """"kernel" (the default) or "step": GpuSolver.set_closure_placement for every row tunnel."""
+        placements = ("kernel", "step")
+        if placement not in placements:
+            raise ValueError(f"Unknown closure placement {placement!r}.")
+        N.check(N.lib().pw_batch_set_closure_placement(self._h, placements.index(placement)))
+
     def time_steps(self, count: int) -> float:
         ms = C.c_float()
         N.check(N.lib().pw_batch_time_steps(self._h, int(count), C.byref(ms)))

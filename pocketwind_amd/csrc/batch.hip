@@ -138,15 +138,18 @@ struct pw_batch {
   float closure_norm[32] = {};
   int closure_on = 0;
   uint32_t closure_every = 1;
+  uint32_t closure_placement = 0;  // pw_closure_placement
   int closure_fresh = 0;
   std::vector<std::pair<const uint32_t*, float4*>> closure_bufs;  // per tunnel: slot table, output
+  bool closure_kernels = false;  // some tunnel evaluates the closure in the closure kernel
+  uint32_t lds = 0;              // dynamic LDS per step block (closure evaluated in the step)
 };
 
 namespace {
 
 void launch_step(pw_batch* b, int parity, bool macro) {
   const BatchLaunch L{b->step_grid, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, b->seeds,
-                      b->params, b->policy, b->chunk};
+                      b->params, b->policy, b->chunk, b->lds};
   switch (b->params.storage) {
     case PW_STORAGE_F16: batch_step_s<StoreF16>(L, parity, macro); break;
     case PW_STORAGE_BF16: batch_step_s<StoreBF16>(L, parity, macro); break;
@@ -176,7 +179,8 @@ void launch_init(pw_batch* b) {
 // The closure of every tunnel on the state the next step reads.
 void launch_closures(pw_batch* b) {
   for (const Tunnel& tu : b->host)
-    launch_closure_tunnel(b->params.storage, b->params.walls == PW_WALLS_BOUZIDI, b->stream, tu.pops[b->parity],
+    if (!tu.p.closure_net)
+      launch_closure_tunnel(b->params.storage, b->params.walls == PW_WALLS_BOUZIDI, b->stream, tu.pops[b->parity],
                           tu.geo, tu.p, tu.wall_cells, tu.n_wall, b->closure_net, b->closure_norm,
                           b->closure_bufs[&tu - b->host.data()].second);
 }
@@ -227,6 +231,32 @@ hipError_t capture_pair(pw_batch* b, int first_parity, hipGraphExec_t* exec) {
 }
 
 pw_status step_plain(pw_batch* b, uint32_t count, int write_macro_on_last);
+
+// Where each tunnel's closure runs once a network is attached: inside the step (p.closure_net)
+// on row tunnels when the placement asks for it, every step; otherwise in the closure kernel
+// into the tunnel's buffer (p.closure_tau). The captured step graphs hold the launch's LDS
+// size and are dropped.
+hipError_t apply_closure_mode(pw_batch* b) {
+  b->closure_kernels = false;
+  b->lds = 0;
+  for (uint32_t t = 0; t < b->count; ++t) {
+    StepParams& p = b->host[t].p;
+    const bool fused = b->closure_placement == PW_CLOSURE_IN_STEP && p.rows && b->params.op == PW_OP_REG && b->params.les;
+    p.closure_net = fused ? b->closure_net : nullptr;
+    p.closure_slot = fused ? nullptr : b->closure_bufs[t].first;
+    p.closure_tau = fused ? nullptr : b->closure_bufs[t].second;
+    if (fused) b->lds = (kBlock / 64) * closure::kFusedWaveBytes;
+    else b->closure_kernels = true;
+  }
+  for (auto& g : b->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  hipError_t e = hipMemcpyAsync(b->tunnels, b->host.data(), b->count * sizeof(Tunnel), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  return e;
+}
 
 bool forced_batch_variant(pw_batch* b, const char* text) {
   if (!text) return false;
@@ -434,7 +464,7 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   // than the resident capacity would leave its last blocks to run after the others).
   b->grid = (unsigned)std::min<uint64_t>(b->n_tiles, 0x7fffffffu);
   if (shared->wall_model == PW_WALL_LEARNED &&
-      (e = dev_alloc(b, reinterpret_cast<char**>(&b->closure_net), mlp_packed_bytes())) != hipSuccess)
+      (e = dev_alloc(b, reinterpret_cast<char**>(&b->closure_net), closure::kNetBlockBytes)) != hipSuccess)
     return fail(e, "batch closure network");
   launch_init(b);
   // PW_BATCH_KERNEL=<cell|cell-nt|cell-nt-stores>/<chunks per XCD> fixes the step variant;
@@ -464,7 +494,7 @@ pw_status pw_batch_step(pw_batch* b, uint32_t count, int write_macro_on_last) {
     set_error("Null batch handle.");
     return PW_EINVAL;
   }
-  if (!b->closure_on) return step_plain(b, count, write_macro_on_last);
+  if (!b->closure_on || !b->closure_kernels) return step_plain(b, count, write_macro_on_last);
   // The closure runs on the state a step reads, before the first step of a run and at every
   // multiple of its interval, as in a single solver.
   uint32_t done = 0;
@@ -500,19 +530,19 @@ pw_status pw_batch_set_closure(pw_batch* b, const pw_mlp* net, const float* norm
     set_error("Closure velocity unit and limit must be positive.");
     return PW_EINVAL;
   }
+  char block_norm[closure::kNetBlockBytes - closure::kNormOffset] = {};
+  std::memcpy(block_norm, norm, n_norm * sizeof(float));
   hipError_t e = hipMemcpyAsync(b->closure_net, mlp_packed(net), mlp_packed_bytes(), hipMemcpyDeviceToDevice, b->stream);
-  for (uint32_t t = 0; t < b->count; ++t) {
-    b->host[t].p.closure_slot = b->closure_bufs[t].first;
-    b->host[t].p.closure_tau = b->closure_bufs[t].second;
-  }
   if (e == hipSuccess)
-    e = hipMemcpyAsync(b->tunnels, b->host.data(), b->count * sizeof(Tunnel), hipMemcpyHostToDevice, b->stream);
+    e = hipMemcpyAsync(static_cast<char*>(b->closure_net) + closure::kNormOffset, block_norm, sizeof block_norm,
+                       hipMemcpyHostToDevice, b->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) return device_error(e, "batch closure attach");
   std::memcpy(b->closure_norm, norm, n_norm * sizeof(float));
   b->closure_on = 1;
   b->closure_fresh = 0;
-  return PW_OK;
+  e = apply_closure_mode(b);
+  return e == hipSuccess ? PW_OK : device_error(e, "batch closure attach");
 }
 
 pw_status pw_batch_force(pw_batch* b, double* out, size_t out_doubles) {
@@ -531,13 +561,27 @@ pw_status pw_batch_force(pw_batch* b, double* out, size_t out_doubles) {
   return e == hipSuccess ? PW_OK : device_error(e, "batch force");
 }
 
+pw_status pw_batch_set_closure_placement(pw_batch* b, uint32_t placement) {
+  if (!b || placement > PW_CLOSURE_IN_STEP) {
+    set_error("Unknown closure placement.");
+    return PW_EINVAL;
+  }
+  b->closure_placement = placement;
+  b->closure_fresh = 0;
+  if (!b->closure_on) return PW_OK;
+  const hipError_t e = apply_closure_mode(b);
+  return e == hipSuccess ? PW_OK : device_error(e, "batch closure placement");
+}
+
 pw_status pw_batch_set_closure_interval(pw_batch* b, uint32_t every) {
   if (!b || every == 0) {
     set_error("The closure interval must be at least one step.");
     return PW_EINVAL;
   }
   b->closure_every = every;
-  return PW_OK;
+  if (!b->closure_on) return PW_OK;
+  const hipError_t e = apply_closure_mode(b);
+  return e == hipSuccess ? PW_OK : device_error(e, "batch closure interval");
 }
 
 }  // extern "C"

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "force_kernels.h"
+#include "step_row.h"
 
 namespace pw {
 namespace batch {
@@ -10,7 +11,7 @@ namespace batch {
 constexpr int kBlock = 256;
 
 struct Tunnel {
-  StepParams p;  // p.closure_tau / p.closure_slot point at this tunnel's closure output once attached
+  StepParams p;  // p.closure_tau / p.closure_slot (closure kernel) or p.closure_net (in the step) once attached
   void* pops[2];
   const uint32_t* geo;
   float4* macro;
@@ -33,6 +34,7 @@ struct BatchLaunch {
   pw_params params;
   uint32_t policy;  // population cache policy (kCached, kNtAll, kNtStores)
   uint32_t chunk;   // consecutive tiles one XCD takes (the grid holds a multiple of 8 chunks)
+  uint32_t lds;     // dynamic LDS bytes per block: the closure evaluated in the step, else 0
 };
 
 // One step of every tunnel. The rounding seed is the step count, read on the device so a
@@ -43,7 +45,8 @@ struct BatchLaunch {
 // cell. Either way a tile is one block of one tunnel.
 // One block per tile. Blocks are renumbered as in the single solver's step so that each XCD
 // takes runs of `chunk` consecutive tiles (neighbouring rows of one tunnel share its L2).
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL>
+// CLOSURE: row tunnels evaluate the learned closure in their wall wavefronts (REG + LES).
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool CLOSURE = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWaves<OP, LES>))) void batch_step_kernel(const Tunnel* __restrict__ tunnels,
                                                             const uint32_t* __restrict__ tile_tunnel,
                                                             const uint32_t* __restrict__ tile_first,
@@ -60,10 +63,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWav
   if (n >= tu.p.cells) return;
   const auto* in = (const typename S::T*)tu.pops[parity];
   auto* out = (typename S::T*)tu.pops[parity ^ 1];
-  if (tu.p.rows)
-    stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL, false>(in, out, tu.geo, tu.macro, tu.p, n, seed);
-  else
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL>(in, out, tu.geo, tu.macro, tu.p, n, seed);
+  if constexpr (CLOSURE) {
+    extern __shared__ __attribute__((aligned(16))) char step_lds[];
+    if (tu.p.rows) {
+      stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL, false, true>(
+          in, out, tu.geo, tu.macro, tu.p, n, seed, step_lds + (threadIdx.x >> 6) * closure::kFusedWaveBytes);
+      return;
+    }
+  } else if (tu.p.rows) {
+    stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL, false>(in, out, tu.geo, tu.macro, tu.p, n, seed, nullptr);
+    return;
+  }
+  stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL>(in, out, tu.geo, tu.macro, tu.p, n, seed);
 }
 
 // Grid of a batched step: the tiles rounded up to whole chunks on each of the 8 XCDs.
@@ -78,7 +89,13 @@ void batch_step_t(const BatchLaunch& L, int parity) {
   auto k = L.policy == kNtAll ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtAll>
            : L.policy == kNtStores ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtStores>
                                    : batch_step_kernel<S, OP, LES, BZ, MACRO, kCached>;
-  hipLaunchKernelGGL(k, dim3(L.grid), dim3(kBlock), 0, L.stream, L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles,
+  if constexpr (OP == kOpReg && LES) {
+    if (L.lds)
+      k = L.policy == kNtAll ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtAll, true>
+          : L.policy == kNtStores ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtStores, true>
+                                  : batch_step_kernel<S, OP, LES, BZ, MACRO, kCached, true>;
+  }
+  hipLaunchKernelGGL(k, dim3(L.grid), dim3(kBlock), L.lds, L.stream, L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles,
                      parity, L.seeds, L.chunk);
 }
 

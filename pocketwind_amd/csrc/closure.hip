@@ -7,8 +7,10 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "closure.h"
+#include "step_row.h"
 
 namespace pw {
 
@@ -47,6 +49,13 @@ void launch_wall_forces_s(pw_solver* s, double* out) {
                      s->wall_cells, (uint32_t)s->n_wall, out);
 }
 
+// The closure runs inside the step when asked to and the step has instances that carry it: the
+// row path, the REG collision with LES and the reference's boundaries.
+bool in_step_closure(const pw_solver* s) {
+  return s->closure_on == 1 && s->closure_placement == PW_CLOSURE_IN_STEP && s->sp.rows && s->params.op == PW_OP_REG &&
+         s->params.les && !s->sp.bc;
+}
+
 pw_status check(pw_solver* s) {
   if (!solver_live(s)) {
     set_error("Null or destroyed solver handle.");
@@ -81,6 +90,12 @@ void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, c
 
 void launch_closure(pw_solver* s, StepParams& p) {
   if (!s->closure_on || s->n_wall == 0) return;
+  // Evaluated every step on the row path: inside the step's wall wavefronts.
+  if (in_step_closure(s) && s->tile_order && s->order_block == s->block && s->order_chunks == s->xcd_chunks) {
+    p.closure_net = s->closure_net;
+    p.tile_order = s->tile_order;
+    return;
+  }
   // closure_on 2: the output was prescribed from the host (pw_debug_set_wall_stress).
   if (s->closure_on == 1 && (!s->closure_fresh || s->steps % s->closure_every == 0)) {
     launch_closure_any(s, s->pops[s->current], s->closure_net, s->closure_tau, nullptr);
@@ -88,6 +103,42 @@ void launch_closure(pw_solver* s, StepParams& p) {
   }
   p.closure_tau = s->closure_tau;
   p.closure_slot = s->closure_slot;
+}
+
+hipError_t closure_tile_order(pw_solver* s) {
+  if (s->params.wall_model != PW_WALL_LEARNED || !s->sp.rows) return hipSuccess;
+  // The launch's grid (launch_step_t): blocks rounded up to whole chunks on the 8 XCDs.
+  const uint64_t blocks = (s->cells + s->block - 1) / s->block;
+  const uint32_t parts = s->xcd_chunks;
+  const uint64_t per_xcd = ((blocks + kXcds - 1) / kXcds + parts - 1) / parts * parts;
+  const uint64_t grid = kXcds * per_xcd;
+  std::vector<uint32_t> cells(s->n_wall);
+  hipError_t e = s->n_wall ? hipMemcpy(cells.data(), s->wall_cells, s->n_wall * 4, hipMemcpyDeviceToHost) : hipSuccess;
+  if (e != hipSuccess) return e;
+  std::vector<uint8_t> wall(grid, 0);
+  for (uint32_t c : cells) wall[c / s->block] = 1;
+  // The logical blocks the launch reaches first are each XCD's first chunk in order; give them
+  // the wall blocks, in lattice order, then every other block in lattice order.
+  std::vector<uint32_t> first, rest;
+  for (uint64_t t = 0; t < grid; ++t) (wall[t] ? first : rest).push_back((uint32_t)t);
+  std::vector<uint32_t> order(grid);
+  const uint64_t chunk = per_xcd / parts;
+  uint64_t k = 0;
+  std::vector<uint32_t> by_dispatch;  // logical block of the b-th dispatched block
+  by_dispatch.reserve(grid);
+  for (uint64_t b = 0; b < grid; ++b) {
+    const uint64_t local = b / kXcds;
+    by_dispatch.push_back((uint32_t)(((local / chunk) * kXcds + b % kXcds) * chunk + local % chunk));
+  }
+  for (uint32_t t : first) order[by_dispatch[k++]] = t;
+  for (uint32_t t : rest) order[by_dispatch[k++]] = t;
+  if (s->tile_order) (void)hipFree(s->tile_order);
+  s->tile_order = nullptr;
+  if ((e = hipMalloc((void**)&s->tile_order, grid * 4)) != hipSuccess) return e;
+  if ((e = hipMemcpy(s->tile_order, order.data(), grid * 4, hipMemcpyHostToDevice)) != hipSuccess) return e;
+  s->order_block = s->block;
+  s->order_chunks = s->xcd_chunks;
+  return hipSuccess;
 }
 
 }  // namespace pw
@@ -137,6 +188,9 @@ pw_status pw_solver_closure_output(pw_solver* s, float* out, size_t n_floats) {
     set_error("Closure output needs a learned-closure solver and 4 floats per wall cell.");
     return PW_EINVAL;
   }
+  // Evaluated inside the step, the output stays in the step; evaluate it on the current state.
+  if (in_step_closure(s) && s->n_wall)
+    launch_closure_any(s, s->pops[s->current], s->closure_net, s->closure_tau, nullptr);
   hipError_t e = s->n_wall ? hipMemcpyAsync(out, s->closure_tau, s->n_wall * sizeof(float4), hipMemcpyDeviceToHost, s->stream)
                            : hipSuccess;
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -185,10 +239,27 @@ pw_status pw_solver_set_closure(pw_solver* s, const pw_mlp* net, const float* no
     set_error("Closure velocity unit and limit must be positive.");
     return PW_EINVAL;
   }
+  char block_norm[closure::kNetBlockBytes - closure::kNormOffset] = {};
+  std::memcpy(block_norm, norm, n_norm * sizeof(float));
   hipError_t e = hipMemcpy(s->closure_net, mlp_packed(net), mlp_packed_bytes(), hipMemcpyDeviceToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(static_cast<char*>(s->closure_net) + closure::kNormOffset, block_norm, sizeof block_norm,
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = closure_tile_order(s);
   if (e != hipSuccess) return device_error(e, "closure network copy");
   std::memcpy(s->closure_norm, norm, n_norm * sizeof(float));
   s->closure_on = 1;
+  s->closure_fresh = 0;
+  return PW_OK;
+}
+
+pw_status pw_solver_set_closure_placement(pw_solver* s, uint32_t placement) {
+  if (pw_status st = check(s)) return st;
+  if (placement > PW_CLOSURE_IN_STEP) {
+    set_error("Unknown closure placement.");
+    return PW_EINVAL;
+  }
+  s->closure_placement = placement;
   s->closure_fresh = 0;
   return PW_OK;
 }

@@ -135,6 +135,10 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
   out->div_ny = make_fast_div(L.ny);
   out->rows = L.nx % 64 == 0 ? 1u : 0u;
   out->wave_desc = nullptr;
+  out->closure_tau = nullptr;
+  out->closure_slot = nullptr;
+  out->closure_net = nullptr;
+  out->tile_order = nullptr;
   out->bc = (prm.sides == PW_SIDES_FREE_SLIP ? kBcFreeSlip : 0u) |
             (prm.outlet == PW_OUTLET_ZERO_GRADIENT ? kBcZeroGradient : 0u);
   return true;
@@ -376,6 +380,7 @@ void free_all(pw_solver* s) {
   if (s->boundary) hipFree(s->boundary);
   if (s->wall_cells) hipFree(s->wall_cells);
   if (s->wave_desc) hipFree(s->wave_desc);
+  if (s->tile_order) hipFree(s->tile_order);
   if (s->closure_slot) hipFree(s->closure_slot);
   if (s->closure_tau) hipFree(s->closure_tau);
   if (s->closure_net) hipFree(s->closure_net);
@@ -574,7 +579,7 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
     for (size_t k = 0; k < wall_list.size(); ++k) slot[wall_list[k]] = (uint32_t)k;
     if ((e = hipMalloc((void**)&s->closure_slot, N * 4)) != hipSuccess ||
         (e = hipMalloc((void**)&s->closure_tau, (s->n_wall ? s->n_wall : 1) * sizeof(float4))) != hipSuccess ||
-        (e = hipMalloc(&s->closure_net, mlp_packed_bytes())) != hipSuccess)
+        (e = hipMalloc(&s->closure_net, closure::kNetBlockBytes)) != hipSuccess)
       return fail(device_error(e, "closure allocation"));
     if ((e = hipMemcpy(s->closure_slot, slot.data(), N * 4, hipMemcpyHostToDevice)) != hipSuccess)
       return fail(device_error(e, "closure upload"));
@@ -812,6 +817,7 @@ pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
     s->current = cur;
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     hipFree(keep);
+    if (e == hipSuccess && s->closure_on) e = closure_tile_order(s);
     if (e != hipSuccess) return device_error(e, "tune");
     return PW_OK;
   }
@@ -821,6 +827,7 @@ pw_status pw_solver_set_kernel(pw_solver* s, uint32_t kernel) {
   }
   s->kernel = kernel;
   s->block = kStepBlock;
+  if (s->closure_on) PW_TRY(closure_tile_order(s), "closure block order");
   return PW_OK;
 }
 
@@ -845,6 +852,7 @@ pw_status pw_solver_set_step_variant(pw_solver* s, uint32_t kernel, uint32_t blo
   s->kernel = kernel;
   s->block = block;
   s->xcd_chunks = xcd_chunks;
+  if (s->closure_on) PW_TRY(closure_tile_order(s), "closure block order");
   return PW_OK;
 }
 

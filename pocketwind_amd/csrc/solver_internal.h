@@ -71,9 +71,13 @@ struct StepParams {
   // every cell's flag.
   const uint32_t* wave_desc;
   // Learned wall closure (null: the wall law): shear stress per wall cell (xyz, w > 0 when
-  // valid) and the wall-cell index of every cell.
+  // valid) and the wall-cell index of every cell, from the closure kernel; or, on the row path,
+  // the network's device block (matrix-core copy, then normalisation), evaluated inside the step.
   const float4* closure_tau;
   const uint32_t* closure_slot;
+  const void* closure_net;
+  // With closure_net: the block order of the step launch, blocks holding wall cells first.
+  const uint32_t* tile_order;
 };
 
 // Descriptor of a 64-cell run on the row path. 0: every cell is a fluid cell without wall
@@ -106,6 +110,11 @@ namespace pw {
 const void* mlp_packed(const pw_mlp* m);
 size_t mlp_packed_bytes();
 void launch_closure(pw_solver* s, StepParams& p);
+// Build the step's block order for the closure evaluated in the step (blocks that hold wall
+// cells first, so their longer wavefronts start with the launch rather than in its middle)
+// for the solver's current block size and XCD chunking. Allocates; called when a network is
+// attached and when the step variant changes.
+hipError_t closure_tile_order(pw_solver* s);
 // The closure kernel over one tunnel's wall-cell list (batched tunnels): `norm` holds the 27
 // normalisation floats, `net` the packed network.
 void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, const void* state,
@@ -156,6 +165,9 @@ struct pw_solver {
   int closure_on = 0;
   uint32_t closure_every = 1;  // steps between closure evaluations
   int closure_fresh = 0;       // closure_tau holds an evaluation of this run
+  uint32_t closure_placement = 0;  // pw_closure_placement
+  uint32_t* tile_order = nullptr;  // step block order for the closure in the step (closure_tile_order)
+  uint32_t order_block = 0, order_chunks = 0;  // the variant tile_order was built for
   uint32_t* guard_flag = nullptr;       // the word after the macros (not a separate allocation)
   uint32_t* guard_host = nullptr;       // pinned readback
   uint32_t kernel = 1;  // pw_kernel actually used by step()

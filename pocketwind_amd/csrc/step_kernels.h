@@ -261,9 +261,11 @@ __host__ __device__ __forceinline__ float friction_velocity(float nu, float spee
 
 // stream_collide.wgsl:18-80: prescribe the smallest post-collision stress whose bounce-back
 // flux along the slip is the wall-law shear; falls back to the bulk collision.
+// `learned`: the learned closure's wall shear stress vector for this cell (w > 0 when valid),
+// used in place of the wall law; w = 0 keeps the law.
 template <int OP, int SC = 0>
 __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float* g, const CellState& c,
-                                                     uint32_t flag, uint32_t n) {
+                                                     uint32_t flag, const float4& learned) {
   float nx = 0, ny = 0, nz = 0, exposed = 0;
 #pragma unroll
   for (int i = 1; i < kQ; ++i)
@@ -292,10 +294,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
   // The wall shear stress along the slip: the learned closure's where it gave a valid value
   // for this cell, else the wall law's rho u_tau^2.
   float stress = -1.0f;
-  if (p.closure_tau) {
-    const float4 tau = p.closure_tau[p.closure_slot[n]];
-    if (tau.w > 0.0f) stress = fmaxf(tau.x * tx + tau.y * ty + tau.z * tz, 0.0f);
-  }
+  if (learned.w > 0.0f) stress = fmaxf(learned.x * tx + learned.y * ty + learned.z * tz, 0.0f);
   if (stress < 0.0f) {
     const float ut = friction_velocity(p.nu, speed);
     stress = c.rho * ut * ut;
@@ -602,14 +601,21 @@ __global__ __launch_bounds__(256) void init_kernel(typename S::T* __restrict__ o
   macro[n] = make_float4(1.0f, wall ? 0.0f : p.u_in, 0.0f, 0.0f);
 }
 
+// The learned closure's output for cell n from the closure kernel's buffer (none: the law).
+__host__ __device__ __forceinline__ float4 buffered_stress(const StepParams& p, uint32_t flag, uint32_t n) {
+  if (p.closure_tau && (flag & kTypeMask) == kFluid && (flag >> 3) != 0) return p.closure_tau[p.closure_slot[n]];
+  return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 // Moments, equilibrium and collision of one gathered fluid cell (stream_collide.wgsl:114-133);
 // g becomes the post-collision state; returns (rho, u).
 template <int OP, bool LES>
-__host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, uint32_t flag, uint32_t n) {
+__host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, uint32_t flag,
+                                                 const float4& learned) {
   CellState c;
   cell_state(g, c);
   if (LES && (flag >> 3) != 0)
-    collide_wall<OP>(p, g, c, flag, n);
+    collide_wall<OP>(p, g, c, flag, learned);
   else
     collide<OP, LES>(p, g, c);
   return make_float4(c.rho, c.ux, c.uy, c.uz);
@@ -620,14 +626,15 @@ __host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, 
 // non-equilibrium stress, so it leaves with that equilibrium (a wall with the rest state, exactly
 // zero) through the same arithmetic as every fluid lane instead of a separate branch.
 template <int OP, bool LES, int SC = 0>
-__host__ __device__ __forceinline__ float4 relax_prescribed(const StepParams& p, float* g, uint32_t flag, uint32_t n,
-                                                            bool prescribed, float ux, float uy, float uz) {
+__host__ __device__ __forceinline__ float4 relax_prescribed(const StepParams& p, float* g, uint32_t flag,
+                                                            const float4& learned, bool prescribed, float ux,
+                                                            float uy, float uz) {
   CellState c;
   cell_moments(g, c);
   if (prescribed) prescribe_moments(c, ux, uy, uz);
   cell_equilibrium<SC>(c);
   if (LES && (flag >> 3) != 0)
-    collide_wall<OP, SC>(p, g, c, flag, n);
+    collide_wall<OP, SC>(p, g, c, flag, learned);
   else
     collide<OP, LES, SC>(p, g, c, prescribed);
   return make_float4(c.rho, c.ux, c.uy, c.uz);
@@ -816,7 +823,7 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
                                  : S::load(byte_offset(in + (size_t)i * P, pnb.at(-cx(i), -cy(i), -cz(i))));
     }
     if (BC && (p.bc & kBcFreeSlip) && on_side(y, z, p)) slip_cell<S, POL == kNtAll>(in, p, pnb, y, z, g);
-    if constexpr (OP != kOpReg) m = relax<OP, LES>(p, g, flag, n);
+    if constexpr (OP != kOpReg) m = relax<OP, LES>(p, g, flag, LES ? buffered_stress(p, flag, n) : float4{});
   } else if (OP == kOpReg) {
     rest_state(g);  // placeholders: an EQ or wall cell's gathered values do not enter its state
   }
@@ -832,7 +839,8 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
       }
       eq_velocity(p, x, up, ux, uy, uz);
     }
-    m = relax_prescribed<OP, LES, kOutLog2<S, OP>>(p, g, flag, n, kind != kFluid, ux, uy, uz);
+    m = relax_prescribed<OP, LES, kOutLog2<S, OP>>(p, g, flag, LES ? buffered_stress(p, flag, n) : float4{},
+                                                   kind != kFluid, ux, uy, uz);
   } else if (kind == kEq) {
     if constexpr (EDGE) {
       RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
@@ -871,235 +879,6 @@ __device__ __forceinline__ float decode_bits(uint32_t b) {
   using T = typename S::T;
   if constexpr (sizeof(T) == 4) return S::decode(__builtin_bit_cast(T, b));
   else return S::decode(__builtin_bit_cast(T, (uint16_t)b));
-}
-
-// A wavefront whose 64 cells lie in one x-row (nx a multiple of 64) shares the row: its
-// first cell n0, coordinates x0, y, z, the source rows of its pulls and its store bases are
-// wavefront-uniform and live in scalar registers, so loads and stores address with the lane's
-// byte offset alone.
-struct RowWave {
-  uint32_t n0, x0, y, z;
-  uint32_t e0;  // population element offset of n0 (its run starts there: x0 is a multiple of 64)
-};
-
-// Gather of a row wavefront. Each plane is read with aligned, unit-stride loads of the pull's
-// source row, so every cache line of the plane is fetched once. An x-shifted pull then takes
-// its neighbour lane's value with a DPP wavefront shift (wave_shr:1 moves lane l - 1 to lane
-// l, wave_shl:1 lane l + 1 to lane l); the lane at the wavefront's edge takes the value one
-// load brings for the whole wavefront. All 64 lanes must be active.
-// Pull of direction I for a row wavefront from the source row `src` of its plane.
-template <class S, bool NT, int I>
-__device__ __forceinline__ float pull_row(const typename S::T* src, uint32_t xs, uint32_t left, uint32_t right) {
-  const uint32_t own = load_bits<S, NT>(byte_offset(src, xs));
-  if constexpr (cx(I) == 0) {
-    return decode_bits<S>(own);
-  } else {
-    const uint32_t edge = load_bits<S, NT>(byte_offset(src, cx(I) > 0 ? left : right));
-    const int v = cx(I) > 0 ? __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x138, 0xf, 0xf, false)
-                            : __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x130, 0xf, 0xf, false);
-    return decode_bits<S>((uint32_t)v);
-  }
-}
-
-// Sources of lane 0 for c_x = +1 and of lane 63 for c_x = -1 (the zero-gradient outlet holds
-// the last wavefront's at its own column), and the lane's own column, as byte offsets.
-template <int SH>
-__device__ __forceinline__ void row_columns(const StepParams& p, uint32_t bc, uint32_t x0, uint32_t lane,
-                                            uint32_t& left, uint32_t& right, uint32_t& xs) {
-  left = pop_x(p, below(x0, p.nx, false)) << SH;
-  right = pop_x(p, above(x0 + 63, p.nx, bc & kBcZeroGradient)) << SH;
-  xs = (pop_x(p, x0) + lane) << SH;
-}
-
-// On the row path every run holds one wavefront's 64 cells, so direction i's run starts i * 64
-// elements after the row's first run: a compile-time offset from one scalar base per source
-// row (nine rows feed a D3Q19 pull), which the loads and stores carry as immediates.
-constexpr uint32_t kRowRun = 64;
-
-template <class S, bool NT>
-__device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in, const StepParams& p, uint32_t bc,
-                                           const RowWave& w, uint32_t lane, float* g) {
-  constexpr int SH = S::kShift;
-  const uint32_t row = kQ * p.nx, plane = row * p.ny;
-  const uint32_t y = w.y, z = w.z;
-  const uint32_t yo[3] = {((y == 0 ? p.ny - 1 : y - 1) * row) << SH, (y * row) << SH,
-                          ((y + 1 == p.ny ? 0 : y + 1) * row) << SH};
-  const uint32_t zo[3] = {((z == 0 ? p.nz - 1 : z - 1) * plane) << SH, (z * plane) << SH,
-                          ((z + 1 == p.nz ? 0 : z + 1) * plane) << SH};
-  uint32_t left, right, xs;
-  row_columns<SH>(p, bc, w.x0, lane, left, right, xs);
-  static_for<0, kQ>([&](auto c) {
-    constexpr int i = decltype(c)::value;
-    g[i] = pull_row<S, NT, i>(byte_offset(in, yo[1 - cy(i)] + zo[1 - cz(i)]) + i * kRowRun, xs, left, right);
-  });
-}
-
-// Free-slip side row, row path: after the plain gather, the pulls that cross a side are read
-// again, from the mirrored plane of the row unshifted in the crossed axis (all wavefront-uniform).
-template <class S, bool NT>
-__device__ __forceinline__ void slip_row(const typename S::T* __restrict__ in, const StepParams& p,
-                                         const RowWave& w, uint32_t lane, float* g) {
-  constexpr int SH = S::kShift;
-  const uint32_t row = kQ * p.nx, plane = row * p.ny;
-  const uint32_t y = w.y, z = w.z;
-  const bool ylo = y == 0, yhi = y + 1 == p.ny, zlo = z == 0, zhi = z + 1 == p.nz;
-  uint32_t left, right, xs;
-  row_columns<SH>(p, p.bc, w.x0, lane, left, right, xs);
-  static_for<1, kQ>([&](auto c) {
-    constexpr int i = decltype(c)::value;
-    if constexpr (cy(i) != 0 || cz(i) != 0) {
-      const bool my = cy(i) > 0 ? ylo : (cy(i) < 0 ? yhi : false);
-      const bool mz = cz(i) > 0 ? zlo : (cz(i) < 0 ? zhi : false);
-      if (my || mz) {  // a pull that does not cross stays in range in that axis
-        const uint32_t sy = my ? y : y - cy(i), sz = mz ? z : z - cz(i);
-        const int j = slip_plane<i>(ylo, yhi, zlo, zhi);
-        g[i] = pull_row<S, NT, i>(byte_offset(in + (size_t)j * p.run, (sy * row + sz * plane) << SH), xs, left,
-                                  right);
-      }
-    }
-  });
-}
-
-// Stores of a row wavefront: run i of the row at e0 (the wavefront's first element), lane
-// `lane`. 16-bit formats round two directions into one register and store its halves.
-// With `moments` (wavefront-uniform) the stored values' moments accumulate into m.
-template <class S, bool NT, bool SCALED = false>
-__device__ __forceinline__ void store_row(typename S::T* __restrict__ out, uint32_t e0, uint32_t lane,
-                                          const float* g, uint32_t h, bool moments, RowMoments& m) {
-  using T = typename S::T;
-  if constexpr (sizeof(T) == 4) {
-    T* b = out + e0 + lane;
-#pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-      const float v = encode_out<S, SCALED>(g[i], S::dir_word(h, i));
-      st<NT>(b + i * kRowRun, v);
-      if (moments) add_moment(m, i, v);
-    }
-  } else {
-    // Every store first; the outlet wavefront's moments afterwards from the packed words.
-    uint16_t* b = reinterpret_cast<uint16_t*>(out) + e0 + lane;
-    const uint16_t v0 = __builtin_bit_cast(uint16_t, encode_out<S, SCALED>(g[0], S::dir_word(h, 0)));
-    st<NT>(b, v0);
-    uint32_t words[kPairs];
-#pragma unroll
-    for (int q = 0; q < kPairs; ++q) {
-      const int i = 2 * q + 1;
-      const float scale = SCALED ? 1.0f : kOut<S::kScaleLog2>;
-      // The pair's first random word is dead after its conversion, so its register takes the pair.
-      const uint32_t ra = S::dir_word(h, i);
-      words[q] = S::encode_pair_scaled(ra, g[i] * scale, ra, g[i + 1] * scale, S::dir_word(h, i + 1));
-      st<NT>(b + i * kRowRun, (uint16_t)(words[q] & 0xffffu));
-      st<NT>(b + (i + 1) * kRowRun, (uint16_t)(words[q] >> 16));
-    }
-    if (moments) {
-      add_moment(m, 0, decode_bits<S>(v0));
-#pragma unroll
-      for (int q = 0; q < kPairs; ++q) {
-        add_moment(m, 2 * q + 1, decode_bits<S>(words[q] & 0xffffu));
-        add_moment(m, 2 * q + 2, decode_bits<S>(words[q] >> 16));
-      }
-    }
-  }
-}
-
-// stream_collide_cell for a row wavefront: the whole wavefront gathers (gather_row), then fluid
-// lanes replace wall-link directions and collide, EQ and wall lanes take their fixed states,
-// and every lane stores. Every load that does not depend on another (the pulls, the flags, the
-// outlet's upstream moments) is issued before any of them is waited for, so a wavefront waits
-// for memory once; a wavefront of walls only stores their rest state again like any other.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool BC>
-__device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __restrict__ in,
-                                                        typename S::T* __restrict__ out,
-                                                        const uint32_t* __restrict__ geo,
-                                                        float4* __restrict__ macro, const StepParams& p,
-                                                        uint32_t n, uint32_t seed) {
-  const uint32_t lane = threadIdx.x & 63u;
-  RowWave w;
-  w.n0 = __builtin_amdgcn_readfirstlane(n - lane);
-  cell_coords(w.n0, p, w.x0, w.y, w.z);
-  w.e0 = pop_x(p, w.x0) + pop_row(p, w.y + p.ny * w.z);
-  const uint32_t row = w.y + p.ny * w.z;
-  // The last wavefront of a row holds the outlet plane (lane 63) and its upstream cell (lane 62).
-  const bool last_wave = w.x0 + 64 == p.nx;
-  RowMoments up{1.0f, 0.0f, 0.0f, 0.0f};
-  if (last_wave) up = row_moments(edge_rows(in, p)[row]);
-  // The run's descriptor (a scalar load) tells plain runs, whose flags follow from the lane,
-  // from the few that touch a body and read every cell's flag.
-  const uint32_t desc = p.wave_desc ? p.wave_desc[w.n0 >> 6] : kWaveGeneral;
-  float g[kQ];
-  const uint32_t bc = BC ? p.bc : 0u;
-  gather_row<S, POL == kNtAll>(in, p, bc, w, lane, g);
-  uint32_t flag;
-  if (desc & kWaveGeneral)
-    flag = geo[n];
-  else
-    flag = ((lane == 0 && (desc & kWaveEqFirst)) || (lane == 63 && (desc & kWaveEqLast))) ? (uint32_t)kEq : 0u;
-  const uint32_t kind = flag & kTypeMask;
-  const bool outlet_wave = last_wave && (__builtin_amdgcn_readlane(kind, 63) == kEq);
-  if (BC && (p.bc & kBcFreeSlip) && on_side(w.y, w.z, p)) slip_row<S, POL == kNtAll>(in, p, w, lane, g);
-  const uint32_t x = w.x0 + lane;
-  if (kind == kFluid && (flag >> 3) != 0) {
-    const Neighbours nb(x, w.y, w.z, p, bc);
-#pragma unroll
-    for (int i = 1; i < kQ; ++i)
-      if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, w.e0 + lane, flag, i);
-  }
-  float4 m;
-  if constexpr (OP == kOpReg) {
-    // Every lane through one collision: EQ and wall lanes with prescribed moments.
-    const bool prescribed = kind != kFluid;
-    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-    if (kind == kEq) eq_velocity(p, x, up, ux, uy, uz);
-    m = relax_prescribed<OP, LES, kOutLog2<S, OP>>(p, g, flag, n, prescribed, ux, uy, uz);
-  } else if (kind == kFluid) {
-    m = relax<OP, LES>(p, g, flag, n);
-  } else if (kind == kEq) {
-    m = eq_from(p, x, up, g);
-  } else {
-    rest_state(g);
-    m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-  }
-  RowMoments stored{1.0f, 0.0f, 0.0f, 0.0f};
-  store_row<S, POL != kCached, (kOutLog2<S, OP> != 0)>(out, w.e0, lane, g, S::cell_word(n, seed), outlet_wave,
-                                                          stored);
-  if (outlet_wave && lane == 62) edge_rows(out, p)[row] = make_float4(stored.mass, stored.mx, stored.my, stored.mz);
-  if (MACRO && !is_wall(kind)) {
-    macro[n] = m;
-    guard_density(macro, p, m.x);
-  }
-}
-
-// Blocks are renumbered so that each of the 8 XCDs (which receive blocks round-robin) steps
-// contiguous chunks of `chunk` blocks: XCD k takes chunks k, k + 8, k + 16, ... The cache lines
-// that neighbouring rows and wavefronts share stay in one XCD's L2 within a chunk. One chunk per
-// XCD (an eighth of the lattice each) suits uniform lattices; smaller chunks spread a body's
-// wall cells, and their slower wavefronts, over all XCDs. The grid holds a multiple of 8 chunks.
-constexpr uint32_t kXcds = 8;
-
-// Waves per SIMD the step kernel is compiled for (a register budget of 512 / waves VGPRs).
-// Seven leaves REG + LES a few spills and measured fastest; TRT + LES keeps the populations,
-// their pair sums and the equilibrium live through the wall function and spills heavily
-// below about 106 registers, so it runs at four.
-template <int OP, bool LES>
-constexpr int kStepWaves = (LES && OP == kOpTrt) ? 4 : 7;
-
-// BC: the instance for boundary options (p.bc != 0); the reference's boundaries take BC = false,
-// which compiles the option code out of the step.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool BC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<OP, LES>))) void stream_collide_kernel(const typename S::T* __restrict__ in,
-                                                             typename S::T* __restrict__ out,
-                                                             const uint32_t* __restrict__ geo,
-                                                             float4* __restrict__ macro, StepParams p,
-                                                             uint32_t seed, uint32_t chunk) {
-  const uint32_t b = blockIdx.x;
-  const uint32_t local = b / kXcds;
-  const uint32_t logical = ((local / chunk) * kXcds + b % kXcds) * chunk + local % chunk;
-  const uint32_t n = logical * blockDim.x + threadIdx.x;
-  if (n >= p.cells) return;  // whole wavefronts when nx is a multiple of 64
-  if (p.rows)
-    stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL, BC>(in, out, geo, macro, p, n, seed);
-  else
-    stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL, true, BC>(in, out, geo, macro, p, n, seed);
 }
 
 }  // namespace pw

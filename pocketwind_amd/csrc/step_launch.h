@@ -5,7 +5,7 @@
 
 #include <type_traits>
 
-#include "step_kernels.h"
+#include "step_row.h"
 
 namespace pw {
 
@@ -26,6 +26,21 @@ void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const St
   // without their code.
   auto k = p.bc ? (macro ? pick(std::true_type{}, std::true_type{}) : pick(std::false_type{}, std::true_type{}))
                 : (macro ? pick(std::true_type{}, std::false_type{}) : pick(std::false_type{}, std::false_type{}));
+  // The learned closure evaluated in the step: its own instances (REG + LES, the reference's
+  // boundaries, row path; launch_closure sets p.closure_net only for those).
+  if constexpr (OP == kOpReg && LES) {
+    if (p.closure_net) {
+      auto pick_c = [&](auto with_macro) {
+        constexpr bool M = decltype(with_macro)::value;
+        switch (s->kernel) {
+          case PW_KERNEL_CELL_NT: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtAll, false, true>;
+          case PW_KERNEL_CELL_NT_STORES: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtStores, false, true>;
+          default: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kCached, false, true>;
+        }
+      };
+      k = macro ? pick_c(std::true_type{}) : pick_c(std::false_type{});
+    }
+  }
   const uint32_t block = s->block;
   const uint64_t blocks = (s->cells + block - 1) / block;
   // Each XCD takes s->xcd_chunks chunks; the grid is rounded up to whole chunks (the extra
@@ -33,7 +48,8 @@ void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const St
   const uint32_t parts = s->xcd_chunks;
   const uint32_t per_xcd = (uint32_t)(((blocks + kXcds - 1) / kXcds + parts - 1) / parts * parts);
   const uint32_t chunk = per_xcd / parts;
-  hipLaunchKernelGGL(k, dim3(kXcds * per_xcd), dim3(block), 0, s->stream, (const typename S::T*)in,
+  const uint32_t lds = p.closure_net ? block / 64 * closure::kFusedWaveBytes : 0;
+  hipLaunchKernelGGL(k, dim3(kXcds * per_xcd), dim3(block), lds, s->stream, (const typename S::T*)in,
                      (typename S::T*)out, s->geometry, s->macro, p, seed, chunk);
 }
 

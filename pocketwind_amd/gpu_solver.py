@@ -294,6 +294,15 @@ class GpuSolver:
         norm = np.ascontiguousarray(norm, np.float32)
         N.check(N.lib().pw_solver_set_closure(self._live(), net._h, norm.ctypes.data, norm.size))
 
+    PLACEMENTS = ("kernel", "step")
+
+    def set_closure_placement(self, placement: str) -> None:
+        """Where the closure runs: "kernel" (its own kernel, every `interval` steps; the default)
+        or "step" (inside the step's wall wavefronts, every step; REG + LES, nx a multiple of 64)."""
+        if placement not in self.PLACEMENTS:
+            raise ValueError(f"Unknown closure placement {placement!r}.")
+        N.check(N.lib().pw_solver_set_closure_placement(self._live(), self.PLACEMENTS.index(placement)))
+
     def prescribe_wall_stress(self, stress, valid) -> None:
         """Test hook: the wall collision uses these stress vectors (n_wall x 3, wall-cell order)
         where `valid`, the wall law elsewhere, in place of a network's output."""
