@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-#define PW_ABI_VERSION 7
+#define PW_ABI_VERSION 8
 
 /* Errors (src/lbm/gpu-resources.ts:3-27, gpu-force.ts:192-205). EINVAL is the reference's
  * RangeError, ENOMEM its AllocationFailure, EBUSY its 'busy' readback. */
@@ -73,6 +73,13 @@ typedef struct {
   uint32_t wall_model; /* pw_wall_model */
   uint32_t sides;      /* pw_sides; no reference counterpart (the reference is periodic) */
   uint32_t outlet;     /* pw_outlet; no reference counterpart (the reference's is equilibrium) */
+  /* Smallest tangential slip speed at which the wall law prescribes its shear; below it a wall
+   * cell takes the bulk collision (stream_collide.wgsl:40 uses 1e-7). 0 selects the default:
+   * 1e-7 with f32 storage; 1e-4 with f16 or bf16 storage, whose stored velocities carry
+   * rounding noise of about 1e-5 (f16), so that at a stagnation point, where the slip vanishes,
+   * the noise does not switch the cell to the law's shear along a random direction. Finite and
+   * non-negative. No reference counterpart (the reference fixes 1e-7). */
+  float slip_floor;
 } pw_params;
 
 /* ForceSample (src/lbm/gpu-force.ts:11); components accumulated and returned in FP64. */

@@ -23,7 +23,7 @@ class Params(C.Structure):
     _fields_ = [("tau0", C.c_double), ("u_in", C.c_double), ("les", C.c_uint32),
                 ("op", C.c_uint32), ("walls", C.c_uint32), ("storage", C.c_uint32),
                 ("ramp_steps", C.c_uint32), ("wall_model", C.c_uint32), ("sides", C.c_uint32),
-                ("outlet", C.c_uint32)]
+                ("outlet", C.c_uint32), ("slip_floor", C.c_float)]
 
 
 class ForceSample(C.Structure):

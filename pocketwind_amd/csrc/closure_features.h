@@ -127,7 +127,7 @@ __device__ __forceinline__ void features_of(const typename S::T* __restrict__ in
   }
   const float speed = sqrtf(sx * sx + sy * sy + sz * sz);
   law[0] = law[1] = law[2] = 0.0f;
-  if (speed >= 1e-7f) {
+  if (speed >= p.slip_floor) {
     const float ut = friction_velocity(p.nu, speed);
     const float st = rho * ut * ut / speed;
     law[0] = st * sx;

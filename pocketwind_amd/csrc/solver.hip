@@ -118,6 +118,10 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
     set_error("Relaxation time must exceed 0.5 and inlet speed must be finite.");
     return false;
   }
+  if (!std::isfinite(prm.slip_floor) || prm.slip_floor < 0.0f) {
+    set_error("The slip floor must be finite and non-negative.");
+    return false;
+  }
   out->nx = L.nx;
   out->ny = L.ny;
   out->nz = L.nz;
@@ -129,6 +133,7 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
   out->lambda = 3.0f / 16.0f;
   out->inv_tau0 = 1.0f / out->tau0;
   out->wm0 = 1.0f / (out->lambda / (out->tau0 - 0.5f) + 0.5f);
+  out->slip_floor = prm.slip_floor > 0.0f ? prm.slip_floor : prm.storage == PW_STORAGE_F32 ? 1e-7f : 1e-4f;
   out->run = pop_run(L.nx);
   out->x_tile = out->run == 64 ? 18u * 64u : 0u;
   out->div_nx = make_fast_div(L.nx);

@@ -64,6 +64,7 @@ struct StepParams {
   float lambda;    // TRT magic parameter 3/16
   float inv_tau0;  // 1 / tau0 in f32: the relaxation rate without LES
   float wm0;       // TRT odd rate 1 / (lambda / (tau0 - 1/2) + 1/2) in f32
+  float slip_floor;  // smallest slip speed the wall law acts on (pw_params::slip_floor resolved)
   FastDiv div_nx, div_ny;
   uint32_t bc;  // kBcFreeSlip | kBcZeroGradient, or 0 for the reference's periodic y/z sides
   uint32_t rows;  // 1: one wavefront per 64-cell x-row (nx a multiple of 64); 0: one cell per lane

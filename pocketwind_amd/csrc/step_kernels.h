@@ -286,7 +286,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
     sz -= d * az;
   }
   const float speed = sqrtf(sx * sx + sy * sy + sz * sz);
-  if (speed < 1e-7f) {
+  if (speed < p.slip_floor) {  // stream_collide.wgsl:40 (1e-7 with f32 storage)
     collide<OP, true, SC>(p, g, c);
     return;
   }

@@ -61,6 +61,9 @@ class SolverParams:
     # its outlet is the equilibrium outlet): "free-slip" sides, a "zero-gradient" outlet.
     sides: str = "periodic"
     outlet: str = "equilibrium"
+    # Smallest slip speed the wall law acts on; 0: 1e-7 for f32 storage (the reference's), 1e-4
+    # for 16-bit storage, whose velocity noise would otherwise switch stagnation cells to the law.
+    slip_floor: float = 0.0
 
 
 @dataclass(frozen=True)
@@ -92,7 +95,7 @@ def _pack(params: SolverParams) -> N.Params:
     return N.Params(float(params.tau0), float(params.u_in), int(bool(params.les)),
                     _OPERATORS[params.operator], _WALLS[params.walls], _STORAGE[params.precision],
                     int(params.ramp_steps), _WALL_MODELS[params.wall_model], _SIDES[params.sides],
-                    _OUTLETS[params.outlet])
+                    _OUTLETS[params.outlet], float(params.slip_floor))
 
 
 class GpuSolver:

@@ -47,7 +47,7 @@ class GpuSolver {
    * src/lbm/gpu-solver.ts:158-335. `grid` carries `geometry` (Uint32Array, the packed
    * [flags N][q offsets N][f32 q per link]) and `boundaryCells` (Uint32Array); `params` is
    * SolverParams {tau0, uIn, les, rampSteps?, walls?, operator?, precision?} plus this library's
-   * wallModel, sides and outlet. Resolves to the solver, or to an AllocationFailure; invalid
+   * wallModel, sides, outlet and slipFloor. Resolves to the solver, or to an AllocationFailure; invalid
    * input throws a RangeError.
    */
   static async create(layout, grid, params) {

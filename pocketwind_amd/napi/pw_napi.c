@@ -125,7 +125,7 @@ static int read_params(napi_env env, napi_value obj, pw_params *p) {
   static const char *const outlets[] = {"equilibrium", "zero-gradient"};
   memset(p, 0, sizeof *p);
   int les = 0;
-  double ramp = 0;
+  double ramp = 0, slip_floor = 0;
   if (!number_prop(env, obj, "tau0", -1, &p->tau0) || !number_prop(env, obj, "uIn", 0, &p->u_in) ||
       !bool_prop(env, obj, "les", &les) || !choice_prop(env, obj, "operator", ops, 2, 0, &p->op) ||
       !choice_prop(env, obj, "walls", walls, 2, 0, &p->walls) ||
@@ -133,10 +133,12 @@ static int read_params(napi_env env, napi_value obj, pw_params *p) {
       !choice_prop(env, obj, "wallModel", models, 2, 0, &p->wall_model) ||
       !choice_prop(env, obj, "sides", sides, 2, 0, &p->sides) ||
       !choice_prop(env, obj, "outlet", outlets, 2, 0, &p->outlet) || !number_prop(env, obj, "rampSteps", 0, &ramp) ||
+      !number_prop(env, obj, "slipFloor", 0, &slip_floor) ||
       !(ramp >= 0) || ramp > 4294967295.0 || ramp != (double)(uint64_t)ramp)
     return 0;
   p->les = (uint32_t)les;
   p->ramp_steps = (uint32_t)ramp;
+  p->slip_floor = (float)slip_floor;
   return 1;
 }
 

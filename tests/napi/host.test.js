@@ -20,7 +20,7 @@ function check(name, fn) {
   checks.push([name, fn]);
 }
 
-check('abi version 7', () => assert.strictEqual(pw.abiVersion(), 7));
+check('abi version 8', () => assert.strictEqual(pw.abiVersion(), 8));
 
 check('device count is a non-negative integer', () => {
   const n = pw.deviceCount();

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) <= set(N.SIGNATURES), set(names) - set(N.SIGNATURES)
-    assert lib.pw_abi_version() == 7
+    assert lib.pw_abi_version() == 8
 
 
 def _create(grid, params, boundary=None):
