@@ -205,7 +205,8 @@ typedef enum {
 pw_status pw_solver_set_kernel(pw_solver *s, uint32_t kernel);
 uint32_t pw_solver_kernel(const pw_solver *s);
 /* An explicit step variant: kernel PW_KERNEL_CELL .. PW_KERNEL_CELL_NT_STORES, `block` threads
- * per block (64, 128 or 256) and `xcd_chunks` (1 or 16) chunks per XCD, without timing. */
+ * per block (64, 128 or 256) and `xcd_chunks` chunks per XCD (a power of two up to 1024),
+ * without timing. */
 pw_status pw_solver_set_step_variant(pw_solver *s, uint32_t kernel, uint32_t block,
                                      uint32_t xcd_chunks);
 /* Test hook: with `on`, step with one cell per lane even when nx is a multiple of 64 (whose
@@ -224,8 +225,9 @@ typedef enum { PW_ROUNDING_HARDWARE = 0, PW_ROUNDING_REFERENCE = 1 } pw_rounding
 pw_status pw_solver_set_rounding(pw_solver *s, uint32_t rounding);
 /* Threads per block of the step launches. */
 uint32_t pw_solver_block_size(const pw_solver *s);
-/* Chunks of the lattice each of the 8 XCDs steps (1: one contiguous eighth each; 16: chunks
- * spread over the lattice), chosen with the variant by AUTO. */
+/* Chunks of the lattice each of the 8 XCDs steps (1: one contiguous eighth each; more: chunks
+ * spread over the lattice, XCD k taking chunks k, k + 8, ...), chosen with the variant by AUTO
+ * among 1, 16, 64 and 256. */
 uint32_t pw_solver_xcd_chunks(const pw_solver *s);
 
 /* The validation stepper stepWithForces (src/app/case-run.ts:154-169) without a host round
@@ -328,8 +330,8 @@ pw_status pw_batch_create(uint32_t count, const pw_layout *layouts, const uint32
                           const size_t *n_boundary, const double *tau0, const double *u_in,
                           const pw_params *shared, pw_batch **out);
 pw_status pw_batch_step(pw_batch *b, uint32_t count, int write_macro_on_last);
-/* The step variant the batch chose at create (timing each cache policy with 1 and 16 chunks
- * per XCD, or PW_BATCH_KERNEL=<cell|cell-nt|cell-nt-stores>/<chunks>): a pw_kernel value, and
+/* The step variant the batch chose at create (timing each cache policy with 1, 16, 64 and 256
+ * chunks per XCD, or PW_BATCH_KERNEL=<cell|cell-nt|cell-nt-stores>/<chunks>): a pw_kernel value, and
  * the chunks per XCD in *xcd_chunks (may be NULL). */
 uint32_t pw_batch_step_variant(const pw_batch *b, uint32_t *xcd_chunks);
 /* As pw_solver_step_sampled for every tunnel: `out` receives [tunnel][sample] x

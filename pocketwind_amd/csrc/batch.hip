@@ -270,17 +270,16 @@ bool forced_batch_variant(pw_batch* b, const char* text) {
   else return false;
   uint32_t chunks = b->xcd_chunks;
   if (a != std::string::npos) chunks = (uint32_t)std::strtoul(v.c_str() + a + 1, nullptr, 10);
-  if (chunks != 1 && chunks != 16) return false;
+  if (!valid_xcd_chunks(chunks)) return false;
   b->policy = policy;
   b->xcd_chunks = chunks;
   return true;
 }
 
-// Times the three cache policies with 1 and 16 chunks per XCD (two runs of a few steps each,
-// the faster counting) and keeps the fastest.
+// Times the three cache policies with each chunking of kXcdChunkings that leaves every chunk
+// two tiles or more (two runs of a few steps each, the faster counting) and keeps the fastest.
 hipError_t tune_batch(pw_batch* b) {
   const uint32_t policies[3] = {kCached, kNtAll, kNtStores};
-  const uint32_t chunkings[2] = {1, 16};
   const uint64_t cells = [&] {
     uint64_t n = 0;
     for (const auto& t : b->host) n += t.p.cells;
@@ -289,8 +288,9 @@ hipError_t tune_batch(pw_batch* b) {
   const int reps = cells >= (1u << 22) ? 4 : 16;
   float best = 0;
   uint32_t best_policy = kNtStores, best_chunks = 16;
-  for (uint32_t chunks : chunkings)
+  for (uint32_t chunks : kXcdChunkings)
     for (uint32_t pol : policies) {
+      if (chunks > 1 && (uint64_t)chunks * 2 * kXcds > b->n_tiles) continue;
       b->policy = pol;
       b->xcd_chunks = chunks;
       b->step_grid = batch_grid(b->n_tiles, chunks, &b->chunk);

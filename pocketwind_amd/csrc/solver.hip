@@ -324,7 +324,7 @@ bool forced_variant(pw_solver* s, const char* text) {
   uint32_t block = s->block, chunks = s->xcd_chunks;
   if (a != std::string::npos) block = (uint32_t)std::strtoul(v.c_str() + a + 1, nullptr, 10);
   if (b != std::string::npos) chunks = (uint32_t)std::strtoul(v.c_str() + b + 1, nullptr, 10);
-  if ((block != 64 && block != 128 && block != 256) || (chunks != 1 && chunks != 16)) return false;
+  if ((block != 64 && block != 128 && block != 256) || !valid_xcd_chunks(chunks)) return false;
   s->kernel = kernel;
   s->block = block;
   s->xcd_chunks = chunks;
@@ -334,15 +334,16 @@ bool forced_variant(pw_solver* s, const char* text) {
 hipError_t autotune(pw_solver* s) {
   const uint32_t kernels[3] = {PW_KERNEL_CELL, PW_KERNEL_CELL_NT, PW_KERNEL_CELL_NT_STORES};
   const uint32_t blocks[3] = {64, 128, 256};
-  const uint32_t chunkings[2] = {1, 16};
   // Each candidate is timed twice over `reps` steps and keeps its faster run, so one slow
   // sample does not decide.
   const int reps = s->cells >= (1u << 20) ? 12 : 2;
   float best = 0;
   uint32_t choice = PW_KERNEL_CELL_NT, choice_block = 256, choice_chunks = 1;
-  for (uint32_t parts : chunkings)
+  for (uint32_t parts : kXcdChunkings)
     for (uint32_t k : kernels)
       for (uint32_t b : blocks) {
+        // A chunking with fewer than two blocks per chunk would only add empty blocks.
+        if (parts > 1 && (uint64_t)parts * 2 * kXcds * b > s->cells) continue;
         s->kernel = k;
         s->block = b;
         s->xcd_chunks = parts;
@@ -850,8 +851,8 @@ pw_status pw_debug_cell_path(pw_solver* s, int on) {
 pw_status pw_solver_set_step_variant(pw_solver* s, uint32_t kernel, uint32_t block, uint32_t xcd_chunks) {
   if (pw_status st = check_handle(s)) return st;
   if (kernel < PW_KERNEL_CELL || kernel > PW_KERNEL_CELL_NT_STORES || (block != 64 && block != 128 && block != 256) ||
-      (xcd_chunks != 1 && xcd_chunks != 16)) {
-    set_error("A step variant needs a kernel 1-3, a block of 64, 128 or 256 and 1 or 16 XCD chunks.");
+      !valid_xcd_chunks(xcd_chunks)) {
+    set_error("A step variant needs a kernel 1-3, a block of 64, 128 or 256 and a power of two up to 1024 XCD chunks.");
     return PW_EINVAL;
   }
   s->kernel = kernel;

@@ -173,6 +173,6 @@ struct pw_solver {
   uint32_t* guard_host = nullptr;       // pinned readback
   uint32_t kernel = 1;  // pw_kernel actually used by step()
   uint32_t block = 256;  // threads per step block
-  uint32_t xcd_chunks = 1;  // chunks of the lattice per XCD in the step launch (1 or 16)
+  uint32_t xcd_chunks = 1;  // chunks of the lattice per XCD in the step launch (valid_xcd_chunks)
   uint32_t rounding = 0;    // pw_rounding of f16 stores
 };

@@ -225,7 +225,16 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
 // that neighbouring rows and wavefronts share stay in one XCD's L2 within a chunk. One chunk per
 // XCD (an eighth of the lattice each) suits uniform lattices; smaller chunks spread a body's
 // wall cells, and their slower wavefronts, over all XCDs. The grid holds a multiple of 8 chunks.
+// A row-wavefront pull copy with no arithmetic (f16, 256x128x128, 128 threads per block) took
+// 58.8 us per step with 1 chunk per XCD, 55.5 us with 16, 53.8 us with 64 and 53.5 us with 256
+// (16 blocks per chunk); the autotuner times the chunkings in kXcdChunkings.
 constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdChunkings[] = {1, 16, 64, 256};
+constexpr uint32_t kMaxXcdChunks = 1024;
+// Chunks per XCD a step launch accepts: a power of two up to kMaxXcdChunks.
+__host__ __device__ constexpr bool valid_xcd_chunks(uint32_t c) {
+  return c >= 1 && c <= kMaxXcdChunks && (c & (c - 1)) == 0;
+}
 
 // Waves per SIMD the step kernel is compiled for (a register budget of 512 / waves VGPRs).
 // Seven leaves REG + LES a few spills and measured fastest; TRT + LES keeps the populations,

@@ -257,7 +257,7 @@ class GpuSolver:
 
     def set_variant(self, kernel: str, block: int, xcd_chunks: int) -> None:
         """An explicit step variant: population cache policy, threads per block (64, 128,
-        256) and chunks per XCD (1, 16), without timing."""
+        256) and chunks per XCD (a power of two up to 1024), without timing."""
         N.check(N.lib().pw_solver_set_step_variant(self._live(), self.KERNELS[kernel], block, xcd_chunks))
 
     @property

@@ -147,7 +147,7 @@ def run_f32(operator, les, walls, kernel, block, chunks):
         diag.destroy()
 
 
-@pytest.mark.parametrize("chunks", [1, 16])
+@pytest.mark.parametrize("chunks", [1, 16, 256])
 @pytest.mark.parametrize("block", [64, 128, 256])
 @pytest.mark.parametrize("kernel", POLICIES)
 def test_row_path_reg_les_matches_oracle(kernel, block, chunks):
