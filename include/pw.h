@@ -407,8 +407,18 @@ pw_status pw_solver_set_closure_interval(pw_solver *s, uint32_t every);
  * solvers keep the kernel. Both give the same stresses bit for bit. In-step evaluation saves
  * the launch and the stress buffer but lengthens the wall wavefronts by the features' gathers
  * and the network, and measures slower than the kernel every step on the 256x128x128 bluff
- * body. No reference counterpart. */
-typedef enum { PW_CLOSURE_KERNEL = 0, PW_CLOSURE_IN_STEP = 1 } pw_closure_placement;
+ * body. OVERLAPPED: the closure kernel of a step k due an evaluation (k a multiple of `every`)
+ * runs on a second stream beside step k, reading the same state; step k still uses the previous
+ * output and steps k + 1 .. k + every use the new one, so each output is one step older when
+ * used than under KERNEL (the first step after attaching or reset evaluates in line, and its
+ * output also serves the step that starts the first overlapped evaluation). The latency-bound
+ * closure kernel then runs within the HBM-bound step instead of before it. Each step call joins
+ * the evaluation it started before it returns. No reference counterpart. */
+typedef enum {
+  PW_CLOSURE_KERNEL = 0,
+  PW_CLOSURE_IN_STEP = 1,
+  PW_CLOSURE_OVERLAPPED = 2
+} pw_closure_placement;
 pw_status pw_solver_set_closure_placement(pw_solver *s, uint32_t placement);
 /* Test hook: the wall collision takes the given stress vectors (x, y, z, w per wall cell, in
  * the order of pw_solver_wall_cells; w > 0 uses the vector, else the wall law) in place of the

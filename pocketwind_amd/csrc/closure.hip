@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "closure.h"
@@ -21,23 +22,29 @@ namespace {
 inline unsigned closure_blocks(size_t n) { return (unsigned)((n + closure::kCellsPerWave - 1) / closure::kCellsPerWave); }
 
 template <class S>
-void launch_closure_s(pw_solver* s, const void* state, const void* net, float4* tau, float* records) {
+void launch_closure_s(pw_solver* s, hipStream_t stream, const void* state, const void* net, float4* tau,
+                      float* records) {
   Norm norm;
   std::memcpy(&norm, s->closure_norm, sizeof norm);
   if (!records && norm.u_ref == 0.0f) norm.u_ref = 1.0f;
   const bool bz = s->params.walls == PW_WALLS_BOUZIDI;
   auto k = bz ? closure::closure_kernel<S, true> : closure::closure_kernel<S, false>;
-  hipLaunchKernelGGL(k, dim3(closure_blocks(s->n_wall)), dim3(64), 0, s->stream, (const typename S::T*)state,
+  hipLaunchKernelGGL(k, dim3(closure_blocks(s->n_wall)), dim3(64), 0, stream, (const typename S::T*)state,
                      s->geometry, s->sp, s->wall_cells, (uint32_t)s->n_wall, (const mlp::Packed*)net, norm, tau,
                      records);
 }
 
-void launch_closure_any(pw_solver* s, const void* state, const void* net, float4* tau, float* records) {
+void launch_closure_on(pw_solver* s, hipStream_t stream, const void* state, const void* net, float4* tau,
+                       float* records) {
   switch (s->params.storage) {
-    case PW_STORAGE_F16: launch_closure_s<StoreF16>(s, state, net, tau, records); break;
-    case PW_STORAGE_BF16: launch_closure_s<StoreBF16>(s, state, net, tau, records); break;
-    default: launch_closure_s<StoreF32>(s, state, net, tau, records); break;
+    case PW_STORAGE_F16: launch_closure_s<StoreF16>(s, stream, state, net, tau, records); break;
+    case PW_STORAGE_BF16: launch_closure_s<StoreBF16>(s, stream, state, net, tau, records); break;
+    default: launch_closure_s<StoreF32>(s, stream, state, net, tau, records); break;
   }
+}
+
+void launch_closure_any(pw_solver* s, const void* state, const void* net, float4* tau, float* records) {
+  launch_closure_on(s, s->stream, state, net, tau, records);
 }
 
 template <class S>
@@ -97,12 +104,35 @@ void launch_closure(pw_solver* s, StepParams& p) {
     return;
   }
   // closure_on 2: the output was prescribed from the host (pw_debug_set_wall_stress).
-  if (s->closure_on == 1 && (!s->closure_fresh || s->steps % s->closure_every == 0)) {
+  if (s->closure_on == 1 && s->closure_placement == PW_CLOSURE_OVERLAPPED) {
+    // The evaluation started beside the previous step is joined before this one; the first
+    // step of a run evaluates in line. An evaluation due at this step reads the state this step
+    // reads, on the side stream, while this step runs on the stream with the previous output;
+    // the step after it takes the new output (settle_closure).
+    settle_closure(s);
+    if (!s->closure_fresh) {
+      launch_closure_any(s, s->pops[s->current], s->closure_net, s->closure_tau, nullptr);
+      s->closure_fresh = 1;
+    } else if (s->steps % s->closure_every == 0) {
+      (void)hipEventRecord(s->state_ready, s->stream);
+      (void)hipStreamWaitEvent(s->side_stream, s->state_ready, 0);
+      launch_closure_on(s, s->side_stream, s->pops[s->current], s->closure_net, s->closure_next, nullptr);
+      (void)hipEventRecord(s->closure_done, s->side_stream);
+      s->closure_pending = 1;
+    }
+  } else if (s->closure_on == 1 && (!s->closure_fresh || s->steps % s->closure_every == 0)) {
     launch_closure_any(s, s->pops[s->current], s->closure_net, s->closure_tau, nullptr);
     s->closure_fresh = 1;
   }
   p.closure_tau = s->closure_tau;
   p.closure_slot = s->closure_slot;
+}
+
+void settle_closure(pw_solver* s) {
+  if (!s->closure_pending) return;
+  (void)hipStreamWaitEvent(s->stream, s->closure_done, 0);
+  std::swap(s->closure_tau, s->closure_next);
+  s->closure_pending = 0;
 }
 
 hipError_t closure_tile_order(pw_solver* s) {
@@ -255,7 +285,7 @@ pw_status pw_solver_set_closure(pw_solver* s, const pw_mlp* net, const float* no
 
 pw_status pw_solver_set_closure_placement(pw_solver* s, uint32_t placement) {
   if (pw_status st = check(s)) return st;
-  if (placement > PW_CLOSURE_IN_STEP) {
+  if (placement > PW_CLOSURE_OVERLAPPED) {
     set_error("Unknown closure placement.");
     return PW_EINVAL;
   }

@@ -388,7 +388,12 @@ void free_all(pw_solver* s) {
   if (s->wave_desc) hipFree(s->wave_desc);
   if (s->tile_order) hipFree(s->tile_order);
   if (s->closure_slot) hipFree(s->closure_slot);
+  if (s->side_stream) hipStreamSynchronize(s->side_stream);
   if (s->closure_tau) hipFree(s->closure_tau);
+  if (s->closure_next) hipFree(s->closure_next);
+  if (s->state_ready) hipEventDestroy(s->state_ready);
+  if (s->closure_done) hipEventDestroy(s->closure_done);
+  if (s->side_stream) hipStreamDestroy(s->side_stream);
   if (s->closure_net) hipFree(s->closure_net);
   if (s->macro) hipFree(s->macro);
   if (s->partials) hipFree(s->partials);
@@ -585,8 +590,13 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
     for (size_t k = 0; k < wall_list.size(); ++k) slot[wall_list[k]] = (uint32_t)k;
     if ((e = hipMalloc((void**)&s->closure_slot, N * 4)) != hipSuccess ||
         (e = hipMalloc((void**)&s->closure_tau, (s->n_wall ? s->n_wall : 1) * sizeof(float4))) != hipSuccess ||
+        (e = hipMalloc((void**)&s->closure_next, (s->n_wall ? s->n_wall : 1) * sizeof(float4))) != hipSuccess ||
         (e = hipMalloc(&s->closure_net, closure::kNetBlockBytes)) != hipSuccess)
       return fail(device_error(e, "closure allocation"));
+    if ((e = hipStreamCreateWithFlags(&s->side_stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&s->state_ready, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&s->closure_done, hipEventDisableTiming)) != hipSuccess)
+      return fail(device_error(e, "closure stream"));
     if ((e = hipMemcpy(s->closure_slot, slot.data(), N * 4, hipMemcpyHostToDevice)) != hipSuccess)
       return fail(device_error(e, "closure upload"));
   }
@@ -667,6 +677,9 @@ pw_status step_run(pw_solver* s, uint32_t count, int write_macro_on_last) {
     s->current ^= 1;
     s->steps += 1;
   }
+  // An overlapped closure evaluation ends with the call that started it, so every other entry
+  // point finds the closure joined.
+  settle_closure(s);
   if (count) PW_TRY(hipGetLastError(), "step launch");
   return PW_OK;
 }

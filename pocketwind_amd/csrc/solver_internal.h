@@ -111,6 +111,9 @@ namespace pw {
 const void* mlp_packed(const pw_mlp* m);
 size_t mlp_packed_bytes();
 void launch_closure(pw_solver* s, StepParams& p);
+// Joins an overlapped closure evaluation still running beside the last step: the solver's
+// stream waits for it and its output becomes the one the next step reads.
+void settle_closure(pw_solver* s);
 // Build the step's block order for the closure evaluated in the step (blocks that hold wall
 // cells first, so their longer wavefronts start with the launch rather than in its middle)
 // for the solver's current block size and XCD chunking. Allocates; called when a network is
@@ -167,6 +170,12 @@ struct pw_solver {
   uint32_t closure_every = 1;  // steps between closure evaluations
   int closure_fresh = 0;       // closure_tau holds an evaluation of this run
   uint32_t closure_placement = 0;  // pw_closure_placement
+  // PW_CLOSURE_OVERLAPPED: the evaluation runs on side_stream beside the step that reads the
+  // same state, into closure_next, which replaces closure_tau before the following step.
+  hipStream_t side_stream = nullptr;
+  hipEvent_t state_ready = nullptr, closure_done = nullptr;
+  float4* closure_next = nullptr;
+  int closure_pending = 0;  // closure_next is being written; the main stream has not joined it
   uint32_t* tile_order = nullptr;  // step block order for the closure in the step (closure_tile_order)
   uint32_t order_block = 0, order_chunks = 0;  // the variant tile_order was built for
   uint32_t* guard_flag = nullptr;       // the word after the macros (not a separate allocation)

@@ -297,11 +297,13 @@ class GpuSolver:
         norm = np.ascontiguousarray(norm, np.float32)
         N.check(N.lib().pw_solver_set_closure(self._live(), net._h, norm.ctypes.data, norm.size))
 
-    PLACEMENTS = ("kernel", "step")
+    PLACEMENTS = ("kernel", "step", "overlapped")
 
     def set_closure_placement(self, placement: str) -> None:
-        """Where the closure runs: "kernel" (its own kernel, every `interval` steps; the default)
-        or "step" (inside the step's wall wavefronts, every step; REG + LES, nx a multiple of 64)."""
+        """Where the closure runs: "kernel" (its own kernel, every `interval` steps; the default),
+        "step" (inside the step's wall wavefronts, every step; REG + LES, nx a multiple of 64) or
+        "overlapped" (its kernel on a second stream beside the step that reads the same state;
+        the output serves the `interval` steps after that one)."""
         if placement not in self.PLACEMENTS:
             raise ValueError(f"Unknown closure placement {placement!r}.")
         N.check(N.lib().pw_solver_set_closure_placement(self._live(), self.PLACEMENTS.index(placement)))
