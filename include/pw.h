@@ -420,10 +420,15 @@ typedef enum {
   PW_CLOSURE_OVERLAPPED = 2
 } pw_closure_placement;
 pw_status pw_solver_set_closure_placement(pw_solver *s, uint32_t placement);
-/* Test hook: the wall collision takes the given stress vectors (x, y, z, w per wall cell, in
- * the order of pw_solver_wall_cells; w > 0 uses the vector, else the wall law) in place of the
- * network's, until pw_solver_set_closure attaches a network again. Used to measure what a
- * perfect closure output would give. Learned wall model only. No reference counterpart. */
+/* Test hook: the wall collision takes the given vectors (x, y, z, w per wall cell, in the
+ * order of pw_solver_wall_cells) in place of the network's output, until pw_solver_set_closure
+ * attaches a network again: w = 1 a wall shear stress used in the wall law's place; w = 2 a
+ * momentum source: the cell keeps the wall law and, after each step, its stored post-collision
+ * populations gain 3 w_i c_i . v, so it gains v each step and no mass (f32 storage only; the
+ * momentum exchange does not count it, so the body's force is the sampled force minus the sum
+ * of the sources); w = 0 the wall law. Used to
+ * measure what a perfect closure output would give. Learned wall model only. No reference
+ * counterpart. */
 pw_status pw_debug_set_wall_stress(pw_solver *s, const float *stress, size_t n_floats);
 /* The last closure evaluation: 4 floats per wall cell, stress xyz and 1 (used) or 0 (wall
  * law). */

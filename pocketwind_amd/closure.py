@@ -318,6 +318,55 @@ def drag_split(forces, normals, frontal_cells: float, u_in: float) -> dict:
             "friction": float((f - normal)[:, 0].sum() * scale)}
 
 
+def _floor_run(grid_layout, grid_length, reynolds, u_in, flow_throughs, average, snapshots, prescribed=None,
+               source=False, log=print):
+    """The slanted bluff body on `grid_layout` with the wall law, or with `prescribed` = (vectors,
+    valid) as the closure's output (shear stresses, or momentum sources with `source`): the
+    window-mean Cd of the body force (for sources, the sampled momentum-exchange force minus the
+    sum of the sources), the time-averaged per-wall-cell momentum-exchange forces over the
+    window, and the wall cells' normals and exposed weights."""
+    from . import bodies, drag
+    from .gpu_solver import GpuSolver, SolverParams
+    from .voxelize import voxelize
+    body, _ = bodies.place_mesh(bodies.bluff_body(), None, grid_length, grid_layout, False)
+    grid = voxelize(body, None, grid_layout, False, link_fractions=True)
+    nu = u_in * grid_length / reynolds
+    s = GpuSolver.create(grid_layout, grid, SolverParams(3 * nu + 0.5, u_in, True, operator="reg",
+                                                         wall_model="law" if prescribed is None else "learned"))
+    try:
+        sources = np.zeros(3)
+        if prescribed is not None:
+            s.prescribe_wall_stress(*prescribed, source=source)
+            if source:
+                vec = np.asarray(prescribed[0], np.float32).astype(np.float64)
+                sources = vec[np.asarray(prescribed[1], bool)].sum(0)
+        flow = drag.flow_through_steps(grid.nx, u_in)
+        total, window = flow_throughs * flow, average * flow
+        samples = [s.step_sampled(total - window, 8)]
+        chunk = max(8, (window // snapshots) // 8 * 8)
+        forces, done = None, total - window
+        while done < total:
+            k = min(chunk, total - done)
+            samples.append(s.step_sampled(k, 8))
+            done += k
+            f = s.wall_forces()
+            forces = f if forces is None else forces + f
+        forces /= -(-window // chunk)
+        fx, _, _ = drag.window_mean_force(np.concatenate(samples), total, window)
+        scale = drag.cd_per_lattice_force(grid.frontal_cells, u_in)
+        rec = s.closure_features(u_in)
+        out = dict(cells=s.wall_cells(), normals=rec[:, 8:11].astype(np.float64),
+                   exposed=rec[:, 11].astype(np.float64), forces=forces, frontal=grid.frontal_cells,
+                   cd=(fx - sources[0]) * scale, cd_exchange=fx * scale, cd_sources=-sources[0] * scale,
+                   split=drag_split(forces, rec[:, 8:11], grid.frontal_cells, u_in))
+    finally:
+        s.destroy()
+    what = "wall law" if prescribed is None else ("prescribed momentum sources" if source else "prescribed shear")
+    log(f"  {grid_layout} {what}: Cd {out['cd']:.6f} (exchange {out['cd_exchange']:.6f}, sources "
+        f"{out['cd_sources']:+.6f}; time-averaged wall forces: {out['split']})")
+    return out
+
+
 def shear_floor(reynolds: float = 320.0, layout=(256, 128, 128), length: int = 64, u_in: float = 0.05,
                 flow_throughs: int = 6, average: int = 2, snapshots: int = 40, log=print) -> dict:
     """What a wall-shear closure can do for Cd at best: the slanted bluff body of evaluate_cd
@@ -326,43 +375,8 @@ def shear_floor(reynolds: float = 320.0, layout=(256, 128, 128), length: int = 6
     (restrict_stress, the closure's training target) prescribed as the closure output, i.e. a
     perfect network. Each run's window-mean Cd and its time-averaged wall forces split into
     pressure (wall-normal) and friction (tangential) parts."""
-    from . import bodies, drag
-    from .gpu_solver import GpuSolver, SolverParams
-    from .voxelize import voxelize
-
-    def run(grid_layout, grid_length, prescribed=None):
-        body, _ = bodies.place_mesh(bodies.bluff_body(), None, grid_length, grid_layout, False)
-        grid = voxelize(body, None, grid_layout, False, link_fractions=True)
-        nu = u_in * grid_length / reynolds
-        s = GpuSolver.create(grid_layout, grid, SolverParams(3 * nu + 0.5, u_in, True, operator="reg",
-                                                             wall_model="law" if prescribed is None else "learned"))
-        try:
-            if prescribed is not None:
-                s.prescribe_wall_stress(*prescribed)
-            flow = drag.flow_through_steps(grid.nx, u_in)
-            total, window = flow_throughs * flow, average * flow
-            samples = [s.step_sampled(total - window, 8)]
-            chunk = max(8, (window // snapshots) // 8 * 8)
-            forces, done = None, total - window
-            while done < total:
-                k = min(chunk, total - done)
-                samples.append(s.step_sampled(k, 8))
-                done += k
-                f = s.wall_forces()
-                forces = f if forces is None else forces + f
-            forces /= -(-window // chunk)
-            fx, _, _ = drag.window_mean_force(np.concatenate(samples), total, window)
-            rec = s.closure_features(u_in)
-            out = dict(cells=s.wall_cells(), normals=rec[:, 8:11].astype(np.float64),
-                       exposed=rec[:, 11].astype(np.float64), forces=forces,
-                       cd=fx * drag.cd_per_lattice_force(grid.frontal_cells, u_in),
-                       split=drag_split(forces, rec[:, 8:11], grid.frontal_cells, u_in))
-        finally:
-            s.destroy()
-        log(f"  {grid_layout} {'wall law' if prescribed is None else 'prescribed fine-grid shear'}: "
-            f"Cd {out['cd']:.6f} (time-averaged wall forces: {out['split']})")
-        return out
-
+    run = lambda grid_layout, grid_length, prescribed=None: _floor_run(
+        grid_layout, grid_length, reynolds, u_in, flow_throughs, average, snapshots, prescribed, False, log)
     fine_layout = tuple(2 * v for v in layout)
     coarse = run(layout, length)
     fine = run(fine_layout, 2 * length)
@@ -376,6 +390,70 @@ def shear_floor(reynolds: float = 320.0, layout=(256, 128, 128), length: int = 6
             "law_vs_2x_grid_pct": pct(coarse["cd"]), "prescribed_vs_2x_grid_pct": pct(perfect["cd"]),
             "split_law": coarse["split"], "split_prescribed": perfect["split"], "split_2x_grid": fine["split"],
             "wall_cells": int(len(coarse["cells"])), "wall_cells_with_target": int(np.sum(support > 0))}
+
+
+def restrict_force(coarse_cells, coarse_layout, coarse_normals, fine_cells, fine_layout, fine_forces, fine_normals,
+                   scale: float, neighbours: int = 8, align: float = 0.0) -> np.ndarray:
+    """Total wall force the fine run puts on the patch of wall of each coarse wall cell: every
+    fine wall cell goes to the nearest coarse wall cell (by the fine cell centre's image at half
+    scale) among its `neighbours` nearest whose normal is within acos(align) of its own (the
+    nearest of all when none is), so the restricted forces sum to the fine total; times `scale`
+    (the coarse over the fine frontal area, the factor between the two grids' forces at equal
+    Cd). Pressure and friction are both kept."""
+    from scipy.spatial import cKDTree
+    cc = _coords(coarse_cells, coarse_layout).astype(np.float64)
+    fc = (_coords(fine_cells, fine_layout).astype(np.float64) + 0.5) / 2.0 - 0.5
+    tree = cKDTree(cc)
+    k = min(neighbours, len(cc))
+    _, idx = tree.query(fc, k=k)
+    idx = idx.reshape(len(fc), k)
+    cn = np.asarray(coarse_normals, np.float64)
+    fn = np.asarray(fine_normals, np.float64)
+    fn = fn / np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-12)
+    dots = np.einsum("fkd,fd->fk", cn[idx], fn)
+    ok = dots >= align
+    pick = np.where(ok.any(1), np.argmax(ok, 1), 0)
+    owner = idx[np.arange(len(fc)), pick]
+    out = np.zeros((len(cc), 3))
+    np.add.at(out, owner, np.asarray(fine_forces, np.float64))
+    return out * scale
+
+
+def source_floor(reynolds: float = 320.0, layout=(256, 128, 128), length: int = 64, u_in: float = 0.05,
+                 flow_throughs: int = 6, average: int = 2, snapshots: int = 40, iterations: int = 3,
+                 log=print) -> dict:
+    """What a closure whose output is a momentum source at each wall cell (pressure and friction
+    together) can do for Cd: the coarse run with the wall law; the 2x finer run; then the coarse
+    run with per-wall-cell sources C = (its momentum-exchange force) - (the fine run's total wall
+    force restricted to that cell, restrict_force), i.e. a network that predicts the fine grid's
+    per-cell force correction exactly for the wall-law flow; the flow's response to the sources
+    changes the exchange, so the sources are then updated from each run's own per-cell forces
+    (C <- its exchange - the restricted fine force) for `iterations` runs in all."""
+    run = lambda grid_layout, grid_length, prescribed=None, source=False: _floor_run(
+        grid_layout, grid_length, reynolds, u_in, flow_throughs, average, snapshots, prescribed, source, log)
+    fine_layout = tuple(2 * v for v in layout)
+    coarse = run(layout, length)
+    fine = run(fine_layout, 2 * length)
+    normals = coarse["normals"] / np.maximum(np.linalg.norm(coarse["normals"], axis=1, keepdims=True), 1e-12)
+    target = restrict_force(coarse["cells"], layout, normals, fine["cells"], fine_layout, fine["forces"],
+                            fine["normals"], coarse["frontal"] / fine["frontal"])
+    scale = 1.0 / (0.5 * u_in * u_in * coarse["frontal"])
+    log(f"  restricted fine force: Cd {target[:, 0].sum() * scale:.6f} (fine run {fine['cd']:.6f}; "
+        f"its time-averaged wall forces {fine['split']['cd']:.6f})")
+    pct = lambda a: 100 * (a / fine["cd"] - 1)
+    history = []
+    exchange = coarse["forces"]
+    valid = np.ones(len(target), bool)
+    for it in range(iterations):
+        sources = exchange - target
+        res = run(layout, length, (sources, valid), source=True)
+        history.append({"cd": res["cd"], "vs_2x_grid_pct": pct(res["cd"]), "cd_exchange": res["cd_exchange"],
+                        "cd_sources": res["cd_sources"], "split_exchange": res["split"]})
+        exchange = res["forces"]
+    return {"reynolds": reynolds, "layout": list(layout), "length_cells": length,
+            "cd_law": coarse["cd"], "cd_law_2x_grid": fine["cd"], "law_vs_2x_grid_pct": pct(coarse["cd"]),
+            "cd_restricted_fine": float(target[:, 0].sum() * scale), "iterations": history,
+            "split_law": coarse["split"], "split_2x_grid": fine["split"], "wall_cells": int(len(coarse["cells"]))}
 
 
 def main(argv=None):
@@ -397,6 +475,8 @@ def main(argv=None):
     ap.add_argument("--flow-throughs", type=float, default=4.0)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--source-floor", type=float, default=None, metavar="RE",
+                    help="measure the Cd a perfect momentum-source closure gives at this Reynolds number")
     ap.add_argument("--shear-floor", type=float, default=None, metavar="RE",
                     help="instead of training, measure Cd with the fine grid's wall shear prescribed "
                          "on the coarse grid (a perfect closure output) beside the wall law and the "
@@ -405,6 +485,13 @@ def main(argv=None):
                     help="instead of training, compare Cd of the learned closure with the wall law "
                          "and the twice finer grid at this Reynolds number (the closure file --out)")
     args = ap.parse_args(argv)
+    if args.source_floor is not None:
+        rec = source_floor(args.source_floor, tuple(args.coarse), args.length, log=lambda m: print(m, flush=True))
+        print(json.dumps(rec, indent=1, default=float))
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump(rec, f, indent=1, default=float)
+        return 0
     if args.shear_floor is not None:
         rec = shear_floor(args.shear_floor, tuple(args.coarse), args.length, log=lambda m: print(m, flush=True))
         print(json.dumps(rec))

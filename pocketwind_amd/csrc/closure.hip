@@ -128,6 +128,27 @@ void launch_closure(pw_solver* s, StepParams& p) {
   p.closure_slot = s->closure_slot;
 }
 
+// The momentum sources prescribed through pw_debug_set_wall_stress (w = 2): each listed wall
+// cell's stored post-collision populations gain 3 w_i c_i . C, which adds C to its momentum and
+// nothing to its mass (sum_i w_i c_i c_i = I / 3). f32 storage only.
+__global__ __launch_bounds__(256) void source_kernel(float* __restrict__ set, StepParams p,
+                                                     const uint32_t* __restrict__ cells, uint32_t n_cells,
+                                                     const float4* __restrict__ source) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_cells) return;
+  const float4 c = source[k];
+  if (!(c.w > 0.0f)) return;
+  float* cell = set + pop_cell(p, cells[k]);
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) cell[(size_t)i * p.run] += 3.0f * wf(i) * cdot(i, c.x, c.y, c.z);
+}
+
+void launch_sources(pw_solver* s, void* set) {
+  if (!s->n_wall || !s->closure_source) return;
+  hipLaunchKernelGGL(source_kernel, dim3((unsigned)((s->n_wall + 255) / 256)), dim3(256), 0, s->stream,
+                     static_cast<float*>(set), s->sp, s->wall_cells, (uint32_t)s->n_wall, s->closure_source);
+}
+
 void settle_closure(pw_solver* s) {
   if (!s->closure_pending) return;
   (void)hipStreamWaitEvent(s->stream, s->closure_done, 0);
@@ -279,6 +300,7 @@ pw_status pw_solver_set_closure(pw_solver* s, const pw_mlp* net, const float* no
   if (e != hipSuccess) return device_error(e, "closure network copy");
   std::memcpy(s->closure_norm, norm, n_norm * sizeof(float));
   s->closure_on = 1;
+  s->closure_sources = 0;
   s->closure_fresh = 0;
   return PW_OK;
 }
@@ -304,12 +326,32 @@ pw_status pw_debug_set_wall_stress(pw_solver* s, const float* stress, size_t n_f
     set_error("Wall stress input needs 4 floats per wall cell.");
     return PW_EINVAL;
   }
+  // w = 2 entries are momentum sources: the wall collision keeps the law for them (w = 0 in
+  // the stress buffer) and a pass after each step adds them (launch_sources).
+  std::vector<float> shear(stress, stress + n_floats), source(n_floats, 0.0f);
+  bool sources = false;
+  for (size_t k = 0; k < s->n_wall; ++k)
+    if (stress[4 * k + 3] > 1.5f) {
+      for (int c = 0; c < 3; ++c) source[4 * k + c] = stress[4 * k + c];
+      source[4 * k + 3] = 1.0f;
+      shear[4 * k + 3] = 0.0f;
+      sources = true;
+    }
+  if (sources && s->params.storage != PW_STORAGE_F32) {
+    set_error("Momentum sources need f32 storage.");
+    return PW_EINVAL;
+  }
   if (s->n_wall) {
-    hipError_t e = hipMemcpyAsync(s->closure_tau, stress, n_floats * sizeof(float), hipMemcpyHostToDevice, s->stream);
+    hipError_t e = hipMemcpyAsync(s->closure_tau, shear.data(), n_floats * sizeof(float), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess && sources && !s->closure_source)
+      e = hipMalloc((void**)&s->closure_source, s->n_wall * sizeof(float4));
+    if (e == hipSuccess && sources)
+      e = hipMemcpyAsync(s->closure_source, source.data(), n_floats * sizeof(float), hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return device_error(e, "wall stress upload");
   }
   s->closure_on = 2;
+  s->closure_sources = sources ? 1 : 0;
   s->closure_fresh = 1;
   return PW_OK;
 }

@@ -249,6 +249,7 @@ void launch_step(pw_solver* s, bool macro, const StepParams& params) {
     case PW_STORAGE_BF16: launch_step_s<StoreBF16>(s, in, out, macro, p, seed); break;
     default: launch_step_s<StoreF32>(s, in, out, macro, p, seed); break;
   }
+  if (s->closure_on == 2 && s->closure_sources) launch_sources(s, out);
 }
 
 template <class S>
@@ -391,6 +392,7 @@ void free_all(pw_solver* s) {
   if (s->side_stream) hipStreamSynchronize(s->side_stream);
   if (s->closure_tau) hipFree(s->closure_tau);
   if (s->closure_next) hipFree(s->closure_next);
+  if (s->closure_source) hipFree(s->closure_source);
   if (s->state_ready) hipEventDestroy(s->state_ready);
   if (s->closure_done) hipEventDestroy(s->closure_done);
   if (s->side_stream) hipStreamDestroy(s->side_stream);

@@ -114,6 +114,8 @@ void launch_closure(pw_solver* s, StepParams& p);
 // Joins an overlapped closure evaluation still running beside the last step: the solver's
 // stream waits for it and its output becomes the one the next step reads.
 void settle_closure(pw_solver* s);
+// Adds the prescribed momentum sources to the wall cells of the set a step just wrote.
+void launch_sources(pw_solver* s, void* set);
 // Build the step's block order for the closure evaluated in the step (blocks that hold wall
 // cells first, so their longer wavefronts start with the launch rather than in its middle)
 // for the solver's current block size and XCD chunking. Allocates; called when a network is
@@ -176,6 +178,9 @@ struct pw_solver {
   hipEvent_t state_ready = nullptr, closure_done = nullptr;
   float4* closure_next = nullptr;
   int closure_pending = 0;  // closure_next is being written; the main stream has not joined it
+  // pw_debug_set_wall_stress momentum sources (w = 2), added after each step (f32 storage)
+  float4* closure_source = nullptr;
+  int closure_sources = 0;
   uint32_t* tile_order = nullptr;  // step block order for the closure in the step (closure_tile_order)
   uint32_t order_block = 0, order_chunks = 0;  // the variant tile_order was built for
   uint32_t* guard_flag = nullptr;       // the word after the macros (not a separate allocation)

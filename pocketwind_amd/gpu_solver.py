@@ -308,13 +308,16 @@ class GpuSolver:
             raise ValueError(f"Unknown closure placement {placement!r}.")
         N.check(N.lib().pw_solver_set_closure_placement(self._live(), self.PLACEMENTS.index(placement)))
 
-    def prescribe_wall_stress(self, stress, valid) -> None:
+    def prescribe_wall_stress(self, stress, valid, source: bool = False) -> None:
         """Test hook: the wall collision uses these stress vectors (n_wall x 3, wall-cell order)
-        where `valid`, the wall law elsewhere, in place of a network's output."""
+        where `valid`, the wall law elsewhere, in place of a network's output. With `source`,
+        the vectors are momentum sources added to the wall cells after the wall law's
+        collision instead (lattice units per step; the body's force then carries minus their
+        sum, which the caller accounts for)."""
         stress = np.asarray(stress, np.float32).reshape(-1, 3)
         out = np.zeros((stress.shape[0], 4), np.float32)
         out[:, :3] = stress
-        out[:, 3] = np.asarray(valid, bool)
+        out[:, 3] = np.asarray(valid, bool) * (2.0 if source else 1.0)
         N.check(N.lib().pw_debug_set_wall_stress(self._live(), out.ctypes.data, out.size))
 
     def set_closure_interval(self, every: int) -> None:
