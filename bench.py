@@ -20,8 +20,9 @@ f32 and 80 B at f16/bf16 storage, over every cell of the lattice.
 Headline: f16 deviation storage with the learned wall closure (the network shipped in
 pocketwind_amd/closure_c2.npz, trained by `python -m pocketwind_amd.closure` on coarse/fine
 pairs of other bodies at this resolution), against plain f32 storage with the wall law. Its
-step is two launches (closure kernel over the wall cells, then the step kernel); the roofline
-is the step kernel's, timed in the f16 run without the closure.
+step is the step kernel plus, every 8 steps, the closure kernel over the wall cells (the wall
+stress is reused in between); the same with the closure evaluated before every step is reported
+beside it. The roofline is the step kernel's, timed in the f16 run without the closure.
 
 Cd: six flow-throughs (6 ceil(nx / u_in) steps) with an FP64 force sample every 8 steps,
 mean over the last two (src/validation/cases.ts:294-320), Cd = fx / (1/2 u^2 A) with A the
@@ -93,19 +94,19 @@ def params_for(precision: str, learned: bool = False, tau0: float = TAU0):
                         wall_model="learned" if learned else "law")
 
 
-def make_solver(grid, params, closure=None):
+def make_solver(grid, params, closure=None, every=CLOSURE_EVERY):
     from pocketwind_amd.gpu_solver import GpuSolver
     solver = GpuSolver.create(grid.layout, grid, params)
     if not isinstance(solver, GpuSolver):
         raise MemoryError(solver.message)
     if closure is not None:
         solver.set_closure(*closure)
-        solver.set_closure_interval(CLOSURE_EVERY)
+        solver.set_closure_interval(every)
     return solver
 
 
-def time_solver(grid, params, steps, warmup, repeats=3, closure=None):
-    solver = make_solver(grid, params, closure)
+def time_solver(grid, params, steps, warmup, repeats=3, closure=None, every=CLOSURE_EVERY):
+    solver = make_solver(grid, params, closure, every)
     try:
         solver.step(warmup, False)
         solver.synchronize()
@@ -285,6 +286,9 @@ def main():
     learned = time_solver(grid, params_for("f16", learned=True), args.steps, args.warmup, closure=(net, norm))
     print(f"# body f16 + learned closure: {learned['mlups']:.1f} MLUPS, {learned['launch_us']:.2f} us/step "
           f"(closure + step kernel), {learned['variant']}, finite={learned['finite']}", file=sys.stderr, flush=True)
+    learned1 = time_solver(grid, params_for("f16", learned=True), args.steps, args.warmup, closure=(net, norm), every=1)
+    print(f"# body f16 + learned closure every step: {learned1['mlups']:.1f} MLUPS, {learned1['launch_us']:.2f} us/step",
+          file=sys.stderr, flush=True)
     empty = {}
     for precision in ("f16", "f32"):
         r = time_solver(empty_grid(LAYOUT), params_for(precision), args.steps, args.warmup)
@@ -323,6 +327,9 @@ def main():
         "f16_wall_law": {"value": round(step16["mlups"], 1), "achieved_gbps": round(step16["gbps"], 1)},
         "bf16_wall_law": {"value": round(body["bf16"]["mlups"], 1), "achieved_gbps": round(body["bf16"]["gbps"], 1)},
         "speedup_vs_f32": round(learned["mlups"] / body["f32"]["mlups"], 3),
+        "f16_learned_every_step": {"value": round(learned1["mlups"], 1),
+                                   "ms_per_step": round(learned1["launch_us"] / 1e3, 5),
+                                   "speedup_vs_f32": round(learned1["mlups"] / body["f32"]["mlups"], 3)},
         "empty_grid": {p: {"value": round(r["mlups"], 1), "achieved_gbps": round(r["gbps"], 1)}
                        for p, r in empty.items()},
         "variants": {p: r["variant"] for p, r in body.items()},

@@ -181,7 +181,7 @@ void launch_closures(pw_batch* b) {
   for (const Tunnel& tu : b->host)
     if (!tu.p.closure_net)
       launch_closure_tunnel(b->params.storage, b->params.walls == PW_WALLS_BOUZIDI, b->stream, tu.pops[b->parity],
-                          tu.geo, tu.p, tu.wall_cells, tu.n_wall, b->closure_net, b->closure_norm,
+                          tu.geo, tu.p, tu.wall_geo, tu.n_wall, b->closure_net, b->closure_norm,
                           b->closure_bufs[&tu - b->host.data()].second);
 }
 
@@ -243,8 +243,9 @@ hipError_t apply_closure_mode(pw_batch* b) {
     StepParams& p = b->host[t].p;
     const bool fused = b->closure_placement == PW_CLOSURE_IN_STEP && p.rows && b->params.op == PW_OP_REG && b->params.les;
     p.closure_net = fused ? b->closure_net : nullptr;
-    p.closure_slot = fused ? nullptr : b->closure_bufs[t].first;
+    p.closure_slot = b->closure_bufs[t].first;  // the kernel's buffer, or the records in the step
     p.closure_tau = fused ? nullptr : b->closure_bufs[t].second;
+    p.closure_geo = b->host[t].wall_geo;
     if (fused) b->lds = (kBlock / 64) * closure::kFusedWaveBytes;
     else b->closure_kernels = true;
   }
@@ -406,19 +407,22 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
           walls.push_back((uint32_t)c);
         }
       uint32_t *wl = nullptr, *sl = nullptr;
-      float4* tau = nullptr;
+      float4 *tau = nullptr, *records = nullptr;
       if ((e = dev_alloc(b, &wl, walls.size())) != hipSuccess || (e = dev_alloc(b, &sl, N)) != hipSuccess ||
-          (e = dev_alloc(b, &tau, walls.size())) != hipSuccess)
+          (e = dev_alloc(b, &tau, walls.size())) != hipSuccess ||
+          (e = dev_alloc(b, &records, walls.size() * (kWallGeoBytes / sizeof(float4)))) != hipSuccess)
         return fail(e, "batch closure buffers");
       if ((walls.size() && (e = hipMemcpy(wl, walls.data(), walls.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) ||
           (e = hipMemcpy(sl, slot.data(), N * 4, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "batch closure upload");
       tu.wall_cells = wl;
       tu.n_wall = (uint32_t)walls.size();
+      tu.wall_geo = records;  // written once the geometry is on the device
       b->closure_bufs.push_back({sl, tau});  // given to the step once a network is attached
     } else {
       tu.wall_cells = nullptr;
       tu.n_wall = 0;
+      tu.wall_geo = nullptr;
     }
     uint32_t *geo = nullptr, *bnd = nullptr;
     float4* macro = nullptr;
@@ -436,6 +440,7 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
       if ((e = hipMemcpy(wd, desc.data(), desc.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "batch upload");
       tu.p.wave_desc = wd;
     }
+    if (tu.wall_geo) launch_wall_geo(b->stream, geo, tu.p, tu.wall_cells, tu.n_wall, const_cast<void*>(tu.wall_geo));
     tu.boundary = bnd;
     tu.n_boundary = (uint32_t)n_boundary[t];
     tu.partial_base = part_base;

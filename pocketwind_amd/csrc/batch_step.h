@@ -21,6 +21,7 @@ struct Tunnel {
   uint32_t n_partials;
   const uint32_t* wall_cells;  // learned closure: fluid cells with wall links, in cell order
   uint32_t n_wall;
+  const void* wall_geo;  // their closure::WallGeo records
 };
 
 struct BatchLaunch {

@@ -20,7 +20,7 @@ constexpr int kCellsPerWave = 16;
 template <class S, bool BOUZIDI>
 __global__ __launch_bounds__(64) void closure_kernel(const typename S::T* __restrict__ in,
                                                      const uint32_t* __restrict__ geo, StepParams p,
-                                                     const uint32_t* __restrict__ cells, uint32_t n_cells,
+                                                     const WallGeo* __restrict__ walls, uint32_t n_cells,
                                                      const mlp::Packed* __restrict__ P, Norm norm,
                                                      float4* __restrict__ tau, float* __restrict__ records) {
   __shared__ __attribute__((aligned(16))) uint16_t xs[kCellsPerWave * mlp::kInPad];
@@ -33,7 +33,7 @@ __global__ __launch_bounds__(64) void closure_kernel(const typename S::T* __rest
   float f[kFeatures], law[3];
   bool in_range = valid;
   if (valid) {
-    wall_features<S, BOUZIDI>(in, geo, p, norm.u_ref, cells[k], f, law);
+    wall_features<S, BOUZIDI>(in, geo, p, norm.u_ref, load_wall_geo(walls, k), f, law);
     if (records) {
       for (int j = 0; j < kFeatures; ++j) records[(size_t)k * kRecord + j] = f[j];
       for (int j = 0; j < 3; ++j) records[(size_t)k * kRecord + kFeatures + j] = law[j];

@@ -21,6 +21,16 @@ namespace {
 
 inline unsigned closure_blocks(size_t n) { return (unsigned)((n + closure::kCellsPerWave - 1) / closure::kCellsPerWave); }
 
+static_assert(sizeof(closure::WallGeo) == kWallGeoBytes, "record size");
+
+// One record per listed wall cell.
+__global__ __launch_bounds__(256) void wall_geo_kernel(const uint32_t* __restrict__ geo, StepParams p,
+                                                       const uint32_t* __restrict__ cells, uint32_t n_cells,
+                                                       closure::WallGeo* __restrict__ out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_cells) out[k] = closure::wall_geo_of(geo, p, cells[k]);
+}
+
 template <class S>
 void launch_closure_s(pw_solver* s, hipStream_t stream, const void* state, const void* net, float4* tau,
                       float* records) {
@@ -30,12 +40,13 @@ void launch_closure_s(pw_solver* s, hipStream_t stream, const void* state, const
   const bool bz = s->params.walls == PW_WALLS_BOUZIDI;
   auto k = bz ? closure::closure_kernel<S, true> : closure::closure_kernel<S, false>;
   hipLaunchKernelGGL(k, dim3(closure_blocks(s->n_wall)), dim3(64), 0, stream, (const typename S::T*)state,
-                     s->geometry, s->sp, s->wall_cells, (uint32_t)s->n_wall, (const mlp::Packed*)net, norm, tau,
-                     records);
+                     s->geometry, s->sp, (const closure::WallGeo*)s->closure_geo, (uint32_t)s->n_wall,
+                     (const mlp::Packed*)net, norm, tau, records);
 }
 
 void launch_closure_on(pw_solver* s, hipStream_t stream, const void* state, const void* net, float4* tau,
                        float* records) {
+  if (!s->closure_geo || s->n_wall == 0) return;  // the records exist from create on; no wall cells, no work
   switch (s->params.storage) {
     case PW_STORAGE_F16: launch_closure_s<StoreF16>(s, stream, state, net, tau, records); break;
     case PW_STORAGE_BF16: launch_closure_s<StoreBF16>(s, stream, state, net, tau, records); break;
@@ -75,24 +86,30 @@ pw_status check(pw_solver* s) {
 
 template <class S>
 void launch_closure_tunnel_s(bool bz, hipStream_t stream, const void* state, const uint32_t* geo, const StepParams& p,
-                             const uint32_t* wall_cells, uint32_t n_wall, const void* net, const float* norm27,
-                             float4* tau) {
+                             const void* walls, uint32_t n_wall, const void* net, const float* norm27, float4* tau) {
   closure::Norm norm;
   std::memcpy(&norm, norm27, sizeof norm);
   auto k = bz ? closure::closure_kernel<S, true> : closure::closure_kernel<S, false>;
   hipLaunchKernelGGL(k, dim3(closure_blocks(n_wall)), dim3(64), 0, stream, (const typename S::T*)state, geo, p,
-                     wall_cells, n_wall, (const mlp::Packed*)net, norm, tau, nullptr);
+                     (const closure::WallGeo*)walls, n_wall, (const mlp::Packed*)net, norm, tau, nullptr);
 }
 
 void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, const void* state,
-                           const uint32_t* geo, const StepParams& p, const uint32_t* wall_cells, uint32_t n_wall,
+                           const uint32_t* geo, const StepParams& p, const void* walls, uint32_t n_wall,
                            const void* net, const float* norm, float4* tau) {
   if (n_wall == 0) return;
   switch (storage) {
-    case PW_STORAGE_F16: launch_closure_tunnel_s<StoreF16>(bouzidi, stream, state, geo, p, wall_cells, n_wall, net, norm, tau); break;
-    case PW_STORAGE_BF16: launch_closure_tunnel_s<StoreBF16>(bouzidi, stream, state, geo, p, wall_cells, n_wall, net, norm, tau); break;
-    default: launch_closure_tunnel_s<StoreF32>(bouzidi, stream, state, geo, p, wall_cells, n_wall, net, norm, tau); break;
+    case PW_STORAGE_F16: launch_closure_tunnel_s<StoreF16>(bouzidi, stream, state, geo, p, walls, n_wall, net, norm, tau); break;
+    case PW_STORAGE_BF16: launch_closure_tunnel_s<StoreBF16>(bouzidi, stream, state, geo, p, walls, n_wall, net, norm, tau); break;
+    default: launch_closure_tunnel_s<StoreF32>(bouzidi, stream, state, geo, p, walls, n_wall, net, norm, tau); break;
   }
+}
+
+void launch_wall_geo(hipStream_t stream, const uint32_t* geo, const StepParams& p, const uint32_t* wall_cells,
+                     uint32_t n_wall, void* out) {
+  if (n_wall == 0) return;
+  hipLaunchKernelGGL(wall_geo_kernel, dim3((n_wall + 255) / 256), dim3(256), 0, stream, geo, p, wall_cells, n_wall,
+                     static_cast<closure::WallGeo*>(out));
 }
 
 void launch_closure(pw_solver* s, StepParams& p) {
@@ -101,6 +118,8 @@ void launch_closure(pw_solver* s, StepParams& p) {
   if (in_step_closure(s) && s->tile_order && s->order_block == s->block && s->order_chunks == s->xcd_chunks) {
     p.closure_net = s->closure_net;
     p.tile_order = s->tile_order;
+    p.closure_slot = s->closure_slot;
+    p.closure_geo = s->closure_geo;
     return;
   }
   // closure_on 2: the output was prescribed from the host (pw_debug_set_wall_stress).

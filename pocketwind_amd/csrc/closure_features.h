@@ -34,11 +34,92 @@ struct Norm {
   float u_ref;      // velocity unit of features 0-5
 };
 
+// The static part of a wall cell's features, from the packed geometry alone: computed once per
+// geometry (wall_geo_kernel) so that evaluating the features reads a 48-byte record and the two
+// cells' populations instead of chasing flags, link fractions and the neighbour's flag.
+struct WallGeo {
+  uint32_t n, flag;    // the wall cell and its flag
+  uint32_t m, flag_m;  // the fluid cell one step along the rounded normal and its flag (else n)
+  float ax, ay, az;    // unit wall normal, n = sum over wall links of w_i c_i (zero when |n| <= 1e-3)
+  float q;             // mean wall-link fraction (1/2 when the geometry carries none)
+  float exposed6;      // 6 sum over wall links of w_i
+  uint32_t bits;       // kUseNeighbour | kProjected
+  uint32_t pad[2];
+};
+static_assert(sizeof(WallGeo) == 48, "three 16-byte loads per record");
+enum : uint32_t { kUseNeighbour = 1u, kProjected = 2u };
+
+__device__ __forceinline__ WallGeo wall_geo_of(const uint32_t* __restrict__ geo, const StepParams& p, uint32_t n) {
+  const size_t N = p.cells;
+  WallGeo w;
+  w.n = n;
+  w.flag = geo[n];
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f, exposed = 0.0f, qsum = 0.0f;
+  int links = 0;
+  const uint32_t base = geo[N + n];
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) {
+    if (!has_wall(w.flag, i)) continue;
+    nx += wf(i) * cx(i);
+    ny += wf(i) * cy(i);
+    nz += wf(i) * cz(i);
+    exposed += wf(i);
+    float q = 0.5f;  // the geometry's link fraction when it carries one, whatever the wall rule
+    if (base != kNoQ) q = as_float(geo[2 * N + base + __builtin_popcount((w.flag >> 3) & ((1u << i) - 1u))]);
+    qsum += q;
+    ++links;
+  }
+  const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+  const bool projected = nl > 1e-3f;
+  w.ax = projected ? nx / nl : 0.0f;
+  w.ay = projected ? ny / nl : 0.0f;
+  w.az = projected ? nz / nl : 0.0f;
+  w.q = links ? qsum / links : 0.5f;
+  w.exposed6 = 6.0f * exposed;
+  w.bits = projected ? kProjected : 0u;
+  w.m = n;
+  w.flag_m = w.flag;
+  // Second sample: the fluid cell one step along the rounded normal.
+  const int dx = (int)rintf(w.ax), dy = (int)rintf(w.ay), dz = (int)rintf(w.az);
+  if (dx | dy | dz) {
+    uint32_t x, y, z;
+    cell_coords(n, p, x, y, z);
+    const uint32_t m = Neighbours(x, y, z, p).index(dx, dy, dz);
+    const uint32_t fm = geo[m];
+    if ((fm & kTypeMask) == kFluid) {
+      w.m = m;
+      w.flag_m = fm;
+      w.bits |= kUseNeighbour;
+    }
+  }
+  w.pad[0] = w.pad[1] = 0u;
+  return w;
+}
+
+__device__ __forceinline__ WallGeo load_wall_geo(const WallGeo* __restrict__ records, uint32_t k) {
+  const uint4* r = reinterpret_cast<const uint4*>(records + k);
+  const uint4 a = r[0], b = r[1], c = r[2];
+  WallGeo w;
+  w.n = a.x;
+  w.flag = a.y;
+  w.m = a.z;
+  w.flag_m = a.w;
+  w.ax = as_float(b.x);
+  w.ay = as_float(b.y);
+  w.az = as_float(b.z);
+  w.q = as_float(b.w);
+  w.exposed6 = as_float(c.x);
+  w.bits = c.y;
+  w.pad[0] = w.pad[1] = 0u;
+  return w;
+}
+
+// The populations of fluid cell n (flag its flag) gathered exactly as the step gathers them,
+// wall links included. Every direction's load is independent of the others.
 template <class S, bool BOUZIDI>
 __device__ __forceinline__ void gather_cell(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
-                                            const StepParams& p, uint32_t n, float* g) {
+                                            const StepParams& p, uint32_t n, uint32_t flag, float* g) {
   const size_t P = p.run;
-  const uint32_t flag = geo[n];
   uint32_t x, y, z;
   cell_coords(n, p, x, y, z);
   const Neighbours nb(x, y, z, p);
@@ -66,43 +147,22 @@ __device__ __forceinline__ void velocity_of(const float* g, float& rho, float& u
   uz = mz / rho;
 }
 
-// Features of wall cell n (flag its flag) whose gathered populations, wall links included, are g.
+// Features of the wall cell whose static geometry is w, and the wall law's stress vector
+// rho u_tau^2 t (stream_collide.wgsl:18-80) for comparison.
 template <class S, bool BOUZIDI>
-__device__ __forceinline__ void features_of(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
-                                            const StepParams& p, float u_ref, uint32_t n, uint32_t flag, const float* g,
-                                            float* f, float* law) {
-  const size_t N = p.cells;
+__device__ __forceinline__ void wall_features(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
+                                              const StepParams& p, float u_ref, const WallGeo& w, float* f,
+                                              float* law) {
+  float g[kQ], h[kQ];
+  gather_cell<S, BOUZIDI>(in, geo, p, w.n, w.flag, g);
+  const bool second = w.bits & kUseNeighbour;
+  if (second) gather_cell<S, BOUZIDI>(in, geo, p, w.m, w.flag_m, h);
   float rho, ux, uy, uz;
   velocity_of(g, rho, ux, uy, uz);
-  float nx = 0.0f, ny = 0.0f, nz = 0.0f, exposed = 0.0f, qsum = 0.0f;
-  int links = 0;
-  const uint32_t base = geo[N + n];
-#pragma unroll
-  for (int i = 1; i < kQ; ++i) {
-    if (!has_wall(flag, i)) continue;
-    nx += wf(i) * cx(i);
-    ny += wf(i) * cy(i);
-    nz += wf(i) * cz(i);
-    exposed += wf(i);
-    float q = 0.5f;  // the geometry's link fraction when it carries one, whatever the wall rule
-    if (base != kNoQ) q = as_float(geo[2 * N + base + __builtin_popcount((flag >> 3) & ((1u << i) - 1u))]);
-    qsum += q;
-    ++links;
-  }
-  const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
-  const float ax = nl > 1e-3f ? nx / nl : 0.0f, ay = nl > 1e-3f ? ny / nl : 0.0f, az = nl > 1e-3f ? nz / nl : 0.0f;
-  // Second sample: the fluid cell one step along the rounded normal.
-  const int dx = (int)rintf(ax), dy = (int)rintf(ay), dz = (int)rintf(az);
   float vx = ux, vy = uy, vz = uz;
-  if (dx | dy | dz) {
-    uint32_t x, y, z;
-    cell_coords(n, p, x, y, z);
-    const uint32_t m = Neighbours(x, y, z, p).index(dx, dy, dz);
-    if ((geo[m] & kTypeMask) == kFluid) {
-      float h[kQ], r2;
-      gather_cell<S, BOUZIDI>(in, geo, p, m, h);
-      velocity_of(h, r2, vx, vy, vz);
-    }
+  if (second) {
+    float r2;
+    velocity_of(h, r2, vx, vy, vz);
   }
   const float inv = 1.0f / u_ref;
   f[0] = ux * inv;
@@ -111,19 +171,18 @@ __device__ __forceinline__ void features_of(const typename S::T* __restrict__ in
   f[3] = vx * inv;
   f[4] = vy * inv;
   f[5] = vz * inv;
-  f[6] = links ? qsum / links : 0.5f;
+  f[6] = w.q;
   f[7] = log10f(p.nu);
-  f[8] = ax;
-  f[9] = ay;
-  f[10] = az;
-  f[11] = 6.0f * exposed;
-  // The wall law's stress vector rho u_tau^2 t (stream_collide.wgsl:18-80), for comparison.
+  f[8] = w.ax;
+  f[9] = w.ay;
+  f[10] = w.az;
+  f[11] = w.exposed6;
   float sx = ux, sy = uy, sz = uz;
-  if (nl > 1e-3f) {
-    const float d = ux * ax + uy * ay + uz * az;
-    sx -= d * ax;
-    sy -= d * ay;
-    sz -= d * az;
+  if (w.bits & kProjected) {
+    const float d = ux * w.ax + uy * w.ay + uz * w.az;
+    sx -= d * w.ax;
+    sy -= d * w.ay;
+    sz -= d * w.az;
   }
   const float speed = sqrtf(sx * sx + sy * sy + sz * sz);
   law[0] = law[1] = law[2] = 0.0f;
@@ -134,14 +193,6 @@ __device__ __forceinline__ void features_of(const typename S::T* __restrict__ in
     law[1] = st * sy;
     law[2] = st * sz;
   }
-}
-
-template <class S, bool BOUZIDI>
-__device__ __forceinline__ void wall_features(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
-                                              const StepParams& p, float u_ref, uint32_t n, float* f, float* law) {
-  float g[kQ];
-  gather_cell<S, BOUZIDI>(in, geo, p, n, g);
-  features_of<S, BOUZIDI>(in, geo, p, u_ref, n, geo[n], g, f, law);
 }
 
 // Inside the step: a wavefront of the row step that holds wall cells (fluid cells with wall
@@ -179,7 +230,8 @@ __device__ __forceinline__ void fused_closure(const typename S::T* __restrict__ 
   bool in_range = wall_lane;
   if (wall_lane) {
     float law[3];
-    wall_features<S, BOUZIDI>(in, geo, p, norm.u_ref, n, f, law);
+    wall_features<S, BOUZIDI>(in, geo, p, norm.u_ref,
+                              load_wall_geo(static_cast<const WallGeo*>(p.closure_geo), p.closure_slot[n]), f, law);
   }
   float4 tau = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   for (uint32_t t = 0; t < count; t += 16) {

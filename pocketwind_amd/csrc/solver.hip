@@ -144,6 +144,7 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
   out->closure_slot = nullptr;
   out->closure_net = nullptr;
   out->tile_order = nullptr;
+  out->closure_geo = nullptr;
   out->bc = (prm.sides == PW_SIDES_FREE_SLIP ? kBcFreeSlip : 0u) |
             (prm.outlet == PW_OUTLET_ZERO_GRADIENT ? kBcZeroGradient : 0u);
   return true;
@@ -393,6 +394,7 @@ void free_all(pw_solver* s) {
   if (s->closure_tau) hipFree(s->closure_tau);
   if (s->closure_next) hipFree(s->closure_next);
   if (s->closure_source) hipFree(s->closure_source);
+  if (s->closure_geo) hipFree(s->closure_geo);
   if (s->state_ready) hipEventDestroy(s->state_ready);
   if (s->closure_done) hipEventDestroy(s->closure_done);
   if (s->side_stream) hipStreamDestroy(s->side_stream);
@@ -587,6 +589,10 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
     sp.wave_desc = s->wave_desc;
     s->sp.wave_desc = s->wave_desc;
   }
+  // Every solver exports closure features (pw_solver_closure_features), so every solver holds
+  // its wall cells' static closure geometry.
+  if ((e = hipMalloc(&s->closure_geo, (s->n_wall ? s->n_wall : 1) * kWallGeoBytes)) != hipSuccess)
+    return fail(device_error(e, "closure geometry allocation"));
   if (learned) {
     std::vector<uint32_t> slot(N, 0xffffffffu);
     for (size_t k = 0; k < wall_list.size(); ++k) slot[wall_list[k]] = (uint32_t)k;
@@ -612,6 +618,7 @@ pw_status pw_solver_create(const pw_layout* layout, const uint32_t* geometry, si
   if ((e = hipMemcpyAsync(s->geometry, geometry, geometry_words * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "geometry upload"));
   if (n_boundary && (e = hipMemcpyAsync(s->boundary, boundary_cells, n_boundary * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "boundary upload"));
   if (s->n_wall && (e = hipMemcpyAsync(s->wall_cells, wall_list.data(), s->n_wall * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess) return fail(device_error(e, "wall cell upload"));
+  launch_wall_geo(s->stream, s->geometry, s->sp, s->wall_cells, (uint32_t)s->n_wall, s->closure_geo);
   launch_init_both(s, full);
   // PW_STEP_KERNEL=<cell|cell-nt|cell-nt-stores>[/<threads per block>[/<chunks per XCD>]]
   // fixes the step variant instead of tuning it (profiling runs, reproducible timings).

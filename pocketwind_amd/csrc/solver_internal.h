@@ -79,6 +79,9 @@ struct StepParams {
   const void* closure_net;
   // With closure_net: the block order of the step launch, blocks holding wall cells first.
   const uint32_t* tile_order;
+  // The wall cells' static closure geometry (closure::WallGeo records, by wall-cell index);
+  // read with closure_slot by the closure evaluated in the step.
+  const void* closure_geo;
 };
 
 // Descriptor of a 64-cell run on the row path. 0: every cell is a fluid cell without wall
@@ -124,8 +127,13 @@ hipError_t closure_tile_order(pw_solver* s);
 // The closure kernel over one tunnel's wall-cell list (batched tunnels): `norm` holds the 27
 // normalisation floats, `net` the packed network.
 void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, const void* state,
-                           const uint32_t* geo, const StepParams& p, const uint32_t* wall_cells, uint32_t n_wall,
+                           const uint32_t* geo, const StepParams& p, const void* walls, uint32_t n_wall,
                            const void* net, const float* norm, float4* tau);
+// Bytes of one wall cell's static closure geometry record, and the kernel that writes the
+// records of the listed wall cells from the packed geometry.
+constexpr size_t kWallGeoBytes = 48;
+void launch_wall_geo(hipStream_t stream, const uint32_t* geo, const StepParams& p, const uint32_t* wall_cells,
+                     uint32_t n_wall, void* out);
 // The macro field, flags, layout and stream of one batched tunnel (false for a bad index).
 bool batch_field(pw_batch* b, uint32_t tunnel, const float4** macro, const uint32_t** geo, pw_layout* layout,
                  hipStream_t* stream);
@@ -166,6 +174,7 @@ struct pw_solver {
   size_t n_wall = 0;
   uint32_t* closure_slot = nullptr;  // wall-cell index of every cell (learned model only)
   float4* closure_tau = nullptr;     // closure output per wall cell
+  void* closure_geo = nullptr;       // closure::WallGeo per wall cell
   void* closure_net = nullptr;       // device copy of the network's matrix-core parameters
   float closure_norm[32] = {};       // closure::Norm
   int closure_on = 0;
