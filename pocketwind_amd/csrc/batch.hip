@@ -122,6 +122,8 @@ struct pw_batch {
   uint32_t xcd_chunks = 16;
   uint32_t chunk = 1;           // tiles per XCD chunk of the step grid
   unsigned step_grid = 0;
+  uint32_t persistent = 0;      // resident blocks of the persistent step form, 0: one block per tile
+  uint32_t cus = 0;             // compute units of the device
   uint32_t* part_tunnel = nullptr;
   uint32_t n_partials = 0;
   double* partials = nullptr;
@@ -149,7 +151,7 @@ namespace {
 
 void launch_step(pw_batch* b, int parity, bool macro) {
   const BatchLaunch L{b->step_grid, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, b->seeds,
-                      b->params, b->policy, b->chunk, b->lds};
+                      b->params, b->policy, b->chunk, b->lds, b->persistent};
   switch (b->params.storage) {
     case PW_STORAGE_F16: batch_step_s<StoreF16>(L, parity, macro); break;
     case PW_STORAGE_BF16: batch_step_s<StoreBF16>(L, parity, macro); break;
@@ -259,6 +261,15 @@ hipError_t apply_closure_mode(pw_batch* b) {
   return e;
 }
 
+// Resident blocks of the persistent step form on this device: the collision's blocks per CU
+// times the CUs, a multiple of 8.
+uint32_t persistent_grid(const pw_batch* b) {
+  const bool reg = b->params.op == PW_OP_REG, les = b->params.les != 0;
+  const uint32_t per_cu = reg ? (les ? persistent_blocks_per_cu<kOpReg, true>() : persistent_blocks_per_cu<kOpReg, false>())
+                              : (les ? persistent_blocks_per_cu<kOpTrt, true>() : persistent_blocks_per_cu<kOpTrt, false>());
+  return per_cu * b->cus / kXcds * kXcds;
+}
+
 bool forced_batch_variant(pw_batch* b, const char* text) {
   if (!text) return false;
   const std::string v(text);
@@ -272,13 +283,18 @@ bool forced_batch_variant(pw_batch* b, const char* text) {
   uint32_t chunks = b->xcd_chunks;
   if (a != std::string::npos) chunks = (uint32_t)std::strtoul(v.c_str() + a + 1, nullptr, 10);
   if (!valid_xcd_chunks(chunks)) return false;
+  const size_t c = a == std::string::npos ? a : v.find('/', a + 1);
+  const bool persist = c != std::string::npos && v.substr(c + 1) == "persistent";
+  if (c != std::string::npos && !persist) return false;
   b->policy = policy;
   b->xcd_chunks = chunks;
+  b->persistent = persist ? persistent_grid(b) : 0u;
   return true;
 }
 
 // Times the three cache policies with each chunking of kXcdChunkings that leaves every chunk
-// two tiles or more (two runs of a few steps each, the faster counting) and keeps the fastest.
+// two tiles or more, each launched one block per tile and in the persistent form (two runs of a
+// few steps each, the faster counting), and keeps the fastest.
 hipError_t tune_batch(pw_batch* b) {
   const uint32_t policies[3] = {kCached, kNtAll, kNtStores};
   const uint64_t cells = [&] {
@@ -288,39 +304,45 @@ hipError_t tune_batch(pw_batch* b) {
   }();
   const int reps = cells >= (1u << 22) ? 4 : 16;
   float best = 0;
-  uint32_t best_policy = kNtStores, best_chunks = 16;
+  uint32_t best_policy = kNtStores, best_chunks = 16, best_persistent = 0;
+  const uint32_t resident = persistent_grid(b);
   for (uint32_t chunks : kXcdChunkings)
-    for (uint32_t pol : policies) {
-      if (chunks > 1 && (uint64_t)chunks * 2 * kXcds > b->n_tiles) continue;
-      b->policy = pol;
-      b->xcd_chunks = chunks;
-      b->step_grid = batch_grid(b->n_tiles, chunks, &b->chunk);
-      for (int w = 0; w < 2; ++w) {
-        launch_step(b, b->parity, false);
-        b->parity ^= 1;
-      }
-      float fastest = 0;
-      for (int round = 0; round < 2; ++round) {
-        hipError_t e = hipEventRecord(b->t0, b->stream);
-        if (e != hipSuccess) return e;
-        for (int r = 0; r < reps; ++r) {
+    for (uint32_t pol : policies)
+      for (uint32_t persist : {0u, resident}) {
+        if (chunks > 1 && (uint64_t)chunks * 2 * kXcds > b->n_tiles) continue;
+        if (persist && (persist >= batch_grid(b->n_tiles, chunks, &b->chunk) || resident == 0)) continue;
+        b->policy = pol;
+        b->xcd_chunks = chunks;
+        b->persistent = persist;
+        b->step_grid = batch_grid(b->n_tiles, chunks, &b->chunk);
+        for (int w = 0; w < 2; ++w) {
           launch_step(b, b->parity, false);
           b->parity ^= 1;
         }
-        if ((e = hipEventRecord(b->t1, b->stream)) != hipSuccess) return e;
-        if ((e = hipEventSynchronize(b->t1)) != hipSuccess) return e;
-        float ms = 0;
-        if ((e = hipEventElapsedTime(&ms, b->t0, b->t1)) != hipSuccess) return e;
-        if (round == 0 || ms < fastest) fastest = ms;
+        float fastest = 0;
+        for (int round = 0; round < 2; ++round) {
+          hipError_t e = hipEventRecord(b->t0, b->stream);
+          if (e != hipSuccess) return e;
+          for (int r = 0; r < reps; ++r) {
+            launch_step(b, b->parity, false);
+            b->parity ^= 1;
+          }
+          if ((e = hipEventRecord(b->t1, b->stream)) != hipSuccess) return e;
+          if ((e = hipEventSynchronize(b->t1)) != hipSuccess) return e;
+          float ms = 0;
+          if ((e = hipEventElapsedTime(&ms, b->t0, b->t1)) != hipSuccess) return e;
+          if (round == 0 || ms < fastest) fastest = ms;
+        }
+        if (best == 0 || fastest < best) {
+          best = fastest;
+          best_policy = pol;
+          best_chunks = chunks;
+          best_persistent = persist;
+        }
       }
-      if (best == 0 || fastest < best) {
-        best = fastest;
-        best_policy = pol;
-        best_chunks = chunks;
-      }
-    }
   b->policy = best_policy;
   b->xcd_chunks = best_chunks;
+  b->persistent = best_persistent;
   return hipGetLastError();
 }
 
@@ -381,6 +403,13 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   };
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "batch stream");
+  {
+    int device = 0, cus = 0;
+    if ((e = hipGetDevice(&device)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+      return fail(e, "device attributes");
+    b->cus = (uint32_t)cus;
+  }
   if ((e = hipEventCreate(&b->t0)) != hipSuccess || (e = hipEventCreate(&b->t1)) != hipSuccess) return fail(e, "batch events");
   const uint32_t elem = shared->storage == PW_STORAGE_F32 ? 4 : 2;
   std::vector<uint32_t> tt, tf(count), pt;
@@ -486,6 +515,10 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail(e, "batch init");
   *out = b;
   return PW_OK;
+}
+
+uint32_t pw_batch_persistent_grid(const pw_batch* b) {
+  return (b && b->persistent && b->persistent < b->step_grid) ? b->persistent : 0u;  // as batch_step_t launches
 }
 
 uint32_t pw_batch_step_variant(const pw_batch* b, uint32_t* xcd_chunks) {

@@ -36,6 +36,10 @@ struct BatchLaunch {
   uint32_t policy;  // population cache policy (kCached, kNtAll, kNtStores)
   uint32_t chunk;   // consecutive tiles one XCD takes (the grid holds a multiple of 8 chunks)
   uint32_t lds;     // dynamic LDS bytes per block: the closure evaluated in the step, else 0
+  // Persistent form (steps without macro writes, the closure not in the step): this many
+  // resident blocks (a multiple of 8) walk the `grid` block indices of the one-per-tile form;
+  // 0 launches one block per tile.
+  uint32_t persistent;
 };
 
 // One step of every tunnel. The rounding seed is the step count, read on the device so a
@@ -44,24 +48,21 @@ struct BatchLaunch {
 // A tunnel whose nx is a multiple of 64 steps its cells as row wavefronts (the single solver's
 // fast path; its sets carry the upstream-outlet moments after the 19 N elements), others per
 // cell. Either way a tile is one block of one tunnel.
-// One block per tile. Blocks are renumbered as in the single solver's step so that each XCD
-// takes runs of `chunk` consecutive tiles (neighbouring rows of one tunnel share its L2).
+// Block b of the one-per-tile grid steps one tile; blocks are renumbered as in the single
+// solver's step so that each XCD takes runs of `chunk` consecutive tiles (neighbouring rows of
+// one tunnel share its L2).
 // CLOSURE: row tunnels evaluate the learned closure in their wall wavefronts (REG + LES).
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool CLOSURE = false>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWaves<OP, LES>))) void batch_step_kernel(const Tunnel* __restrict__ tunnels,
-                                                            const uint32_t* __restrict__ tile_tunnel,
-                                                            const uint32_t* __restrict__ tile_first,
-                                                            uint32_t n_tiles, int parity, uint32_t* seeds,
-                                                            uint32_t chunk) {
-  const uint32_t seed = seeds[parity];
-  if (blockIdx.x == 0 && threadIdx.x == 0) seeds[parity ^ 1] = seed + 1;
-  const uint32_t b = blockIdx.x, local = b / kXcds;
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool CLOSURE>
+__device__ __forceinline__ void batch_block(const Tunnel* __restrict__ tunnels, const uint32_t* __restrict__ tile_tunnel,
+                                            const uint32_t* __restrict__ tile_first, uint32_t n_tiles, int parity,
+                                            uint32_t seed, uint32_t chunk, uint32_t b) {
+  const uint32_t local = b / kXcds;
   const uint32_t tile = ((local / chunk) * kXcds + b % kXcds) * chunk + local % chunk;
   if (tile >= n_tiles) return;
   const uint32_t t = tile_tunnel[tile];
   const Tunnel& tu = tunnels[t];
   const uint32_t n = (tile - tile_first[t]) * kBlock + threadIdx.x;
-  if (n >= tu.p.cells) return;
+  if (n >= tu.p.cells) return;  // whole wavefronts on row tunnels (their cell count is a multiple of 64)
   const auto* in = (const typename S::T*)tu.pops[parity];
   auto* out = (typename S::T*)tu.pops[parity ^ 1];
   if constexpr (CLOSURE) {
@@ -78,6 +79,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWav
   stream_collide_cell<S, OP, LES, BOUZIDI, MACRO, POL>(in, out, tu.geo, tu.macro, tu.p, n, seed);
 }
 
+// PERSIST: gridDim.x resident blocks walk the `blocks` block indices of the one-per-tile grid
+// (block b takes b, b + gridDim.x, ...; gridDim.x is a multiple of 8, so every index stays on the
+// XCD that the one-per-tile launch gives it). No block waits for another, so no launch shape can
+// deadlock; the tuner times it against one block per tile.
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool CLOSURE = false, bool PERSIST = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStepWaves<OP, LES>))) void batch_step_kernel(const Tunnel* __restrict__ tunnels,
+                                                            const uint32_t* __restrict__ tile_tunnel,
+                                                            const uint32_t* __restrict__ tile_first,
+                                                            uint32_t n_tiles, int parity, uint32_t* seeds,
+                                                            uint32_t chunk, uint32_t blocks) {
+  const uint32_t seed = seeds[parity];
+  if (blockIdx.x == 0 && threadIdx.x == 0) seeds[parity ^ 1] = seed + 1;
+  if constexpr (PERSIST) {
+    for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x)
+      batch_block<S, OP, LES, BOUZIDI, MACRO, POL, CLOSURE>(tunnels, tile_tunnel, tile_first, n_tiles, parity, seed,
+                                                            chunk, b);
+  } else {
+    batch_block<S, OP, LES, BOUZIDI, MACRO, POL, CLOSURE>(tunnels, tile_tunnel, tile_first, n_tiles, parity, seed,
+                                                          chunk, blockIdx.x);
+  }
+}
+
 // Grid of a batched step: the tiles rounded up to whole chunks on each of the 8 XCDs.
 inline unsigned batch_grid(uint32_t n_tiles, uint32_t chunks_per_xcd, uint32_t* chunk) {
   const uint32_t per_xcd = ((n_tiles + kXcds - 1) / kXcds + chunks_per_xcd - 1) / chunks_per_xcd * chunks_per_xcd;
@@ -90,14 +113,30 @@ void batch_step_t(const BatchLaunch& L, int parity) {
   auto k = L.policy == kNtAll ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtAll>
            : L.policy == kNtStores ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtStores>
                                    : batch_step_kernel<S, OP, LES, BZ, MACRO, kCached>;
+  unsigned grid = L.grid;
+  if constexpr (!MACRO) {
+    if (L.persistent && !L.lds && L.persistent < L.grid) {
+      k = L.policy == kNtAll ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtAll, false, true>
+          : L.policy == kNtStores ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtStores, false, true>
+                                  : batch_step_kernel<S, OP, LES, BZ, MACRO, kCached, false, true>;
+      grid = L.persistent;
+    }
+  }
   if constexpr (OP == kOpReg && LES) {
     if (L.lds)
       k = L.policy == kNtAll ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtAll, true>
           : L.policy == kNtStores ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtStores, true>
                                   : batch_step_kernel<S, OP, LES, BZ, MACRO, kCached, true>;
   }
-  hipLaunchKernelGGL(k, dim3(L.grid), dim3(kBlock), L.lds, L.stream, L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles,
-                     parity, L.seeds, L.chunk);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), L.lds, L.stream, L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles,
+                     parity, L.seeds, L.chunk, L.grid);
+}
+
+// Resident blocks of the persistent form for a collision: the kernel's waves per SIMD (its
+// register budget) over the waves of a block, on every CU, rounded down to a multiple of 8.
+template <int OP, bool LES>
+constexpr uint32_t persistent_blocks_per_cu() {
+  return (uint32_t)(kStepWaves<OP, LES> * 4 / (kBlock / 64));
 }
 
 template <class S, bool MACRO>

@@ -111,3 +111,29 @@ def test_batch_with_learned_closure_and_particles_equals_single_tunnels():
         for p in bparts:
             p.destroy()
         batch.destroy()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16"])
+def test_persistent_step_form_equals_one_block_per_tile(monkeypatch, precision):
+    """The batched step's persistent form (resident blocks walking the tiles) gives the
+    one-block-per-tile form's populations and force samples bit for bit."""
+    # enough tiles (5 216) that the resident blocks (7 per CU for REG + LES) each walk several
+    tunnels = [tunnel(*c) for c in [((128, 64, 64), "sphere", 1), ((96, 48, 48), "box", 2),
+                                     ((128, 64, 64), "bluff", 3), ((64, 32, 32), "sphere", 4)]]
+    shared = SolverParams(0.6, 0.05, True, operator="reg", precision=precision)
+    out = {}
+    for form in ("cell-nt-stores/16/persistent", "cell-nt-stores/16"):
+        monkeypatch.setenv("PW_BATCH_KERNEL", form)
+        batch = TunnelBatch.create([t[0] for t in tunnels], [t[1] for t in tunnels], [t[2] for t in tunnels], shared)
+        try:
+            variant = batch.variant
+            samples = batch.step_sampled(101, 8)
+            batch.step(1, True)
+            out[form] = (variant, samples, [batch.read_populations(k).copy() for k in range(len(tunnels))])
+        finally:
+            batch.destroy()
+    (vp, sp, pp), (vn, sn, pn) = out["cell-nt-stores/16/persistent"], out["cell-nt-stores/16"]
+    assert "/persistent-" in vp and "persistent" not in vn
+    assert np.array_equal(sp, sn)
+    for a, b in zip(pp, pn):
+        assert np.array_equal(a.view(np.uint16), b.view(np.uint16))
