@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "lattice.h"
@@ -453,6 +454,32 @@ bool finite_mesh(const float* p, size_t tris) {
 using namespace pw;
 using namespace pw::vox;
 
+namespace pw {
+namespace vox {
+
+// Pinned staging for the results' device-to-host copies: copies into pageable memory ran at
+// 2-4 GB/s (7.3 + 3.9 ms for the 2N geometry words of 256x128x128), into pinned memory at
+// PCIe rate. One buffer per process, grown on demand and kept; callers take it in turn.
+std::mutex g_staging_mutex;
+void* g_staging = nullptr;
+size_t g_staging_bytes = 0;
+
+hipError_t staging(size_t bytes, void** out) {
+  if (bytes > g_staging_bytes) {
+    if (g_staging) (void)hipHostFree(g_staging);
+    g_staging = nullptr;
+    g_staging_bytes = 0;
+    hipError_t e = hipHostMalloc(&g_staging, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    g_staging_bytes = bytes;
+  }
+  *out = g_staging;
+  return hipSuccess;
+}
+
+}  // namespace vox
+}  // namespace pw
+
 extern "C" pw_status pw_voxelize(const float* body, size_t body_tris, const float* supports, size_t support_tris,
                                  const pw_layout* layout, int floor, int link_fractions, pw_voxel_grid* out) {
   if (!layout || !out || (!body && body_tris) || (!supports && support_tris)) {
@@ -547,13 +574,22 @@ extern "C" pw_status pw_voxelize(const float* body, size_t body_tris, const floa
       status = PW_ENOMEM;
       goto done;
     }
-    if ((e = hipMemcpyAsync(out->geometry, flags, N * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
-    if ((e = hipMemcpyAsync(out->geometry + N, bases, N * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
-    if (n_links && (e = hipMemcpyAsync(out->geometry + 2 * N, q, n_links * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
-    if (n_bound && (e = hipMemcpyAsync(out->boundary_cells, bcells, n_bound * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
-    if ((e = hipMemcpyAsync(tot, totals, 32, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
-    if ((e = hipMemcpyAsync(&open_h, open, 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) goto fail;
+    {
+      // geometry (2N + links words) and boundary list through the pinned staging buffer
+      std::lock_guard<std::mutex> lock(g_staging_mutex);
+      const size_t words = out->geometry_words + n_bound;
+      uint32_t* stage = nullptr;
+      if ((e = staging(words * 4, reinterpret_cast<void**>(&stage))) != hipSuccess) goto fail;
+      if ((e = hipMemcpyAsync(stage, flags, N * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
+      if ((e = hipMemcpyAsync(stage + N, bases, N * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
+      if (n_links && (e = hipMemcpyAsync(stage + 2 * N, q, n_links * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
+      if (n_bound && (e = hipMemcpyAsync(stage + out->geometry_words, bcells, n_bound * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
+      if ((e = hipMemcpyAsync(tot, totals, 32, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
+      if ((e = hipMemcpyAsync(&open_h, open, 4, hipMemcpyDeviceToHost, st)) != hipSuccess) goto fail;
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) goto fail;
+      std::memcpy(out->geometry, stage, out->geometry_words * 4);
+      if (n_bound) std::memcpy(out->boundary_cells, stage + out->geometry_words, n_bound * 4);
+    }
     out->solid_cells = tot[0];
     out->frontal_cells = tot[1];
     out->blockage = (double)tot[2] / ((double)size[1] * size[2]);
@@ -569,6 +605,7 @@ extern "C" pw_status pw_voxelize(const float* body, size_t body_tris, const floa
   (void)hipStreamDestroy(st);
   return status;
 }
+
 
 extern "C" void pw_voxel_grid_free(pw_voxel_grid* g) {
   if (!g) return;

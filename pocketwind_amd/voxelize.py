@@ -24,6 +24,29 @@ def _mesh(m, name):
     return arr
 
 
+class _GridMemory:
+    """The library's arrays of one pw_voxel_grid, freed once the last numpy view of them is gone
+    (the geometry is 2N words and more; it is handed over without a copy)."""
+
+    def __init__(self, out):
+        self._out = out
+
+    def __del__(self):
+        try:
+            N.lib().pw_voxel_grid_free(C.byref(self._out))
+        except Exception:  # interpreter shutdown: the process frees it
+            pass
+
+
+def _view(pointer, count: int, owner: _GridMemory) -> np.ndarray:
+    if count == 0:
+        return np.zeros(0, np.uint32)
+    owned = type("_OwnedWords", (C.c_uint32 * count,), {})
+    words = owned.from_address(C.cast(pointer, C.c_void_p).value)
+    words._owner = owner  # the numpy array keeps `words`, which keeps the owner
+    return np.frombuffer(words, dtype=np.uint32)
+
+
 def voxelize(body, supports, layout, floor: bool, link_fractions: bool = True) -> VoxelGrid:
     nx, ny, nz = (int(v) for v in layout)
     b = _mesh(body, "Mesh")
@@ -35,11 +58,8 @@ def voxelize(body, supports, layout, floor: bool, link_fractions: bool = True) -
         None if s is None else (s.ctypes.data if s.size else None), 0 if s is None else s.size // 9,
         C.byref(lay), int(bool(floor)), int(bool(link_fractions)), C.byref(out))
     N.check(status)
-    try:
-        geo = np.ctypeslib.as_array(out.geometry, shape=(out.geometry_words,)).copy()
-        bc = (np.ctypeslib.as_array(out.boundary_cells, shape=(out.n_boundary,)).copy()
-              if out.n_boundary else np.zeros(0, np.uint32))
-        return VoxelGrid(nx, ny, nz, geo, bc, int(out.solid_cells), int(out.frontal_cells),
-                         float(out.blockage), bool(out.watertight), int(out.missed_links))
-    finally:
-        N.lib().pw_voxel_grid_free(C.byref(out))
+    owner = _GridMemory(out)
+    geo = _view(out.geometry, int(out.geometry_words), owner)
+    bc = _view(out.boundary_cells, int(out.n_boundary), owner)
+    return VoxelGrid(nx, ny, nz, geo, bc, int(out.solid_cells), int(out.frontal_cells),
+                     float(out.blockage), bool(out.watertight), int(out.missed_links))
