@@ -331,8 +331,8 @@ pw_status pw_batch_create(uint32_t count, const pw_layout *layouts, const uint32
                           const pw_params *shared, pw_batch **out);
 pw_status pw_batch_step(pw_batch *b, uint32_t count, int write_macro_on_last);
 /* The step variant the batch chose at create (timing each cache policy with 1, 16, 64 and 256
- * chunks per XCD, one block per tile and persistent, or
- * PW_BATCH_KERNEL=<cell|cell-nt|cell-nt-stores>/<chunks>[/persistent]): a pw_kernel value, and
+ * chunks per XCD, blocks of 64, 128 and 256 threads, one block per tile and persistent, or
+ * PW_BATCH_KERNEL=<cell|cell-nt|cell-nt-stores>/<chunks>[/<block>][/persistent]): a pw_kernel value, and
  * the chunks per XCD in *xcd_chunks (may be NULL). */
 uint32_t pw_batch_step_variant(const pw_batch *b, uint32_t *xcd_chunks);
 /* Resident blocks of the persistent form of the batched step when the tuner chose it (or
@@ -342,6 +342,10 @@ uint32_t pw_batch_step_variant(const pw_batch *b, uint32_t *xcd_chunks);
  * macros and the closure evaluated in the step always launch one block per tile. No reference
  * counterpart. */
 uint32_t pw_batch_persistent_grid(const pw_batch *b);
+/* Threads per block of the batched step, which is also the cells per tile (64, 128 or 256),
+ * chosen with the variant at create (PW_BATCH_KERNEL=<kernel>/<chunks>/<block> fixes it). No
+ * reference counterpart. */
+uint32_t pw_batch_block_size(const pw_batch *b);
 /* As pw_solver_step_sampled for every tunnel: `out` receives [tunnel][sample] x
  * (step, fx, fy, fz), 4 doubles each. */
 pw_status pw_batch_step_sampled(pw_batch *b, uint32_t count, uint32_t every, double *out,

@@ -79,6 +79,7 @@ SIGNATURES = {
     "pw_solver_set_step_variant": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     "pw_solver_xcd_chunks": (C.c_uint32, [C.c_void_p]),
     "pw_batch_persistent_grid": (C.c_uint32, [C.c_void_p]),
+    "pw_batch_block_size": (C.c_uint32, [C.c_void_p]),
     "pw_solver_kernel": (C.c_uint32, [C.c_void_p]),
     "pw_solver_synchronize": (C.c_int, [C.c_void_p]),
     "pw_solver_step_sampled": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t]),

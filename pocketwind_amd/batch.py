@@ -68,11 +68,12 @@ class TunnelBatch:
 
     @property
     def variant(self) -> str:
-        """The step variant chosen at create: cache policy / chunks per XCD [/ persistent grid]."""
+        """The step variant chosen at create: cache policy / chunks per XCD / block [/ persistent grid]."""
         chunks = C.c_uint32()
         code = int(N.lib().pw_batch_step_variant(self._h, C.byref(chunks)))
         persistent = int(N.lib().pw_batch_persistent_grid(self._h))
-        name = f"{ {1: 'cell', 2: 'cell-nt', 3: 'cell-nt-stores'}.get(code, '?') }/xcd-chunks-{chunks.value}"
+        block = int(N.lib().pw_batch_block_size(self._h))
+        name = f"{ {1: 'cell', 2: 'cell-nt', 3: 'cell-nt-stores'}.get(code, '?') }/xcd-chunks-{chunks.value}/block-{block}"
         return name + (f"/persistent-{persistent}" if persistent else "")
 
     def read_populations(self, tunnel: int) -> np.ndarray:

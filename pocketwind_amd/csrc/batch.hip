@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(const Tunnel* __rest
   for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     const uint32_t t = tile_tunnel[tile];
     const Tunnel& tu = tunnels[t];
-    const uint32_t n = (tile - tile_first[t]) * kBlock + threadIdx.x;
+    const uint32_t n = (tile - tile_first[t]) * blockDim.x + threadIdx.x;
     if (n >= tu.p.cells) continue;
     const uint32_t kind = tu.geo[n] & kTypeMask;
     const float ux = is_wall(kind) ? 0.0f : tu.p.u_in;
@@ -123,6 +123,8 @@ struct pw_batch {
   uint32_t chunk = 1;           // tiles per XCD chunk of the step grid
   unsigned step_grid = 0;
   uint32_t persistent = 0;      // resident blocks of the persistent step form, 0: one block per tile
+  uint32_t block = kBlock;      // threads per step block = cells per tile (the tile table's)
+  std::vector<uint32_t> tunnel_cells;  // cells of each tunnel (the tile tables are built from them)
   uint32_t cus = 0;             // compute units of the device
   uint32_t* part_tunnel = nullptr;
   uint32_t n_partials = 0;
@@ -151,7 +153,7 @@ namespace {
 
 void launch_step(pw_batch* b, int parity, bool macro) {
   const BatchLaunch L{b->step_grid, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, b->seeds,
-                      b->params, b->policy, b->chunk, b->lds, b->persistent};
+                      b->params, b->policy, b->chunk, b->lds, b->persistent, b->block};
   switch (b->params.storage) {
     case PW_STORAGE_F16: batch_step_s<StoreF16>(L, parity, macro); break;
     case PW_STORAGE_BF16: batch_step_s<StoreBF16>(L, parity, macro); break;
@@ -171,9 +173,9 @@ void launch_init(pw_batch* b) {
   const unsigned grid = b->grid;
   for (int set = 0; set < 2; ++set) {
     switch (b->params.storage) {
-      case PW_STORAGE_F16: hipLaunchKernelGGL(batch_init_kernel<StoreF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); launch_edges<StoreF16>(b, set); break;
-      case PW_STORAGE_BF16: hipLaunchKernelGGL(batch_init_kernel<StoreBF16>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); launch_edges<StoreBF16>(b, set); break;
-      default: hipLaunchKernelGGL(batch_init_kernel<StoreF32>, dim3(grid), dim3(kBlock), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); launch_edges<StoreF32>(b, set); break;
+      case PW_STORAGE_F16: hipLaunchKernelGGL(batch_init_kernel<StoreF16>, dim3(grid), dim3(b->block), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); launch_edges<StoreF16>(b, set); break;
+      case PW_STORAGE_BF16: hipLaunchKernelGGL(batch_init_kernel<StoreBF16>, dim3(grid), dim3(b->block), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); launch_edges<StoreBF16>(b, set); break;
+      default: hipLaunchKernelGGL(batch_init_kernel<StoreF32>, dim3(grid), dim3(b->block), 0, b->stream, b->tunnels, b->tile_tunnel, b->tile_first, b->n_tiles, set); launch_edges<StoreF32>(b, set); break;
     }
   }
 }
@@ -248,7 +250,7 @@ hipError_t apply_closure_mode(pw_batch* b) {
     p.closure_slot = b->closure_bufs[t].first;  // the kernel's buffer, or the records in the step
     p.closure_tau = fused ? nullptr : b->closure_bufs[t].second;
     p.closure_geo = b->host[t].wall_geo;
-    if (fused) b->lds = (kBlock / 64) * closure::kFusedWaveBytes;
+    if (fused) b->lds = (b->block / 64) * closure::kFusedWaveBytes;
     else b->closure_kernels = true;
   }
   for (auto& g : b->graph)
@@ -261,12 +263,34 @@ hipError_t apply_closure_mode(pw_batch* b) {
   return e;
 }
 
+// Tile tables for `block` cells per tile (one block of that many threads per tile): the tiles
+// of every tunnel in order, each tunnel's first tile. Uploaded on the batch's stream.
+hipError_t set_block(pw_batch* b, uint32_t block) {
+  if (block == b->block && b->n_tiles) return hipSuccess;
+  std::vector<uint32_t> tt, tf(b->count);
+  for (uint32_t t = 0; t < b->count; ++t) {
+    tf[t] = (uint32_t)tt.size();
+    const size_t n = (b->tunnel_cells[t] + block - 1) / block;
+    tt.insert(tt.end(), n, t);
+  }
+  hipError_t e = hipMemcpyAsync(b->tile_tunnel, tt.data(), tt.size() * 4, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->tile_first, tf.data(), b->count * 4, hipMemcpyHostToDevice, b->stream);
+  // the host vectors must outlive the copies
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return e;
+  b->block = block;
+  b->n_tiles = (uint32_t)tt.size();
+  b->grid = (unsigned)std::min<uint64_t>(b->n_tiles, 0x7fffffffu);
+  return hipSuccess;
+}
+
 // Resident blocks of the persistent step form on this device: the collision's blocks per CU
 // times the CUs, a multiple of 8.
 uint32_t persistent_grid(const pw_batch* b) {
   const bool reg = b->params.op == PW_OP_REG, les = b->params.les != 0;
-  const uint32_t per_cu = reg ? (les ? persistent_blocks_per_cu<kOpReg, true>() : persistent_blocks_per_cu<kOpReg, false>())
-                              : (les ? persistent_blocks_per_cu<kOpTrt, true>() : persistent_blocks_per_cu<kOpTrt, false>());
+  const uint32_t B = b->block;
+  const uint32_t per_cu = reg ? (les ? persistent_blocks_per_cu<kOpReg, true>(B) : persistent_blocks_per_cu<kOpReg, false>(B))
+                              : (les ? persistent_blocks_per_cu<kOpTrt, true>(B) : persistent_blocks_per_cu<kOpTrt, false>(B));
   return per_cu * b->cus / kXcds * kXcds;
 }
 
@@ -283,9 +307,18 @@ bool forced_batch_variant(pw_batch* b, const char* text) {
   uint32_t chunks = b->xcd_chunks;
   if (a != std::string::npos) chunks = (uint32_t)std::strtoul(v.c_str() + a + 1, nullptr, 10);
   if (!valid_xcd_chunks(chunks)) return false;
-  const size_t c = a == std::string::npos ? a : v.find('/', a + 1);
-  const bool persist = c != std::string::npos && v.substr(c + 1) == "persistent";
-  if (c != std::string::npos && !persist) return false;
+  // further tokens: a block size (64, 128, 256) and/or "persistent"
+  uint32_t block = kBlock;
+  bool persist = false;
+  for (size_t c = a == std::string::npos ? a : v.find('/', a + 1); c != std::string::npos;) {
+    const size_t d = v.find('/', c + 1);
+    const std::string tok = v.substr(c + 1, d == std::string::npos ? std::string::npos : d - c - 1);
+    if (tok == "persistent") persist = true;
+    else if (tok == "64" || tok == "128" || tok == "256") block = (uint32_t)std::strtoul(tok.c_str(), nullptr, 10);
+    else return false;
+    c = d;
+  }
+  if (set_block(b, block) != hipSuccess) return false;
   b->policy = policy;
   b->xcd_chunks = chunks;
   b->persistent = persist ? persistent_grid(b) : 0u;
@@ -304,42 +337,47 @@ hipError_t tune_batch(pw_batch* b) {
   }();
   const int reps = cells >= (1u << 22) ? 4 : 16;
   float best = 0;
-  uint32_t best_policy = kNtStores, best_chunks = 16, best_persistent = 0;
-  const uint32_t resident = persistent_grid(b);
-  for (uint32_t chunks : kXcdChunkings)
-    for (uint32_t pol : policies)
-      for (uint32_t persist : {0u, resident}) {
-        if (chunks > 1 && (uint64_t)chunks * 2 * kXcds > b->n_tiles) continue;
-        if (persist && (persist >= batch_grid(b->n_tiles, chunks, &b->chunk) || resident == 0)) continue;
-        b->policy = pol;
-        b->xcd_chunks = chunks;
-        b->persistent = persist;
-        b->step_grid = batch_grid(b->n_tiles, chunks, &b->chunk);
-        for (int w = 0; w < 2; ++w) {
-          launch_step(b, b->parity, false);
-          b->parity ^= 1;
-        }
-        float fastest = 0;
-        for (int round = 0; round < 2; ++round) {
-          hipError_t e = hipEventRecord(b->t0, b->stream);
-          if (e != hipSuccess) return e;
-          for (int r = 0; r < reps; ++r) {
+  uint32_t best_policy = kNtStores, best_chunks = 16, best_persistent = 0, best_block = kBlock;
+  for (uint32_t block : kBatchBlocks) {
+    if (hipError_t e = set_block(b, block)) return e;
+    const uint32_t resident = persistent_grid(b);
+    for (uint32_t chunks : kXcdChunkings)
+      for (uint32_t pol : policies)
+        for (uint32_t persist : {0u, resident}) {
+          if (chunks > 1 && (uint64_t)chunks * 2 * kXcds > b->n_tiles) continue;
+          if (persist && (persist >= batch_grid(b->n_tiles, chunks, &b->chunk) || resident == 0)) continue;
+          b->policy = pol;
+          b->xcd_chunks = chunks;
+          b->persistent = persist;
+          b->step_grid = batch_grid(b->n_tiles, chunks, &b->chunk);
+          for (int w = 0; w < 2; ++w) {
             launch_step(b, b->parity, false);
             b->parity ^= 1;
           }
-          if ((e = hipEventRecord(b->t1, b->stream)) != hipSuccess) return e;
-          if ((e = hipEventSynchronize(b->t1)) != hipSuccess) return e;
-          float ms = 0;
-          if ((e = hipEventElapsedTime(&ms, b->t0, b->t1)) != hipSuccess) return e;
-          if (round == 0 || ms < fastest) fastest = ms;
+          float fastest = 0;
+          for (int round = 0; round < 2; ++round) {
+            hipError_t e = hipEventRecord(b->t0, b->stream);
+            if (e != hipSuccess) return e;
+            for (int r = 0; r < reps; ++r) {
+              launch_step(b, b->parity, false);
+              b->parity ^= 1;
+            }
+            if ((e = hipEventRecord(b->t1, b->stream)) != hipSuccess) return e;
+            if ((e = hipEventSynchronize(b->t1)) != hipSuccess) return e;
+            float ms = 0;
+            if ((e = hipEventElapsedTime(&ms, b->t0, b->t1)) != hipSuccess) return e;
+            if (round == 0 || ms < fastest) fastest = ms;
+          }
+          if (best == 0 || fastest < best) {
+            best = fastest;
+            best_policy = pol;
+            best_chunks = chunks;
+            best_persistent = persist;
+            best_block = block;
+          }
         }
-        if (best == 0 || fastest < best) {
-          best = fastest;
-          best_policy = pol;
-          best_chunks = chunks;
-          best_persistent = persist;
-        }
-      }
+  }
+  if (hipError_t e = set_block(b, best_block)) return e;
   b->policy = best_policy;
   b->xcd_chunks = best_chunks;
   b->persistent = best_persistent;
@@ -388,7 +426,7 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   b->params = *shared;
   b->count = count;
   for (uint32_t t = 0; t < count; ++t) {
-    tiles += (sps[t].cells + kBlock - 1) / kBlock;
+    tiles += (sps[t].cells + 63) / 64;  // the smallest tile the tuner tries
     partials += (n_boundary[t] + kForceBlock - 1) / kForceBlock;
   }
   if (tiles > 0xffffffffull || partials > 0x7fffffffull) {
@@ -478,11 +516,12 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
     part_base += tu.n_partials;
     tf[t] = (uint32_t)tt.size();
     for (size_t k = 0; k < (N + kBlock - 1) / kBlock; ++k) tt.push_back(t);
+    b->tunnel_cells.push_back((uint32_t)N);
   }
   b->n_tiles = (uint32_t)tt.size();
   b->n_partials = (uint32_t)pt.size();
   if ((e = dev_alloc(b, &b->tunnels, count)) != hipSuccess) return fail(e, "batch table");
-  if ((e = dev_alloc(b, &b->tile_tunnel, tt.size())) != hipSuccess) return fail(e, "batch table");
+  if ((e = dev_alloc(b, &b->tile_tunnel, (size_t)tiles)) != hipSuccess) return fail(e, "batch table");  // room for 64-cell tiles
   if ((e = dev_alloc(b, &b->tile_first, count)) != hipSuccess) return fail(e, "batch table");
   if ((e = dev_alloc(b, &b->part_tunnel, pt.size())) != hipSuccess) return fail(e, "batch table");
   if ((e = dev_alloc(b, &b->partials, 3 * (size_t)(pt.size() ? pt.size() : 1))) != hipSuccess) return fail(e, "batch partials");
@@ -516,6 +555,8 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   *out = b;
   return PW_OK;
 }
+
+uint32_t pw_batch_block_size(const pw_batch* b) { return b ? b->block : 0u; }
 
 uint32_t pw_batch_persistent_grid(const pw_batch* b) {
   return (b && b->persistent && b->persistent < b->step_grid) ? b->persistent : 0u;  // as batch_step_t launches

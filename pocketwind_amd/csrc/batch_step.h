@@ -8,7 +8,8 @@
 namespace pw {
 namespace batch {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;  // largest block of a batched step (the kernels' launch bound)
+constexpr uint32_t kBatchBlocks[] = {64, 128, 256};  // the block sizes the tuner times
 
 struct Tunnel {
   StepParams p;  // p.closure_tau / p.closure_slot (closure kernel) or p.closure_net (in the step) once attached
@@ -40,6 +41,7 @@ struct BatchLaunch {
   // resident blocks (a multiple of 8) walk the `grid` block indices of the one-per-tile form;
   // 0 launches one block per tile.
   uint32_t persistent;
+  uint32_t block;  // threads per block = cells per tile (64, 128 or 256)
 };
 
 // One step of every tunnel. The rounding seed is the step count, read on the device so a
@@ -61,7 +63,7 @@ __device__ __forceinline__ void batch_block(const Tunnel* __restrict__ tunnels, 
   if (tile >= n_tiles) return;
   const uint32_t t = tile_tunnel[tile];
   const Tunnel& tu = tunnels[t];
-  const uint32_t n = (tile - tile_first[t]) * kBlock + threadIdx.x;
+  const uint32_t n = (tile - tile_first[t]) * blockDim.x + threadIdx.x;
   if (n >= tu.p.cells) return;  // whole wavefronts on row tunnels (their cell count is a multiple of 64)
   const auto* in = (const typename S::T*)tu.pops[parity];
   auto* out = (typename S::T*)tu.pops[parity ^ 1];
@@ -128,15 +130,15 @@ void batch_step_t(const BatchLaunch& L, int parity) {
           : L.policy == kNtStores ? batch_step_kernel<S, OP, LES, BZ, MACRO, kNtStores, true>
                                   : batch_step_kernel<S, OP, LES, BZ, MACRO, kCached, true>;
   }
-  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), L.lds, L.stream, L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles,
+  hipLaunchKernelGGL(k, dim3(grid), dim3(L.block), L.lds, L.stream, L.tunnels, L.tile_tunnel, L.tile_first, L.n_tiles,
                      parity, L.seeds, L.chunk, L.grid);
 }
 
-// Resident blocks of the persistent form for a collision: the kernel's waves per SIMD (its
-// register budget) over the waves of a block, on every CU, rounded down to a multiple of 8.
+// Resident blocks per CU of the persistent form for a collision: the kernel's waves per SIMD
+// (its register budget) on the CU's 4 SIMDs over the waves of a block.
 template <int OP, bool LES>
-constexpr uint32_t persistent_blocks_per_cu() {
-  return (uint32_t)(kStepWaves<OP, LES> * 4 / (kBlock / 64));
+constexpr uint32_t persistent_blocks_per_cu(uint32_t block) {
+  return (uint32_t)(kStepWaves<OP, LES> * 4) / (block / 64);
 }
 
 template <class S, bool MACRO>
