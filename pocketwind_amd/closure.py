@@ -456,6 +456,43 @@ def source_floor(reynolds: float = 320.0, layout=(256, 128, 128), length: int = 
             "split_law": coarse["split"], "split_2x_grid": fine["split"], "wall_cells": int(len(coarse["cells"]))}
 
 
+def wall_rule_cd(reynolds: float = 320.0, layout=(256, 128, 128), length: int = 64, u_in: float = 0.05,
+                 flow_throughs: int = 6, average: int = 2, log=print) -> list:
+    """The slanted bluff body on `layout` and on the twice finer grid with each wall rule (halfway
+    bounce-back, Bouzidi), with and without the LES wall model, REG collision, no closure: the
+    window-mean Cd of both grids and the largest velocity component of the last state."""
+    from . import bodies, drag
+    from .gpu_solver import GpuSolver, SolverParams
+    from .voxelize import voxelize
+
+    def cd(grid_layout, grid_length, walls, les):
+        body, _ = bodies.place_mesh(bodies.bluff_body(), None, grid_length, grid_layout, False)
+        grid = voxelize(body, None, grid_layout, False, link_fractions=True)
+        nu = u_in * grid_length / reynolds
+        s = GpuSolver.create(grid_layout, grid, SolverParams(3 * nu + 0.5, u_in, les, operator="reg", walls=walls))
+        try:
+            flow = drag.flow_through_steps(grid.nx, u_in)
+            samples = s.step_sampled(flow_throughs * flow, 8)
+            fx, _, _ = drag.window_mean_force(samples, flow_throughs * flow, average * flow)
+            s.step(1, True)
+            umax = float(np.abs(s.read_macro()[:, 1:]).max())
+        finally:
+            s.destroy()
+        return fx * drag.cd_per_lattice_force(grid.frontal_cells, u_in), umax
+
+    rows = []
+    fine_layout = tuple(2 * v for v in layout)
+    for les in (True, False):
+        for walls in ("bb", "bouzidi"):
+            c, uc = cd(layout, length, walls, les)
+            f, uf = cd(fine_layout, 2 * length, walls, les)
+            rows.append({"les": les, "walls": walls, "cd": c, "cd_2x_grid": f, "vs_2x_grid_pct": 100 * (c / f - 1),
+                         "max_velocity": uc, "max_velocity_2x_grid": uf})
+            log(f"  LES {les}, walls {walls}: Cd {c:.5f}, 2x grid {f:.5f} ({100 * (c / f - 1):+.2f}%), "
+                f"largest |u| {uc:.3f} / {uf:.3f}")
+    return rows
+
+
 def main(argv=None):
     """Train the closure on coarse/fine pairs of random bodies and save it.
 
@@ -475,6 +512,8 @@ def main(argv=None):
     ap.add_argument("--flow-throughs", type=float, default=4.0)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--wall-rules", type=float, default=None, metavar="RE",
+                    help="Cd of the bluff body on both grids with each wall rule, with and without LES")
     ap.add_argument("--source-floor", type=float, default=None, metavar="RE",
                     help="measure the Cd a perfect momentum-source closure gives at this Reynolds number")
     ap.add_argument("--shear-floor", type=float, default=None, metavar="RE",
@@ -485,6 +524,13 @@ def main(argv=None):
                     help="instead of training, compare Cd of the learned closure with the wall law "
                          "and the twice finer grid at this Reynolds number (the closure file --out)")
     args = ap.parse_args(argv)
+    if args.wall_rules is not None:
+        rec = wall_rule_cd(args.wall_rules, tuple(args.coarse), args.length, log=lambda m: print(m, flush=True))
+        print(json.dumps(rec))
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump(rec, f, indent=1)
+        return 0
     if args.source_floor is not None:
         rec = source_floor(args.source_floor, tuple(args.coarse), args.length, log=lambda m: print(m, flush=True))
         print(json.dumps(rec, indent=1, default=float))
