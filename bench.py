@@ -31,7 +31,8 @@ the 2x finer grid (512x256x256, L = 128, the same Re), the metric's Cd reference
 
 The CPU baseline is the FP64 C restatement of the reference CPU solver (the oracle) stepping
 the same body grid single-threaded, like the reference's JS path, for a bounded number of
-steps.
+steps. Beside it: scripts/drag-cpu.ts's small Ahmed case and the laminar sphere in 96x48x48
+(free-slip sides, zero-gradient outlet), each timed on the device and on the oracle.
 """
 from __future__ import annotations
 
@@ -212,6 +213,47 @@ def drag_cpu_case(oracle_steps: int = 480):
                                                        f"operator, {oracle_steps} steps on one thread"}}
 
 
+C1_LAYOUT, C1_CENTRE, C1_DIAMETER, C1_REYNOLDS = (96, 48, 48), (32, 24, 24), 12, 100
+
+
+def sphere_case(steps: int = 4000, oracle_seconds: float = 5.0):
+    """The small laminar sphere (D = 12 at (32, 24, 24) in 96x48x48, Re 100, REG, f32, free-slip
+    sides, zero-gradient outlet): device MLUPS over `steps` steps timed with HIP events on the
+    solver's stream (median of three), beside the oracle stepping the same grid on one thread for
+    about `oracle_seconds`. At 221 184 cells a step is a few microseconds, so this line shows the
+    launch-bound end of the device, not its bandwidth."""
+    import oracle
+    from pocketwind_amd.gpu_solver import GpuSolver, SolverParams
+    from pocketwind_amd.lattice import grid_from_kinds, sphere_kinds
+    grid = grid_from_kinds(C1_LAYOUT, sphere_kinds(C1_LAYOUT, C1_CENTRE, C1_DIAMETER))
+    tau0 = 3 * (U_IN * C1_DIAMETER / C1_REYNOLDS) + 0.5
+    solver = GpuSolver.create(C1_LAYOUT, grid, SolverParams(tau0, U_IN, False, operator="reg",
+                                                            sides="free-slip", outlet="zero-gradient"))
+    try:
+        solver.step(200, False)
+        ms = sorted(solver.time_steps(steps) for _ in range(3))[1]
+        solver.step(1, True)
+        finite = bool(np.all(np.isfinite(solver.read_macro())))
+        variant = f"{solver.kernel}/{solver.block_size}/xcd-chunks-{solver.xcd_chunks}"
+    finally:
+        solver.destroy()
+    cpu = oracle.Solver(C1_LAYOUT, tau0, oracle.REG, False, flags=grid.flags,
+                        equilibrium_state=(1.0, (U_IN, 0.0, 0.0)), threads=1,
+                        sides="free-slip", outlet="zero-gradient")
+    cpu.step(1)
+    n, elapsed, t0 = 0, 0.0, time.perf_counter()
+    while elapsed < oracle_seconds:
+        cpu.step(4)
+        n += 4
+        elapsed = time.perf_counter() - t0
+    return {"layout": list(C1_LAYOUT), "operator": "reg", "storage": "f32", "sides": "free-slip",
+            "outlet": "zero-gradient", "device_mlups": round(grid.cells * steps / (ms * 1e-3) / 1e6, 1),
+            "us_per_step": round(ms * 1e3 / steps, 2), "variant": variant, "finite": finite,
+            "cpu_baseline": {"value": round(grid.cells * n / elapsed / 1e6, 3), "unit": "MLUPS", "cores": 1,
+                             "kind": "port", "sample": f"FP64 C oracle, the same grid and boundaries, {n} steps "
+                                                       f"in {elapsed:.1f} s on one thread"}}
+
+
 def _oracle_mlups(grid, threads: int, seconds_target: float):
     import oracle
     cpu = oracle.Solver(grid.layout, TAU0, oracle.REG, True, flags=grid.flags,
@@ -356,6 +398,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(grid)
         line["voxelize"] = voxelize_timing()
         line["drag_cpu_case"] = drag_cpu_case()
+        line["sphere_case"] = sphere_case()
+        print(f"# sphere case: {line['sphere_case']}", file=sys.stderr, flush=True)
         print(f"# voxelize: {line['voxelize']}", file=sys.stderr, flush=True)
         print(f"# drag-cpu case: {line['drag_cpu_case']}", file=sys.stderr, flush=True)
     print(json.dumps(line))

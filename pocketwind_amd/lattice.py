@@ -99,3 +99,40 @@ def step_bytes(grid, precision: str) -> int:
     if grid.nx % 64:
         return pops + 4 * cells
     return pops + 4 * (cells // 64) + 4 * 64 * general_runs(grid.flags)
+
+
+def grid_from_kinds(layout, kinds) -> VoxelGrid:
+    """Packed geometry of a lattice given by its cell kinds, with halfway walls (every q offset
+    NO_Q): each fluid cell's flag carries the wall mask of its pulls (bit 3 + i when the cell at
+    x - c_i, wrapping, is SOLID, SUPPORT or FLOOR), and the boundary list is the fluid cells with
+    any wall link, as tests/browser/force.spec.ts:40-72 builds its grids."""
+    nx, ny, nz = (int(v) for v in layout)
+    cells = nx * ny * nz
+    kinds = np.asarray(kinds, dtype=np.uint32).reshape(-1)
+    if kinds.size != cells:
+        raise ValueError("One kind per cell is needed.")
+    geometry = np.zeros(2 * cells, dtype=np.uint32)
+    geometry[:cells] = kinds
+    geometry[cells:] = NO_Q
+    # Rolling the wall field by c brings the source x - c of every cell to x.
+    grid3 = kinds.reshape(nz, ny, nx) & CELL_TYPE_MASK
+    wall = (grid3 >= SOLID) & (grid3 <= FLOOR)
+    mask = np.zeros((nz, ny, nx), dtype=np.uint32)
+    for i, (cx, cy, cz) in enumerate(DIRECTIONS):
+        mask |= np.roll(wall, (int(cz), int(cy), int(cx)), axis=(0, 1, 2)).astype(np.uint32) << np.uint32(3 + i)
+    fluid = kinds == FLUID
+    geometry[:cells][fluid] = mask.reshape(-1)[fluid]
+    boundary = np.flatnonzero(fluid & (mask.reshape(-1) != 0)).astype(np.uint32)
+    return VoxelGrid(nx, ny, nz, geometry, boundary, solid_cells=int(np.sum(kinds == SOLID)))
+
+
+def sphere_kinds(layout, centre, diameter) -> np.ndarray:
+    """Cell kinds of an analytic sphere in a channel: SOLID where the cell centre lies strictly
+    inside the sphere, the planes x = 0 and x = nx - 1 EQ, FLUID elsewhere."""
+    nx, ny, nz = (int(v) for v in layout)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r2 = (x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2
+    kinds = np.where(r2 < (diameter / 2) ** 2, SOLID, FLUID).astype(np.uint32)
+    kinds[:, :, 0] = EQ
+    kinds[:, :, nx - 1] = EQ
+    return kinds.reshape(-1)

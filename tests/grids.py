@@ -2,9 +2,9 @@
 
 blob_grid        tests/browser/gpu-vs-cpu.spec.ts:76-116 (LCG seed 0x4a3b2c1d, radius 5 + 2u)
 link_fractions   tests/browser/gpu-vs-cpu.spec.ts:57-74  (seed 0x1b873593, every 7th q = 0.5)
-grid_from_kinds  tests/browser/force.spec.ts:40-72
+grid_from_kinds  tests/browser/force.spec.ts:40-72 (pocketwind_amd.lattice, re-exported)
 sphere_kinds     an analytic sphere in a channel with EQ inlet and outlet planes (no reference
-                 counterpart; the free-slip / zero-gradient channel cases)
+                 counterpart; the free-slip / zero-gradient channel cases; pocketwind_amd.lattice)
 """
 from __future__ import annotations
 
@@ -12,8 +12,8 @@ import math
 
 import numpy as np
 
-from pocketwind_amd.lattice import (CELL_TYPE_MASK, DIRECTIONS, EQ, FLOOR, FLUID, NO_Q, SOLID,
-                                    SUPPORT, VoxelGrid)
+from pocketwind_amd.lattice import (CELL_TYPE_MASK, DIRECTIONS, EQ, FLOOR, FLUID, SOLID, SUPPORT,
+                                    VoxelGrid, grid_from_kinds, sphere_kinds)  # noqa: F401 (re-exported)
 
 
 def lcg(seed: int) -> int:
@@ -29,27 +29,6 @@ def pull_wall_mask(kinds: np.ndarray, layout, cell: int) -> int:
         if SOLID <= (int(kinds[src]) & CELL_TYPE_MASK) <= FLOOR:
             mask |= 1 << (3 + i)
     return mask
-
-
-def grid_from_kinds(layout, kinds) -> VoxelGrid:
-    """Packed geometry with halfway q offsets (NO_Q) and the boundary list of fluid cells
-    with any wall link, as force.spec.ts builds it."""
-    nx, ny, nz = layout
-    cells = nx * ny * nz
-    kinds = np.asarray(kinds, dtype=np.uint32)
-    geometry = np.zeros(2 * cells, dtype=np.uint32)
-    geometry[:cells] = kinds
-    geometry[cells:] = NO_Q
-    # pull_wall_mask for every cell at once: rolling by c brings the source x - c to x.
-    grid3 = kinds.reshape(nz, ny, nx) & CELL_TYPE_MASK
-    wall = (grid3 >= SOLID) & (grid3 <= FLOOR)
-    mask = np.zeros((nz, ny, nx), dtype=np.uint32)
-    for i, (cx, cy, cz) in enumerate(DIRECTIONS):
-        mask |= np.roll(wall, (cz, cy, cx), axis=(0, 1, 2)).astype(np.uint32) << np.uint32(3 + i)
-    fluid = kinds == FLUID
-    geometry[:cells][fluid] = mask.reshape(-1)[fluid]
-    boundary = np.flatnonzero(fluid & (mask.reshape(-1) != 0)).astype(np.uint32)
-    return VoxelGrid(nx, ny, nz, geometry, boundary, solid_cells=int(np.sum(kinds == SOLID)))
 
 
 def blob_kinds(layout=(32, 32, 32)) -> np.ndarray:
@@ -93,14 +72,3 @@ def blob_grid(walls: str = "bb") -> VoxelGrid:
     layout = (32, 32, 32)
     grid = grid_from_kinds(layout, blob_kinds(layout))
     return link_fractions(grid) if walls == "bouzidi" else grid
-
-
-def sphere_kinds(layout, centre, diameter) -> np.ndarray:
-    """Solid where the cell centre lies strictly inside the sphere; x = 0 and x = nx - 1 EQ."""
-    nx, ny, nz = layout
-    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    r2 = (x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2
-    kinds = np.where(r2 < (diameter / 2) ** 2, SOLID, FLUID).astype(np.uint32)
-    kinds[:, :, 0] = EQ
-    kinds[:, :, nx - 1] = EQ
-    return kinds.reshape(-1)

@@ -165,3 +165,27 @@ def test_unknown_or_destroyed_handles_are_rejected_without_being_read():
     out = C.c_void_p()
     assert lib.pw_particles_create(stale, 1, 1, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1),
                                    C.byref(out)) == N.PW_EINVAL
+
+
+def test_grid_from_kinds_matches_the_per_cell_wall_mask():
+    """The packed flags of grid_from_kinds (rolled wall fields) equal force.spec.ts's per-cell
+    loop over the 19 pulls with periodic wrap (grids.pull_wall_mask) on the seeded blob, whose
+    SUPPORT and FLOOR cells count as walls, and the analytic C1 sphere has the solid and boundary
+    counts the oracle's golden run recorded."""
+    import json
+
+    from grids import blob_kinds, pull_wall_mask, sphere_kinds
+    from pocketwind_amd.lattice import FLUID
+    layout = (32, 32, 32)
+    kinds = blob_kinds(layout)
+    g = grid_from_kinds(layout, kinds)
+    for cell in range(kinds.size):
+        expected = pull_wall_mask(kinds, layout, cell) if kinds[cell] == FLUID else int(kinds[cell])
+        assert int(g.geometry[cell]) == expected, cell
+    assert np.all(g.geometry[kinds.size:] == NO_Q)
+    fluid_walled = [c for c in range(kinds.size) if kinds[c] == FLUID and g.geometry[c] >> 3]
+    assert g.boundary_cells.tolist() == fluid_walled
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "c1_sphere_oracle.json")))
+    sphere = grid_from_kinds(golden["layout"], sphere_kinds(golden["layout"], golden["centre"], golden["diameter"]))
+    assert sphere.solid_cells == golden["solid_cells"]
+    assert sphere.boundary_cells.size == golden["boundary_cells"]
