@@ -425,11 +425,15 @@ pw_status pw_solver_set_closure_interval(pw_solver *s, uint32_t every);
  * used than under KERNEL (the first step after attaching or reset evaluates in line, and its
  * output also serves the step that starts the first overlapped evaluation). The latency-bound
  * closure kernel then runs within the HBM-bound step instead of before it. Each step call joins
- * the evaluation it started before it returns. No reference counterpart. */
+ * the evaluation it started before it returns. ALONGSIDE: OVERLAPPED's schedule and outputs bit
+ * for bit, with the evaluation in step k's own launch: its first blocks run the network over the
+ * wall-cell list while the others step, so no second stream or event is involved (the REG step;
+ * TRT solvers keep the kernel). No reference counterpart. */
 typedef enum {
   PW_CLOSURE_KERNEL = 0,
   PW_CLOSURE_IN_STEP = 1,
-  PW_CLOSURE_OVERLAPPED = 2
+  PW_CLOSURE_OVERLAPPED = 2,
+  PW_CLOSURE_ALONGSIDE = 3
 } pw_closure_placement;
 pw_status pw_solver_set_closure_placement(pw_solver *s, uint32_t placement);
 /* Test hook: the wall collision takes the given vectors (x, y, z, w per wall cell, in the

@@ -9,56 +9,15 @@
 namespace pw {
 namespace closure {
 
-// Wall cells per wavefront: one 16-row matrix-core tile. The wall cells are few (thousands),
-// so one tile per wavefront spreads them over many more wavefronts than four tiles would, and
-// each wavefront runs one network pass instead of four in sequence.
-constexpr int kCellsPerWave = 16;
-
-// One wavefront per 16 wall cells: features (optionally written raw to `records`), then, when
-// a network is attached (P non-null), its stress for every cell into tau (w = 1 valid, 0 use
-// the wall law). Lanes 0-15 own the cells; all 64 lanes run the network's tile.
+// The closure kernel: one wavefront per 16-cell tile of the wall-cell list.
 template <class S, bool BOUZIDI>
 __global__ __launch_bounds__(64) void closure_kernel(const typename S::T* __restrict__ in,
                                                      const uint32_t* __restrict__ geo, StepParams p,
                                                      const WallGeo* __restrict__ walls, uint32_t n_cells,
                                                      const mlp::Packed* __restrict__ P, Norm norm,
                                                      float4* __restrict__ tau, float* __restrict__ records) {
-  __shared__ __attribute__((aligned(16))) uint16_t xs[kCellsPerWave * mlp::kInPad];
-  __shared__ __attribute__((aligned(16))) uint16_t h1[16 * mlp::kHidden];
-  __shared__ __attribute__((aligned(16))) uint16_t h2[16 * mlp::kHidden];
-  __shared__ float ys[kCellsPerWave * 4];
-  const int l = threadIdx.x;
-  const uint32_t k = blockIdx.x * kCellsPerWave + l;
-  const bool valid = l < kCellsPerWave && k < n_cells;
-  float f[kFeatures], law[3];
-  bool in_range = valid;
-  if (valid) {
-    wall_features<S, BOUZIDI>(in, geo, p, norm.u_ref, load_wall_geo(walls, k), f, law);
-    if (records) {
-      for (int j = 0; j < kFeatures; ++j) records[(size_t)k * kRecord + j] = f[j];
-      for (int j = 0; j < 3; ++j) records[(size_t)k * kRecord + kFeatures + j] = law[j];
-    }
-  }
-  if (!P) return;
-  if (l < kCellsPerWave)
-    for (int j = 0; j < mlp::kInPad; ++j) {
-      float v = 0.0f;
-      if (valid && j < kFeatures) {
-        v = (f[j] - norm.mean[j]) * norm.scale[j];
-        in_range = in_range && fabsf(v) <= norm.limit;
-      }
-      xs[l * mlp::kInPad + j] = mlp::bf16_bits(v);
-    }
-  mlp::wave_sync();
-  const int r = l & 15, gq = l >> 4;
-  const mlp::f32x4_t y = mlp::forward16(P, xs, h1, h2);
-  if (r < 3)
-    for (int j = 0; j < 4; ++j) ys[(4 * gq + j) * 4 + r] = y[j];
-  mlp::wave_sync();
-  if (!valid) return;
-  const float tx = ys[l * 4] * norm.out_scale, ty = ys[l * 4 + 1] * norm.out_scale, tz = ys[l * 4 + 2] * norm.out_scale;
-  const bool ok = in_range && isfinite(tx) && isfinite(ty) && isfinite(tz);
-  tau[k] = make_float4(tx, ty, tz, ok ? 1.0f : 0.0f);
+  __shared__ __attribute__((aligned(16))) char lds[kFusedWaveBytes];
+  closure_tile<S, BOUZIDI>(in, geo, p, walls, n_cells, P, norm, tau, records, blockIdx.x, lds);
 }
 
 // Momentum-exchange force on the body from each wall cell, the mean of both states (the

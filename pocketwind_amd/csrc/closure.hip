@@ -74,6 +74,21 @@ bool in_step_closure(const pw_solver* s) {
          s->params.les && !s->sp.bc;
 }
 
+// The closure runs beside the step when asked to and the step has instances that carry it: the
+// REG collision (a learned solver has LES and the reference's boundaries).
+bool alongside_closure(const pw_solver* s) {
+  return s->closure_on == 1 && s->closure_placement == PW_CLOSURE_ALONGSIDE && s->params.op == PW_OP_REG &&
+         s->params.les && !s->sp.bc;
+}
+
+// Blocks of a step launch that carry the closure beside the step: one 16-cell tile per
+// wavefront, rounded up to a multiple of the XCD count so the step blocks keep their XCD.
+uint32_t alongside_blocks(size_t n_wall, uint32_t block) {
+  const uint64_t per_block = block / 64;
+  const uint64_t blocks = (closure_blocks(n_wall) + per_block - 1) / per_block;
+  return (uint32_t)((blocks + kXcds - 1) / kXcds * kXcds);
+}
+
 pw_status check(pw_solver* s) {
   if (!solver_live(s)) {
     set_error("Null or destroyed solver handle.");
@@ -123,7 +138,23 @@ void launch_closure(pw_solver* s, StepParams& p) {
     return;
   }
   // closure_on 2: the output was prescribed from the host (pw_debug_set_wall_stress).
-  if (s->closure_on == 1 && s->closure_placement == PW_CLOSURE_OVERLAPPED) {
+  if (alongside_closure(s)) {
+    // As OVERLAPPED, with the evaluation in the step's own launch (its first blocks) instead of
+    // on a second stream: the output of an evaluation beside the previous step replaces the
+    // current one before this step.
+    settle_closure(s);
+    if (!s->closure_fresh) {
+      launch_closure_any(s, s->pops[s->current], s->closure_net, s->closure_tau, nullptr);
+      s->closure_fresh = 1;
+    } else if (s->steps % s->closure_every == 0) {
+      p.closure_net = s->closure_net;
+      p.closure_geo = s->closure_geo;
+      p.closure_out = s->closure_next;
+      p.closure_cells = (uint32_t)s->n_wall;
+      p.closure_blocks = alongside_blocks(s->n_wall, s->block);
+      s->closure_pending = 2;
+    }
+  } else if (s->closure_on == 1 && s->closure_placement == PW_CLOSURE_OVERLAPPED) {
     // The evaluation started beside the previous step is joined before this one; the first
     // step of a run evaluates in line. An evaluation due at this step reads the state this step
     // reads, on the side stream, while this step runs on the stream with the previous output;
@@ -170,7 +201,7 @@ void launch_sources(pw_solver* s, void* set) {
 
 void settle_closure(pw_solver* s) {
   if (!s->closure_pending) return;
-  (void)hipStreamWaitEvent(s->stream, s->closure_done, 0);
+  if (s->closure_pending == 1) (void)hipStreamWaitEvent(s->stream, s->closure_done, 0);
   std::swap(s->closure_tau, s->closure_next);
   s->closure_pending = 0;
 }
@@ -326,7 +357,7 @@ pw_status pw_solver_set_closure(pw_solver* s, const pw_mlp* net, const float* no
 
 pw_status pw_solver_set_closure_placement(pw_solver* s, uint32_t placement) {
   if (pw_status st = check(s)) return st;
-  if (placement > PW_CLOSURE_OVERLAPPED) {
+  if (placement > PW_CLOSURE_ALONGSIDE) {
     set_error("Unknown closure placement.");
     return PW_EINVAL;
   }

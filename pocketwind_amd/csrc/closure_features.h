@@ -273,5 +273,58 @@ __device__ __forceinline__ float4 fused_stress(const char* lds) {
   return reinterpret_cast<const float4*>(lds + kFusedH1)[threadIdx.x & 63u];
 }
 
+// Wall cells per wavefront: one 16-row matrix-core tile. The wall cells are few (thousands),
+// so one tile per wavefront spreads them over many more wavefronts than four tiles would, and
+// each wavefront runs one network pass instead of four in sequence.
+constexpr int kCellsPerWave = 16;
+
+// One 16-cell tile of the wall-cell list (cells 16 tile .. 16 tile + 15) by one whole
+// wavefront: lanes 0-15 gather their cell's features (written raw to `records` when given),
+// then, when a network is attached (P non-null), all 64 lanes run its matrix-core tile and lanes
+// 0-15 store their stress into tau (w = 1 valid, 0 use the wall law). `lds` is this
+// wavefront's kFusedWaveBytes of LDS (h1, the inputs that h2 overwrites, the outputs).
+template <class S, bool BOUZIDI>
+__device__ __forceinline__ void closure_tile(const typename S::T* __restrict__ in, const uint32_t* __restrict__ geo,
+                                             const StepParams& p, const WallGeo* __restrict__ walls,
+                                             uint32_t n_cells, const mlp::Packed* __restrict__ P, const Norm& norm,
+                                             float4* __restrict__ tau, float* __restrict__ records, uint32_t tile,
+                                             char* lds) {
+  uint16_t* h1 = reinterpret_cast<uint16_t*>(lds + kFusedH1);
+  uint16_t* xs = reinterpret_cast<uint16_t*>(lds + kFusedX);
+  float* ys = reinterpret_cast<float*>(lds + kFusedY);
+  const int l = threadIdx.x & 63;
+  const uint32_t k = tile * kCellsPerWave + l;
+  const bool valid = l < kCellsPerWave && k < n_cells;
+  float f[kFeatures], law[3];
+  bool in_range = valid;
+  if (valid) {
+    wall_features<S, BOUZIDI>(in, geo, p, norm.u_ref, load_wall_geo(walls, k), f, law);
+    if (records) {
+      for (int j = 0; j < kFeatures; ++j) records[(size_t)k * kRecord + j] = f[j];
+      for (int j = 0; j < 3; ++j) records[(size_t)k * kRecord + kFeatures + j] = law[j];
+    }
+  }
+  if (!P) return;
+  if (l < kCellsPerWave)
+    for (int j = 0; j < mlp::kInPad; ++j) {
+      float v = 0.0f;
+      if (valid && j < kFeatures) {
+        v = (f[j] - norm.mean[j]) * norm.scale[j];
+        in_range = in_range && fabsf(v) <= norm.limit;
+      }
+      xs[l * mlp::kInPad + j] = mlp::bf16_bits(v);
+    }
+  mlp::wave_sync();
+  const int r = l & 15, gq = l >> 4;
+  const mlp::f32x4_t y = mlp::forward16(P, xs, h1, xs);
+  if (r < 3)
+    for (int j = 0; j < 4; ++j) ys[(4 * gq + j) * 4 + r] = y[j];
+  mlp::wave_sync();
+  if (!valid) return;
+  const float tx = ys[l * 4] * norm.out_scale, ty = ys[l * 4 + 1] * norm.out_scale, tz = ys[l * 4 + 2] * norm.out_scale;
+  const bool ok = in_range && isfinite(tx) && isfinite(ty) && isfinite(tz);
+  tau[k] = make_float4(tx, ty, tz, ok ? 1.0f : 0.0f);
+}
+
 }  // namespace closure
 }  // namespace pw

@@ -145,6 +145,9 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
   out->closure_net = nullptr;
   out->tile_order = nullptr;
   out->closure_geo = nullptr;
+  out->closure_out = nullptr;
+  out->closure_cells = 0;
+  out->closure_blocks = 0;
   out->bc = (prm.sides == PW_SIDES_FREE_SLIP ? kBcFreeSlip : 0u) |
             (prm.outlet == PW_OUTLET_ZERO_GRADIENT ? kBcZeroGradient : 0u);
   return true;
@@ -155,6 +158,7 @@ bool pack_params(const pw_layout& L, const pw_params& prm, double u_in, StepPara
 std::vector<uint32_t> wave_descriptors(const pw_layout& L, const uint32_t* flags) {
   const size_t runs = (size_t)L.nx * L.ny * L.nz / 64;
   std::vector<uint32_t> desc(runs, 0u);
+  uint32_t walls = 0;  // below kMaxRowWallCells (validate_tunnel)
   for (size_t r = 0; r < runs; ++r) {
     const uint32_t* f = flags + 64 * r;
     uint32_t d = 0;
@@ -164,7 +168,8 @@ std::vector<uint32_t> wave_descriptors(const pw_layout& L, const uint32_t* flags
       else if (l == 63 && f[l] == kEq) d |= kWaveEqLast;
       else d = kWaveGeneral;
     }
-    desc[r] = d;
+    desc[r] = d | (walls << kWaveBaseShift);
+    for (uint32_t l = 0; l < 64; ++l) walls += (f[l] & kTypeMask) == kFluid && (f[l] >> 3) != 0;
   }
   return desc;
 }
@@ -466,6 +471,14 @@ pw_status validate_tunnel(const pw_layout& L, const uint32_t* geometry, size_t g
   if (!pack_params(L, params, params.u_in, full)) return PW_EINVAL;
   if (!validate_geometry(L, geometry, geometry_words)) return PW_EINVAL;
   if (params.sides == PW_SIDES_FREE_SLIP && !walls_clear_of_sides(L, geometry)) return PW_EINVAL;
+  if (full->rows) {
+    uint64_t walls = 0;
+    for (size_t c = 0; c < full->cells; ++c) walls += (geometry[c] & kTypeMask) == kFluid && (geometry[c] >> 3) != 0;
+    if (walls >= kMaxRowWallCells) {
+      set_error("A lattice whose nx is a multiple of 64 holds at most 2^29 - 1 wall cells (fluid cells with wall links).");
+      return PW_EINVAL;
+    }
+  }
   const size_t N = (size_t)L.nx * L.ny * L.nz;
   // gpu-force.ts:76-81
   for (size_t k = 0; k < n_boundary; ++k) {

@@ -82,13 +82,23 @@ struct StepParams {
   // The wall cells' static closure geometry (closure::WallGeo records, by wall-cell index);
   // read with closure_slot by the closure evaluated in the step.
   const void* closure_geo;
+  // The closure evaluated beside the step (non-null closure_out): the launch's first
+  // closure_blocks blocks run the network (closure_net) over the closure_cells wall cells
+  // (closure_geo) on the state the step reads, into closure_out, while the other blocks step
+  // with closure_tau.
+  float4* closure_out;
+  uint32_t closure_cells, closure_blocks;
 };
 
 // Descriptor of a 64-cell run on the row path. 0: every cell is a fluid cell without wall
 // links (flag 0). kWaveEqFirst / kWaveEqLast: the same, except that the first / last cell is an
 // EQ cell (flag kEq, the inlet and outlet planes). kWaveGeneral: anything else; the step reads
-// each cell's flag.
+// each cell's flag. Bits kWaveBaseShift and up: the number of wall cells (fluid cells with wall
+// links) in the runs before this one, i.e. the wall-cell list index of the run's first wall
+// cell, so a wall lane finds its entry of a per-wall-cell buffer without a per-cell index.
 enum : uint32_t { kWaveGeneral = 1u, kWaveEqFirst = 2u, kWaveEqLast = 4u };
+constexpr uint32_t kWaveBaseShift = 3;
+constexpr uint64_t kMaxRowWallCells = (uint64_t)1 << (32 - kWaveBaseShift);  // wall cells a row lattice may hold
 // The descriptors of a lattice whose nx is a multiple of 64, from its cell flags.
 std::vector<uint32_t> wave_descriptors(const pw_layout& layout, const uint32_t* flags);
 
@@ -182,11 +192,14 @@ struct pw_solver {
   int closure_fresh = 0;       // closure_tau holds an evaluation of this run
   uint32_t closure_placement = 0;  // pw_closure_placement
   // PW_CLOSURE_OVERLAPPED: the evaluation runs on side_stream beside the step that reads the
-  // same state, into closure_next, which replaces closure_tau before the following step.
+  // same state, into closure_next, which replaces closure_tau before the following step
+  // (PW_CLOSURE_ALONGSIDE: the same, in the step's own launch).
   hipStream_t side_stream = nullptr;
   hipEvent_t state_ready = nullptr, closure_done = nullptr;
   float4* closure_next = nullptr;
-  int closure_pending = 0;  // closure_next is being written; the main stream has not joined it
+  // closure_next is being written and has not replaced closure_tau: 1 on side_stream (the main
+  // stream joins closure_done first), 2 by the last step launch (PW_CLOSURE_ALONGSIDE).
+  int closure_pending = 0;
   // pw_debug_set_wall_stress momentum sources (w = 2), added after each step (f32 storage)
   float4* closure_source = nullptr;
   int closure_sources = 0;

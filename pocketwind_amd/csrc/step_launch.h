@@ -40,6 +40,20 @@ void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const St
       };
       k = macro ? pick_c(std::true_type{}) : pick_c(std::false_type{});
     }
+    // The learned closure evaluated beside the step (launch_closure sets p.closure_out only for
+    // the reference's boundaries).
+    if (p.closure_out) {
+      auto pick_a = [&](auto with_macro) {
+        constexpr bool M = decltype(with_macro)::value;
+        switch (s->kernel) {
+          case PW_KERNEL_CELL_NT: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtAll, false, false, true>;
+          case PW_KERNEL_CELL_NT_STORES:
+            return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kNtStores, false, false, true>;
+          default: return stream_collide_kernel<S, OP, LES, BOUZIDI, M, kCached, false, false, true>;
+        }
+      };
+      k = macro ? pick_a(std::true_type{}) : pick_a(std::false_type{});
+    }
   }
   const uint32_t block = s->block;
   const uint64_t blocks = (s->cells + block - 1) / block;
@@ -49,7 +63,7 @@ void launch_step_t(pw_solver* s, const void* in, void* out, bool macro, const St
   const uint32_t per_xcd = (uint32_t)(((blocks + kXcds - 1) / kXcds + parts - 1) / parts * parts);
   const uint32_t chunk = per_xcd / parts;
   const uint32_t lds = p.closure_net ? block / 64 * closure::kFusedWaveBytes : 0;
-  hipLaunchKernelGGL(k, dim3(kXcds * per_xcd), dim3(block), lds, s->stream, (const typename S::T*)in,
+  hipLaunchKernelGGL(k, dim3(kXcds * per_xcd + p.closure_blocks), dim3(block), lds, s->stream, (const typename S::T*)in,
                      (typename S::T*)out, s->geometry, s->macro, p, seed, chunk);
 }
 

@@ -22,18 +22,31 @@ struct RowWave {
 // its neighbour lane's value with a DPP wavefront shift (wave_shr:1 moves lane l - 1 to lane
 // l, wave_shl:1 lane l + 1 to lane l); the lane at the wavefront's edge takes the value one
 // load brings for the whole wavefront. All 64 lanes must be active.
-// Pull of direction I for a row wavefront from the source row `src` of its plane.
+// Pull of direction I for a row wavefront from the source row `src` of its plane, in two halves:
+// the loads (the lane's own column and, for c_x != 0, the edge value), then the shift and decode.
 template <class S, bool NT, int I>
-__device__ __forceinline__ float pull_row(const typename S::T* src, uint32_t xs, uint32_t left, uint32_t right) {
-  const uint32_t own = load_bits<S, NT>(byte_offset(src, xs));
+__device__ __forceinline__ void pull_row_load(const typename S::T* src, uint32_t xs, uint32_t left, uint32_t right,
+                                              uint32_t& own, uint32_t& edge) {
+  own = load_bits<S, NT>(byte_offset(src, xs));
+  if constexpr (cx(I) != 0) edge = load_bits<S, NT>(byte_offset(src, cx(I) > 0 ? left : right));
+}
+
+template <class S, int I>
+__device__ __forceinline__ float pull_row_value(uint32_t own, uint32_t edge) {
   if constexpr (cx(I) == 0) {
     return decode_bits<S>(own);
   } else {
-    const uint32_t edge = load_bits<S, NT>(byte_offset(src, cx(I) > 0 ? left : right));
     const int v = cx(I) > 0 ? __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x138, 0xf, 0xf, false)
                             : __builtin_amdgcn_update_dpp((int)edge, (int)own, 0x130, 0xf, 0xf, false);
     return decode_bits<S>((uint32_t)v);
   }
+}
+
+template <class S, bool NT, int I>
+__device__ __forceinline__ float pull_row(const typename S::T* src, uint32_t xs, uint32_t left, uint32_t right) {
+  uint32_t own, edge = 0;
+  pull_row_load<S, NT, I>(src, xs, left, right, own, edge);
+  return pull_row_value<S, I>(own, edge);
 }
 
 // Sources of lane 0 for c_x = +1 and of lane 63 for c_x = -1 (the zero-gradient outlet holds
@@ -51,9 +64,15 @@ __device__ __forceinline__ void row_columns(const StepParams& p, uint32_t bc, ui
 // row (nine rows feed a D3Q19 pull), which the loads and stores carry as immediates.
 constexpr uint32_t kRowRun = 64;
 
+// The raw loads of a row wavefront's pulls, in flight until gather_row_values takes them, so
+// that other loads (a learned solver's stress) can be issued between the two.
+struct RowPulls {
+  uint32_t own[kQ], edge[kQ];
+};
+
 template <class S, bool NT>
-__device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in, const StepParams& p, uint32_t bc,
-                                           const RowWave& w, uint32_t lane, float* g) {
+__device__ __forceinline__ void gather_row_loads(const typename S::T* __restrict__ in, const StepParams& p, uint32_t bc,
+                                                 const RowWave& w, uint32_t lane, RowPulls& r) {
   constexpr int SH = S::kShift;
   const uint32_t row = kQ * p.nx, plane = row * p.ny;
   const uint32_t y = w.y, z = w.z;
@@ -65,7 +84,17 @@ __device__ __forceinline__ void gather_row(const typename S::T* __restrict__ in,
   row_columns<SH>(p, bc, w.x0, lane, left, right, xs);
   static_for<0, kQ>([&](auto c) {
     constexpr int i = decltype(c)::value;
-    g[i] = pull_row<S, NT, i>(byte_offset(in, yo[1 - cy(i)] + zo[1 - cz(i)]) + i * kRowRun, xs, left, right);
+    r.edge[i] = 0;
+    pull_row_load<S, NT, i>(byte_offset(in, yo[1 - cy(i)] + zo[1 - cz(i)]) + i * kRowRun, xs, left, right, r.own[i],
+                            r.edge[i]);
+  });
+}
+
+template <class S>
+__device__ __forceinline__ void gather_row_values(const RowPulls& r, float* g) {
+  static_for<0, kQ>([&](auto c) {
+    constexpr int i = decltype(c)::value;
+    g[i] = pull_row_value<S, i>(r.own[i], r.edge[i]);
   });
 }
 
@@ -137,7 +166,25 @@ __device__ __forceinline__ void store_row(typename S::T* __restrict__ out, uint3
   }
 }
 
-// stream_collide_cell for a row wavefront: the whole wavefront gathers (gather_row), then fluid
+// The closure buffer's stress for the wall lanes (fluid cells with wall links) of a row
+// wavefront, the only lanes whose collision reads it. The wall lanes of a run are consecutive in
+// the wall-cell list from the base its descriptor carries (a row lattice always has descriptors
+// when a buffer is attached), so a wall lane's entry is that base plus the wall lanes below it.
+// A wavefront with wall lanes loads on every lane (the others take the base's entry, which
+// exists) without a branch, so the load waits for the flag alone and its result is first waited
+// for where a wall collision uses it. All 64 lanes must be active.
+__device__ __forceinline__ float4 row_stress(const StepParams& p, uint32_t desc, uint32_t flag) {
+  float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const bool wall_lane = (flag & kTypeMask) == kFluid && (flag >> 3) != 0;
+  const uint64_t walls = __ballot(wall_lane);
+  if (p.closure_tau && walls) {
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(walls >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)walls, 0u));
+    t = p.closure_tau[(desc >> kWaveBaseShift) + (wall_lane ? rank : 0u)];
+  }
+  return t;
+}
+
+// stream_collide_cell for a row wavefront: the whole wavefront gathers (gather_row_loads, gather_row_values), then fluid
 // lanes replace wall-link directions and collide, EQ and wall lanes take their fixed states,
 // and every lane stores. Every load that does not depend on another (the pulls, the flags, the
 // outlet's upstream moments) is issued before any of them is waited for, so a wavefront waits
@@ -171,15 +218,21 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     const uint32_t f0 = geo[n];
     closure::fused_closure<S, BOUZIDI>(in, geo, p, n, (f0 & kTypeMask) == kFluid && (f0 >> 3) != 0, lds);
   }
-  float g[kQ];
-  const uint32_t bc = BC ? p.bc : 0u;
-  gather_row<S, POL == kNtAll>(in, p, bc, w, lane, g);
+  // The flag first: a learned solver's stress load depends on it and can then be in flight
+  // with the pulls.
   uint32_t flag;
   if (desc & kWaveGeneral)
     flag = geo[n];
   else
     flag = ((lane == 0 && (desc & kWaveEqFirst)) || (lane == 63 && (desc & kWaveEqLast))) ? (uint32_t)kEq : 0u;
+  const uint32_t bc = BC ? p.bc : 0u;
+  RowPulls pulls;
+  gather_row_loads<S, POL == kNtAll>(in, p, bc, w, lane, pulls);
   const uint32_t kind = flag & kTypeMask;
+  float4 learned = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if constexpr (!CLOSURE && LES) learned = row_stress(p, desc, flag);
+  float g[kQ];
+  gather_row_values<S>(pulls, g);
   const bool outlet_wave = last_wave && (__builtin_amdgcn_readlane(kind, 63) == kEq);
   if (BC && (p.bc & kBcFreeSlip) && on_side(w.y, w.z, p)) slip_row<S, POL == kNtAll>(in, p, w, lane, g);
   const uint32_t x = w.x0 + lane;
@@ -189,11 +242,8 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     for (int i = 1; i < kQ; ++i)
       if (has_wall(flag, i)) g[i] = wall_value<S, BOUZIDI>(in, geo, p, nb, n, w.e0 + lane, flag, i);
   }
-  float4 learned = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if constexpr (CLOSURE) {
     if (fused && kind == kFluid && (flag >> 3) != 0) learned = closure::fused_stress(lds);
-  } else if constexpr (LES) {
-    learned = buffered_stress(p, flag, n);
   }
   float4 m;
   if constexpr (OP == kOpReg) {
@@ -244,22 +294,42 @@ template <int OP, bool LES>
 constexpr int kStepWaves = (LES && OP == kOpTrt) ? 4 : 7;
 
 // BC: the instance for boundary options (p.bc != 0); the reference's boundaries take BC = false,
-// which compiles the option code out of the step.
-template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool BC, bool CLOSURE = false>
+// which compiles the option code out of the step. ALONG: the instance whose first
+// p.closure_blocks blocks evaluate the learned closure over the wall-cell list (one 16-cell tile
+// per wavefront, closure::closure_tile) into p.closure_out while the rest of the grid steps; the
+// launch dispatches them first, so their dependent gathers overlap the step's streaming instead
+// of preceding it. p.closure_blocks is a multiple of the XCD count, so the step blocks keep
+// their XCD.
+template <class S, int OP, bool LES, bool BOUZIDI, bool MACRO, int POL, bool BC, bool CLOSURE = false,
+          bool ALONG = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<OP, LES>))) void stream_collide_kernel(const typename S::T* __restrict__ in,
                                                              typename S::T* __restrict__ out,
                                                              const uint32_t* __restrict__ geo,
                                                              float4* __restrict__ macro, StepParams p,
                                                              uint32_t seed, uint32_t chunk) {
-  const uint32_t b = blockIdx.x;
+  // Dynamic LDS of the closure instances: closure::kFusedWaveBytes per wavefront.
+  extern __shared__ __attribute__((aligned(16))) char step_lds[];
+  uint32_t b = blockIdx.x;
+  if constexpr (ALONG) {
+    if (b < p.closure_blocks) {
+      const uint32_t tile = b * (blockDim.x >> 6) + (threadIdx.x >> 6);
+      if (tile * closure::kCellsPerWave < p.closure_cells)  // wavefront-uniform: whole tiles only
+        closure::closure_tile<S, BOUZIDI>(
+            in, geo, p, static_cast<const closure::WallGeo*>(p.closure_geo), p.closure_cells,
+            static_cast<const mlp::Packed*>(p.closure_net),
+            *reinterpret_cast<const closure::Norm*>(static_cast<const char*>(p.closure_net) + closure::kNormOffset),
+            p.closure_out, nullptr, tile, step_lds + (threadIdx.x >> 6) * closure::kFusedWaveBytes);
+      return;
+    }
+    b -= p.closure_blocks;
+  }
   const uint32_t local = b / kXcds;
   uint32_t logical = ((local / chunk) * kXcds + b % kXcds) * chunk + local % chunk;
   if constexpr (CLOSURE) logical = p.tile_order[logical];
   const uint32_t n = logical * blockDim.x + threadIdx.x;
   if (n >= p.cells) return;  // whole wavefronts when nx is a multiple of 64
   if constexpr (CLOSURE) {
-    // Dynamic LDS: closure::kFusedWaveBytes per wavefront (the launch guarantees p.rows).
-    extern __shared__ __attribute__((aligned(16))) char step_lds[];
+    // The launch guarantees p.rows.
     stream_collide_row_wave<S, OP, LES, BOUZIDI, MACRO, POL, BC, true>(
         in, out, geo, macro, p, n, seed, step_lds + (threadIdx.x >> 6) * closure::kFusedWaveBytes);
   } else if (p.rows)

@@ -297,13 +297,15 @@ class GpuSolver:
         norm = np.ascontiguousarray(norm, np.float32)
         N.check(N.lib().pw_solver_set_closure(self._live(), net._h, norm.ctypes.data, norm.size))
 
-    PLACEMENTS = ("kernel", "step", "overlapped")
+    PLACEMENTS = ("kernel", "step", "overlapped", "alongside")
 
     def set_closure_placement(self, placement: str) -> None:
         """Where the closure runs: "kernel" (its own kernel, every `interval` steps; the default),
-        "step" (inside the step's wall wavefronts, every step; REG + LES, nx a multiple of 64) or
+        "step" (inside the step's wall wavefronts, every step; REG + LES, nx a multiple of 64),
         "overlapped" (its kernel on a second stream beside the step that reads the same state;
-        the output serves the `interval` steps after that one)."""
+        the output serves the `interval` steps after that one) or "alongside" (the same
+        schedule and outputs as "overlapped", evaluated by the first blocks of that step's own
+        launch; REG)."""
         if placement not in self.PLACEMENTS:
             raise ValueError(f"Unknown closure placement {placement!r}.")
         N.check(N.lib().pw_solver_set_closure_placement(self._live(), self.PLACEMENTS.index(placement)))

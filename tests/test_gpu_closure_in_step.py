@@ -111,6 +111,6 @@ def test_placement_is_validated():
     try:
         with pytest.raises(ValueError):
             s.set_closure_placement("wall")
-        assert N.lib().pw_solver_set_closure_placement(s._live(), 3) == N.PW_EINVAL
+        assert N.lib().pw_solver_set_closure_placement(s._live(), 4) == N.PW_EINVAL
     finally:
         s.destroy()
