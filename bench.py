@@ -19,10 +19,11 @@ f32 and 80 B at f16/bf16 storage, over every cell of the lattice.
 
 Headline: f16 deviation storage with the learned wall closure (the network shipped in
 pocketwind_amd/closure_c2.npz, trained by `python -m pocketwind_amd.closure` on coarse/fine
-pairs of other bodies at this resolution), against plain f32 storage with the wall law. Its
-step is the step kernel plus, every 8 steps, the closure kernel over the wall cells (the wall
-stress is reused in between); the same with the closure evaluated before every step is reported
-beside it. The roofline is the step kernel's, timed in the f16 run without the closure.
+pairs of other bodies at this resolution), against plain f32 storage with the wall law. Every
+8 steps the closure is evaluated over the wall cells by the first blocks of that step's own
+launch, on the state the step reads, and its stress serves the 8 steps after it; the same with
+an evaluation in every step's launch is reported beside it. The roofline is the step kernel's,
+timed in the f16 run without the closure.
 
 Cd: six flow-throughs (6 ceil(nx / u_in) steps) with an FP64 force sample every 8 steps,
 mean over the last two (src/validation/cases.ts:294-320), Cd = fx / (1/2 u^2 A) with A the
@@ -78,6 +79,9 @@ def empty_grid(layout):
 
 CLOSURE = os.path.join(ROOT, "pocketwind_amd", "closure_c2.npz")
 CLOSURE_EVERY = 8  # steps between closure evaluations (the wall stress is reused in between)
+# The closure runs in the first blocks of an evaluating step's own launch, on the state that step
+# reads; its output serves the next CLOSURE_EVERY steps (GpuSolver.set_closure_placement).
+CLOSURE_PLACEMENT = "alongside"
 
 
 def body_grid(layout=LAYOUT, length=BODY_LENGTH):
@@ -95,7 +99,7 @@ def params_for(precision: str, learned: bool = False, tau0: float = TAU0):
                         wall_model="learned" if learned else "law")
 
 
-def make_solver(grid, params, closure=None, every=CLOSURE_EVERY):
+def make_solver(grid, params, closure=None, every=CLOSURE_EVERY, placement=CLOSURE_PLACEMENT):
     from pocketwind_amd.gpu_solver import GpuSolver
     solver = GpuSolver.create(grid.layout, grid, params)
     if not isinstance(solver, GpuSolver):
@@ -103,11 +107,13 @@ def make_solver(grid, params, closure=None, every=CLOSURE_EVERY):
     if closure is not None:
         solver.set_closure(*closure)
         solver.set_closure_interval(every)
+        solver.set_closure_placement(placement)
     return solver
 
 
-def time_solver(grid, params, steps, warmup, repeats=3, closure=None, every=CLOSURE_EVERY):
-    solver = make_solver(grid, params, closure, every)
+def time_solver(grid, params, steps, warmup, repeats=3, closure=None, every=CLOSURE_EVERY,
+                placement=CLOSURE_PLACEMENT):
+    solver = make_solver(grid, params, closure, every, placement)
     try:
         solver.step(warmup, False)
         solver.synchronize()
@@ -146,9 +152,9 @@ def measured_traffic(precision: str, variant: str):
     return table.get(f"{precision}/{kernel}/{chunking}")
 
 
-def body_cd(grid, params, closure=None):
+def body_cd(grid, params, closure=None, every=CLOSURE_EVERY):
     from pocketwind_amd import drag
-    solver = make_solver(grid, params, closure)
+    solver = make_solver(grid, params, closure, every)
     try:
         flow = drag.flow_through_steps(grid.nx, U_IN)
         samples = solver.step_sampled(6 * flow, 8)
@@ -283,6 +289,25 @@ def cpu_baseline(grid, seconds_target: float = 15.0, parallel_seconds: float = 8
                                    f"{par_elapsed:.1f} s"}}
 
 
+def time_placements(steps: int, warmup: int) -> int:
+    """The f16 learned-closure step on the body grid under each closure placement, beside the f16
+    wall-law step (one line each on stderr)."""
+    from pocketwind_amd import closure as closure_mod
+    grid = body_grid()
+    net, norm, _ = closure_mod.load(CLOSURE)
+    law = time_solver(grid, params_for("f16"), steps, warmup)
+    print(f"# f16 wall law: {law['launch_us']:.2f} us/step, {law['variant']}", file=sys.stderr, flush=True)
+    for every in (CLOSURE_EVERY, 1):
+        for placement in ("kernel", "overlapped", "alongside"):
+            r = time_solver(grid, params_for("f16", learned=True), steps, warmup, closure=(net, norm), every=every,
+                            placement=placement)
+            print(f"# f16 learned, {placement}, every {every}: {r['launch_us']:.2f} us/step "
+                  f"(+{r['launch_us'] - law['launch_us']:.2f}), {r['variant']}, finite={r['finite']}",
+                  file=sys.stderr, flush=True)
+    net.destroy()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,6 +318,9 @@ def main():
                          "grid, then one short copy probe, and nothing else (profiling runs)")
     ap.add_argument("--grid", choices=["body", "empty"], default="body",
                     help="grid of an --only run: the bluff body or the reference's all-fluid grid")
+    ap.add_argument("--placements", action="store_true",
+                    help="time the f16 learned-closure step on the body grid under each closure placement "
+                         "(kernel, overlapped, alongside) at intervals 8 and 1, and nothing else (profiling runs)")
     ap.add_argument("--no-cd", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -307,6 +335,8 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         return 1
 
+    if args.placements:
+        return time_placements(args.steps, args.warmup)
     grid = empty_grid(LAYOUT) if (args.only and args.grid == "empty") else body_grid()
     precisions = args.only.split(",") if args.only else ["f16", "f32", "bf16"]
     if any(p not in ("f16", "f32", "bf16") for p in precisions):
@@ -355,7 +385,7 @@ def main():
         "config": {"workload": "bluff body L=64 (1.0x0.37x0.28 L, rounded nose, 25 deg slant) in a "
                                "256x128x128 D3Q19 tunnel, REG + Smagorinsky Cs 0.1, halfway bounce-back, "
                                "Re 320 (tau0 0.53), u_in 0.05, device-voxelised",
-                   "precision": "f16 deviation storage", "wall_closure": f"learned (bf16 MFMA MLP, evaluated every {CLOSURE_EVERY} steps), wall-law fallback",
+                   "precision": "f16 deviation storage", "wall_closure": f"learned (bf16 MFMA MLP, evaluated every {CLOSURE_EVERY} steps in the step launch), wall-law fallback",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(step16["gbps"], 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(step16["gbps"] / HBM_PEAK_GBPS, 4),
@@ -379,6 +409,7 @@ def main():
     }
     if not args.no_cd:
         cds = {"f16_learned": body_cd(grid, params_for("f16", learned=True), closure=(net, norm)),
+               "f16_learned_every_step": body_cd(grid, params_for("f16", learned=True), closure=(net, norm), every=1),
                "f16_law": body_cd(grid, params_for("f16")),
                "f32_law": body_cd(grid, params_for("f32")),
                "bf16_law": body_cd(grid, params_for("bf16"))}
@@ -390,7 +421,7 @@ def main():
         print(f"# Cd f32_law_2x_grid: {cds['f32_law_2x_grid']:.6f}", file=sys.stderr, flush=True)
         ref = cds["f32_law_2x_grid"]
         line["cd"] = {k: round(v, 6) for k, v in cds.items()}
-        for k in ("f16_learned", "f16_law", "f32_law", "bf16_law"):
+        for k in ("f16_learned", "f16_learned_every_step", "f16_law", "f32_law", "bf16_law"):
             line["cd"][f"{k}_vs_2x_grid_pct"] = round(100 * (cds[k] / ref - 1), 3)
         line["cd"]["f16_law_vs_f32_law_pct"] = round(100 * (cds["f16_law"] / cds["f32_law"] - 1), 4)
     net.destroy()
