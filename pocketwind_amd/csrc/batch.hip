@@ -86,6 +86,25 @@ __global__ __launch_bounds__(kForceBlock) void batch_force_kernel(const Tunnel* 
   }
 }
 
+// The learned closure of every tunnel in one launch: one wavefront per 16-cell tile of a
+// tunnel's wall-cell list (ctile_tunnel / ctile_first map a tile to its tunnel and the tunnel's
+// first tile, as the step's tile tables do), on the set of parity `parity`, into the tunnel's
+// closure buffer. Tunnels that evaluate the closure inside the step skip their tiles. A tile's
+// stresses are the single solver's closure kernel's bit for bit (closure::closure_tile).
+template <class S, bool BOUZIDI>
+__global__ __launch_bounds__(64) void batch_closure_kernel(const Tunnel* __restrict__ tunnels,
+                                                           const uint32_t* __restrict__ ctile_tunnel,
+                                                           const uint32_t* __restrict__ ctile_first, int parity,
+                                                           const mlp::Packed* __restrict__ P, closure::Norm norm) {
+  __shared__ __attribute__((aligned(16))) char lds[closure::kFusedWaveBytes];
+  const uint32_t t = ctile_tunnel[blockIdx.x];
+  const Tunnel& tu = tunnels[t];
+  if (tu.p.closure_net) return;
+  closure::closure_tile<S, BOUZIDI>((const typename S::T*)tu.pops[parity], tu.geo, tu.p,
+                                    static_cast<const closure::WallGeo*>(tu.wall_geo), tu.n_wall, P, norm,
+                                    tu.closure_buf, nullptr, blockIdx.x - ctile_first[t], lds);
+}
+
 // One block per tunnel reduces that tunnel's partials in the single-solver order.
 __global__ __launch_bounds__(kForceBlock) void batch_reduce_kernel(const Tunnel* __restrict__ tunnels,
                                                                    const double* __restrict__ partials,
@@ -145,6 +164,11 @@ struct pw_batch {
   uint32_t closure_placement = 0;  // pw_closure_placement
   int closure_fresh = 0;
   std::vector<std::pair<const uint32_t*, float4*>> closure_bufs;  // per tunnel: slot table, output
+  // The closure kernel's tiles (16 wall cells each) over all tunnels: tunnel of each tile, first
+  // tile of each tunnel.
+  uint32_t* ctile_tunnel = nullptr;
+  uint32_t* ctile_first = nullptr;
+  uint32_t n_ctiles = 0;
   bool closure_kernels = false;  // some tunnel evaluates the closure in the closure kernel
   uint32_t lds = 0;              // dynamic LDS per step block (closure evaluated in the step)
 };
@@ -181,12 +205,22 @@ void launch_init(pw_batch* b) {
 }
 
 // The closure of every tunnel on the state the next step reads.
+template <class S>
+void launch_closures_s(pw_batch* b) {
+  closure::Norm norm;
+  std::memcpy(&norm, b->closure_norm, sizeof norm);
+  auto k = b->params.walls == PW_WALLS_BOUZIDI ? batch_closure_kernel<S, true> : batch_closure_kernel<S, false>;
+  hipLaunchKernelGGL(k, dim3(b->n_ctiles), dim3(64), 0, b->stream, b->tunnels, b->ctile_tunnel, b->ctile_first,
+                     b->parity, static_cast<const mlp::Packed*>(b->closure_net), norm);
+}
+
 void launch_closures(pw_batch* b) {
-  for (const Tunnel& tu : b->host)
-    if (!tu.p.closure_net)
-      launch_closure_tunnel(b->params.storage, b->params.walls == PW_WALLS_BOUZIDI, b->stream, tu.pops[b->parity],
-                          tu.geo, tu.p, tu.wall_geo, tu.n_wall, b->closure_net, b->closure_norm,
-                          b->closure_bufs[&tu - b->host.data()].second);
+  if (!b->n_ctiles) return;
+  switch (b->params.storage) {
+    case PW_STORAGE_F16: launch_closures_s<StoreF16>(b); break;
+    case PW_STORAGE_BF16: launch_closures_s<StoreBF16>(b); break;
+    default: launch_closures_s<StoreF32>(b); break;
+  }
 }
 
 void launch_force(pw_batch* b, double* dest) {
@@ -485,11 +519,13 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
       tu.wall_cells = wl;
       tu.n_wall = (uint32_t)walls.size();
       tu.wall_geo = records;  // written once the geometry is on the device
+      tu.closure_buf = tau;
       b->closure_bufs.push_back({sl, tau});  // given to the step once a network is attached
     } else {
       tu.wall_cells = nullptr;
       tu.n_wall = 0;
       tu.wall_geo = nullptr;
+      tu.closure_buf = nullptr;
     }
     uint32_t *geo = nullptr, *bnd = nullptr;
     float4* macro = nullptr;
@@ -536,9 +572,22 @@ pw_status pw_batch_create(uint32_t count, const pw_layout* layouts, const uint32
   // One block per tile: a grid the device schedules as blocks retire (a persistent grid larger
   // than the resident capacity would leave its last blocks to run after the others).
   b->grid = (unsigned)std::min<uint64_t>(b->n_tiles, 0x7fffffffu);
-  if (shared->wall_model == PW_WALL_LEARNED &&
-      (e = dev_alloc(b, reinterpret_cast<char**>(&b->closure_net), closure::kNetBlockBytes)) != hipSuccess)
-    return fail(e, "batch closure network");
+  if (shared->wall_model == PW_WALL_LEARNED) {
+    if ((e = dev_alloc(b, reinterpret_cast<char**>(&b->closure_net), closure::kNetBlockBytes)) != hipSuccess)
+      return fail(e, "batch closure network");
+    std::vector<uint32_t> ct, cf(count);
+    for (uint32_t t = 0; t < count; ++t) {
+      cf[t] = (uint32_t)ct.size();
+      for (uint32_t k = 0; k < b->host[t].n_wall; k += closure::kCellsPerWave) ct.push_back(t);
+    }
+    b->n_ctiles = (uint32_t)ct.size();
+    if ((e = dev_alloc(b, &b->ctile_tunnel, ct.size() ? ct.size() : 1)) != hipSuccess ||
+        (e = dev_alloc(b, &b->ctile_first, count)) != hipSuccess)
+      return fail(e, "batch closure tables");
+    if ((ct.size() && (e = hipMemcpyAsync(b->ctile_tunnel, ct.data(), ct.size() * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess) ||
+        (e = hipMemcpyAsync(b->ctile_first, cf.data(), count * 4, hipMemcpyHostToDevice, b->stream)) != hipSuccess)
+      return fail(e, "batch closure tables");
+  }
   launch_init(b);
   // PW_BATCH_KERNEL=<cell|cell-nt|cell-nt-stores>/<chunks per XCD> fixes the step variant;
   // otherwise each is timed on these tunnels and the fastest kept. Timing leaves the state

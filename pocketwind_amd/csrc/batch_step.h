@@ -23,6 +23,7 @@ struct Tunnel {
   const uint32_t* wall_cells;  // learned closure: fluid cells with wall links, in cell order
   uint32_t n_wall;
   const void* wall_geo;  // their closure::WallGeo records
+  float4* closure_buf;   // the closure kernel's output for them (the step reads it as p.closure_tau)
 };
 
 struct BatchLaunch {

@@ -99,27 +99,6 @@ pw_status check(pw_solver* s) {
 
 }  // namespace
 
-template <class S>
-void launch_closure_tunnel_s(bool bz, hipStream_t stream, const void* state, const uint32_t* geo, const StepParams& p,
-                             const void* walls, uint32_t n_wall, const void* net, const float* norm27, float4* tau) {
-  closure::Norm norm;
-  std::memcpy(&norm, norm27, sizeof norm);
-  auto k = bz ? closure::closure_kernel<S, true> : closure::closure_kernel<S, false>;
-  hipLaunchKernelGGL(k, dim3(closure_blocks(n_wall)), dim3(64), 0, stream, (const typename S::T*)state, geo, p,
-                     (const closure::WallGeo*)walls, n_wall, (const mlp::Packed*)net, norm, tau, nullptr);
-}
-
-void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, const void* state,
-                           const uint32_t* geo, const StepParams& p, const void* walls, uint32_t n_wall,
-                           const void* net, const float* norm, float4* tau) {
-  if (n_wall == 0) return;
-  switch (storage) {
-    case PW_STORAGE_F16: launch_closure_tunnel_s<StoreF16>(bouzidi, stream, state, geo, p, walls, n_wall, net, norm, tau); break;
-    case PW_STORAGE_BF16: launch_closure_tunnel_s<StoreBF16>(bouzidi, stream, state, geo, p, walls, n_wall, net, norm, tau); break;
-    default: launch_closure_tunnel_s<StoreF32>(bouzidi, stream, state, geo, p, walls, n_wall, net, norm, tau); break;
-  }
-}
-
 void launch_wall_geo(hipStream_t stream, const uint32_t* geo, const StepParams& p, const uint32_t* wall_cells,
                      uint32_t n_wall, void* out) {
   if (n_wall == 0) return;

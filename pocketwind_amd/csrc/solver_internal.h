@@ -134,11 +134,6 @@ void launch_sources(pw_solver* s, void* set);
 // for the solver's current block size and XCD chunking. Allocates; called when a network is
 // attached and when the step variant changes.
 hipError_t closure_tile_order(pw_solver* s);
-// The closure kernel over one tunnel's wall-cell list (batched tunnels): `norm` holds the 27
-// normalisation floats, `net` the packed network.
-void launch_closure_tunnel(uint32_t storage, bool bouzidi, hipStream_t stream, const void* state,
-                           const uint32_t* geo, const StepParams& p, const void* walls, uint32_t n_wall,
-                           const void* net, const float* norm, float4* tau);
 // Bytes of one wall cell's static closure geometry record, and the kernel that writes the
 // records of the listed wall cells from the packed geometry.
 constexpr size_t kWallGeoBytes = 48;
