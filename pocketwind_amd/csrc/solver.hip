@@ -5,6 +5,8 @@
 // slot that is skipped while occupied and returns busy when empty).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -341,47 +343,61 @@ bool forced_variant(pw_solver* s, const char* text) {
 hipError_t autotune(pw_solver* s) {
   const uint32_t kernels[3] = {PW_KERNEL_CELL, PW_KERNEL_CELL_NT, PW_KERNEL_CELL_NT_STORES};
   const uint32_t blocks[3] = {64, 128, 256};
-  // Each candidate is timed twice over `reps` steps and keeps its faster run, so one slow
-  // sample does not decide.
+  struct Candidate {
+    float ms;
+    uint32_t kernel, block, chunks;
+  };
+  // The fastest of `rounds` timed runs of `reps` steps each, after two warm-up steps.
+  auto time_variant = [&](const Candidate& c, int rounds, int reps, float* fastest) -> hipError_t {
+    s->kernel = c.kernel;
+    s->block = c.block;
+    s->xcd_chunks = c.chunks;
+    for (int w = 0; w < 2; ++w) {
+      launch_step(s, false, s->sp);
+      s->current ^= 1;
+    }
+    for (int round = 0; round < rounds; ++round) {
+      hipError_t e = hipEventRecord(s->t0, s->stream);
+      if (e != hipSuccess) return e;
+      for (int r = 0; r < reps; ++r) {
+        launch_step(s, false, s->sp);
+        s->current ^= 1;
+      }
+      if ((e = hipEventRecord(s->t1, s->stream)) != hipSuccess) return e;
+      if ((e = hipEventSynchronize(s->t1)) != hipSuccess) return e;
+      float ms = 0;
+      if ((e = hipEventElapsedTime(&ms, s->t0, s->t1)) != hipSuccess) return e;
+      if (round == 0 || ms < *fastest) *fastest = ms;
+    }
+    return hipSuccess;
+  };
+  // A screen of every candidate (each timed twice over `reps` steps, the faster run kept), then
+  // the four fastest timed again over four times as many steps, three times: the variants lie
+  // within a few percent of each other, and one short sample can misorder them.
   const int reps = s->cells >= (1u << 20) ? 12 : 2;
-  float best = 0;
-  uint32_t choice = PW_KERNEL_CELL_NT, choice_block = 256, choice_chunks = 1;
+  std::vector<Candidate> screened;
   for (uint32_t parts : kXcdChunkings)
     for (uint32_t k : kernels)
       for (uint32_t b : blocks) {
         // A chunking with fewer than two blocks per chunk would only add empty blocks.
         if (parts > 1 && (uint64_t)parts * 2 * kXcds * b > s->cells) continue;
-        s->kernel = k;
-        s->block = b;
-        s->xcd_chunks = parts;
-        for (int w = 0; w < 2; ++w) {  // warm-up
-          launch_step(s, false, s->sp);
-          s->current ^= 1;
-        }
-        float fastest = 0;
-        for (int round = 0; round < 2; ++round) {
-          hipError_t e = hipEventRecord(s->t0, s->stream);
-          if (e != hipSuccess) return e;
-          for (int r = 0; r < reps; ++r) {
-            launch_step(s, false, s->sp);
-            s->current ^= 1;
-          }
-          if ((e = hipEventRecord(s->t1, s->stream)) != hipSuccess) return e;
-          if ((e = hipEventSynchronize(s->t1)) != hipSuccess) return e;
-          float ms = 0;
-          if ((e = hipEventElapsedTime(&ms, s->t0, s->t1)) != hipSuccess) return e;
-          if (round == 0 || ms < fastest) fastest = ms;
-        }
-        if (best == 0 || fastest < best) {
-          best = fastest;
-          choice = k;
-          choice_block = b;
-          choice_chunks = parts;
-        }
+        Candidate c{0.0f, k, b, parts};
+        if (hipError_t e = time_variant(c, 2, reps, &c.ms)) return e;
+        screened.push_back(c);
       }
-  s->kernel = choice;
-  s->block = choice_block;
-  s->xcd_chunks = choice_chunks;
+  std::sort(screened.begin(), screened.end(), [](const Candidate& x, const Candidate& y) { return x.ms < y.ms; });
+  Candidate best = screened.front();
+  if (s->cells >= (1u << 20)) {
+    best.ms = 0.0f;
+    for (size_t i = 0; i < screened.size() && i < 4; ++i) {
+      Candidate c = screened[i];
+      if (hipError_t e = time_variant(c, 3, 4 * reps, &c.ms)) return e;
+      if (best.ms == 0.0f || c.ms < best.ms) best = c;
+    }
+  }
+  s->kernel = best.kernel;
+  s->block = best.block;
+  s->xcd_chunks = best.chunks;
   s->current = 0;
   return hipGetLastError();
 }
