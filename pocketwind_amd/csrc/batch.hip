@@ -370,8 +370,39 @@ hipError_t tune_batch(pw_batch* b) {
     return n;
   }();
   const int reps = cells >= (1u << 22) ? 4 : 16;
-  float best = 0;
-  uint32_t best_policy = kNtStores, best_chunks = 16, best_persistent = 0, best_block = kBlock;
+  struct Candidate {
+    float ms;
+    uint32_t block, chunks, policy, persistent;
+  };
+  // The fastest of `rounds` timed runs of `n` steps each, after two warm-up steps (the tile
+  // tables for the candidate's block are in place).
+  auto time_variant = [&](Candidate& c, int rounds, int n) -> hipError_t {
+    b->policy = c.policy;
+    b->xcd_chunks = c.chunks;
+    b->persistent = c.persistent;
+    b->step_grid = batch_grid(b->n_tiles, c.chunks, &b->chunk);
+    for (int w = 0; w < 2; ++w) {
+      launch_step(b, b->parity, false);
+      b->parity ^= 1;
+    }
+    for (int round = 0; round < rounds; ++round) {
+      hipError_t e = hipEventRecord(b->t0, b->stream);
+      if (e != hipSuccess) return e;
+      for (int r = 0; r < n; ++r) {
+        launch_step(b, b->parity, false);
+        b->parity ^= 1;
+      }
+      if ((e = hipEventRecord(b->t1, b->stream)) != hipSuccess) return e;
+      if ((e = hipEventSynchronize(b->t1)) != hipSuccess) return e;
+      float ms = 0;
+      if ((e = hipEventElapsedTime(&ms, b->t0, b->t1)) != hipSuccess) return e;
+      if (round == 0 || ms < c.ms) c.ms = ms;
+    }
+    return hipSuccess;
+  };
+  // A screen of every candidate (two runs of `reps` steps), then the four fastest timed again
+  // over four times as many steps, three times, as the single solver's tuner does.
+  std::vector<Candidate> screened;
   for (uint32_t block : kBatchBlocks) {
     if (hipError_t e = set_block(b, block)) return e;
     const uint32_t resident = persistent_grid(b);
@@ -380,41 +411,25 @@ hipError_t tune_batch(pw_batch* b) {
         for (uint32_t persist : {0u, resident}) {
           if (chunks > 1 && (uint64_t)chunks * 2 * kXcds > b->n_tiles) continue;
           if (persist && (persist >= batch_grid(b->n_tiles, chunks, &b->chunk) || resident == 0)) continue;
-          b->policy = pol;
-          b->xcd_chunks = chunks;
-          b->persistent = persist;
-          b->step_grid = batch_grid(b->n_tiles, chunks, &b->chunk);
-          for (int w = 0; w < 2; ++w) {
-            launch_step(b, b->parity, false);
-            b->parity ^= 1;
-          }
-          float fastest = 0;
-          for (int round = 0; round < 2; ++round) {
-            hipError_t e = hipEventRecord(b->t0, b->stream);
-            if (e != hipSuccess) return e;
-            for (int r = 0; r < reps; ++r) {
-              launch_step(b, b->parity, false);
-              b->parity ^= 1;
-            }
-            if ((e = hipEventRecord(b->t1, b->stream)) != hipSuccess) return e;
-            if ((e = hipEventSynchronize(b->t1)) != hipSuccess) return e;
-            float ms = 0;
-            if ((e = hipEventElapsedTime(&ms, b->t0, b->t1)) != hipSuccess) return e;
-            if (round == 0 || ms < fastest) fastest = ms;
-          }
-          if (best == 0 || fastest < best) {
-            best = fastest;
-            best_policy = pol;
-            best_chunks = chunks;
-            best_persistent = persist;
-            best_block = block;
-          }
+          Candidate c{0.0f, block, chunks, pol, persist};
+          if (hipError_t e = time_variant(c, 2, reps)) return e;
+          screened.push_back(c);
         }
   }
-  if (hipError_t e = set_block(b, best_block)) return e;
-  b->policy = best_policy;
-  b->xcd_chunks = best_chunks;
-  b->persistent = best_persistent;
+  std::sort(screened.begin(), screened.end(), [](const Candidate& x, const Candidate& y) { return x.ms < y.ms; });
+  Candidate best = screened.front();
+  best.ms = 0.0f;
+  for (size_t i = 0; i < screened.size() && i < 4; ++i) {
+    Candidate c = screened[i];
+    c.ms = 0.0f;
+    if (hipError_t e = set_block(b, c.block)) return e;
+    if (hipError_t e = time_variant(c, 3, 4 * reps)) return e;
+    if (best.ms == 0.0f || c.ms < best.ms) best = c;
+  }
+  if (hipError_t e = set_block(b, best.block)) return e;
+  b->policy = best.policy;
+  b->xcd_chunks = best.chunks;
+  b->persistent = best.persistent;
   return hipGetLastError();
 }
 
