@@ -363,7 +363,9 @@ void pw_batch_destroy(pw_batch *b);
  * same closure. */
 pw_status pw_batch_set_closure(pw_batch *b, const pw_mlp *net, const float *norm, size_t n_norm);
 pw_status pw_batch_set_closure_interval(pw_batch *b, uint32_t every);
-/* pw_solver_set_closure_placement for every tunnel of the batch (row tunnels only). */
+/* Closure placement of every tunnel of the batch: PW_CLOSURE_KERNEL (one launch over all the
+ * tunnels' wall cells) or PW_CLOSURE_IN_STEP (row tunnels; the others keep the kernel); other
+ * values are rejected. */
 pw_status pw_batch_set_closure_placement(pw_batch *b, uint32_t placement);
 /* The momentum-exchange force of every tunnel now (fx, fy, fz per tunnel, FP64; the mean over
  * both states as a force sample), waiting for the batch's queued work. */
