@@ -123,6 +123,9 @@ struct StoreF32 : HashedWords {
   static constexpr int kBytes = 4;
   static constexpr int kShift = 2;
   static constexpr int kScaleLog2 = 0;  // stored value = g * 2^kScaleLog2
+  // The REG collision in its regrouped form (step_kernels.h collide_reg_regrouped) rather than
+  // through the per-direction equilibrium.
+  static constexpr bool kRegroupedReg = false;
   __host__ __device__ __forceinline__ static float decode(T v) { return v; }
   __host__ __device__ __forceinline__ static float load(const T* p) { return *p; }
   __host__ __device__ __forceinline__ static T encode(float g, uint32_t) { return g; }
@@ -137,6 +140,7 @@ struct StoreF16 : HashedWords {
   static constexpr int kBytes = 2;
   static constexpr int kShift = 1;
   static constexpr int kScaleLog2 = 15;
+  static constexpr bool kRegroupedReg = true;
   static constexpr float kScale = 32768.0f;
   // An fma with a zero addend (one v_fma_mix_f32 from the half operand) rather than a conversion
   // and a multiply; the scale is a power of two, so the result is the same exact product.
@@ -181,6 +185,9 @@ __host__ __device__ __forceinline__ _Float16 round_f16_reference(float v, uint32
 }
 
 struct StoreF16Reference : StoreF16 {
+  // The same arithmetic as f32 storage, so its stored bits compare with the reference's encoding
+  // of the f32 solver's values.
+  static constexpr bool kRegroupedReg = false;
   // The words carry the cell and direction only (prelude-f16.wgsl:10); the value's bits join at
   // the rounding, so the seed (the step) takes no part.
   __host__ __device__ __forceinline__ static uint32_t cell_word(uint32_t n, uint32_t) { return n * 0x9e3779b9u; }
@@ -203,6 +210,7 @@ struct StoreBF16 : HashedWords {
   static constexpr int kBytes = 2;
   static constexpr int kShift = 1;
   static constexpr int kScaleLog2 = 0;
+  static constexpr bool kRegroupedReg = false;
   __host__ __device__ __forceinline__ static float decode(T v) { return as_float(static_cast<uint32_t>(v) << 16); }
   __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
   __host__ __device__ __forceinline__ static T encode(float g, uint32_t r) { return round_bf16_sr(g, r); }

@@ -216,7 +216,58 @@ __host__ __device__ __forceinline__ float projected(int i, const Stress& s, floa
 // equilibrium: the state of an EQ cell, or the rest state of a wall.
 // c holds the equilibrium scaled by 2^SC (cell_equilibrium<SC>), and g receives the state at the
 // same scale; the TRT relaxation works on the gathered values and takes SC = 0.
-template <int OP, bool LES, int SC = 0>
+// The REG collision regrouped so that it needs only the cell's moments (cell_moments), not its
+// per-direction equilibrium: with M = rho u u + (1 - omega) Pi, direction i leaves with
+//   w_i (rho - 1) + 4.5 w_i (c_i c_i - I/3) : M  +-  3 w_i c_i . (rho u)
+// (a pair shares the even part and flips the odd one): collide-reg.wgsl:3-18 with
+// rho (4.5 (c.u)^2 - 1.5 u.u) = 4.5 (c c - I/3) : rho u u. It takes fewer vector instructions
+// than forming the nine pairs' equilibrium first: the f16 step on the 256x128x128 body ran
+// 0.7-0.9 us faster (same-box comparisons). Its f32 roundings differ from the equilibrium form's by about an ulp, far below the f16
+// store's 11 bits, so the f16 storage with the library's rounding takes it (S::kRegroupedReg);
+// f32 storage keeps the equilibrium form, whose density the FP64-oracle parity holds to three f32
+// ulps on a transient that amplifies either form's roundings (this form lands 3.8 there), and so
+// does the f16 storage with the reference's rounding, whose stored bits are pinned against the
+// reference's encoding of the f32 step's values.
+template <int SC>
+__host__ __device__ __forceinline__ void collide_reg_regrouped(float* g, const CellState& c, const Stress& s,
+                                                               float wp) {
+  const float om = 1.0f - wp;
+  const float rx = c.rho * c.ux, ry = c.rho * c.uy, rz = c.rho * c.uz;
+  const float mxx = rx * c.ux + om * s.xx, myy = ry * c.uy + om * s.yy, mzz = rz * c.uz + om * s.zz;
+  const float mxy = rx * c.uy + om * s.xy, mxz = rx * c.uz + om * s.xz, myz = ry * c.uz + om * s.yz;
+  const float t3 = (mxx + myy + mzz) * (1.0f / 3.0f);
+  g[0] = (wf(0) * kOut<SC>) * c.rd - (4.5f * wf(0) * kOut<SC>) * t3;
+  const float e1 = (wf(1) * kOut<SC>) * c.rd, e2 = (wf(7) * kOut<SC>) * c.rd;
+  constexpr float w1 = 4.5f * wf(1) * kOut<SC>, w2 = 4.5f * wf(7) * kOut<SC>;
+  constexpr float o1 = 3.0f * wf(1) * kOut<SC>, o2 = 3.0f * wf(7) * kOut<SC>;
+  const float pxy = mxx + myy - t3, pxz = mxx + mzz - t3, pyz = myy + mzz - t3;
+#pragma unroll
+  for (int q = 0; q < kPairs; ++q) {
+    const int i = 2 * q + 1;
+    float even, odd;
+    if (wf(i) == wf(1)) {
+      even = e1 + w1 * ((cx(i) ? mxx : cy(i) ? myy : mzz) - t3);
+      odd = o1 * cdot(i, rx, ry, rz);
+    } else {
+      // (c_i c_i - I/3) : M for (+-1, +-1) in one plane: the plane's diagonal sum less the trace
+      // third, + 2 c_a c_b m_ab.
+      const float sum = cz(i) == 0 ? pxy : cy(i) == 0 ? pxz : pyz;
+      const float m = cz(i) == 0 ? mxy : cy(i) == 0 ? mxz : myz;
+      const int sign = cz(i) == 0 ? cx(i) * cy(i) : cy(i) == 0 ? cx(i) * cz(i) : cy(i) * cz(i);
+      even = e2 + w2 * (sign > 0 ? sum + 2.0f * m : sum - 2.0f * m);
+      odd = o2 * cdot(i, rx, ry, rz);
+    }
+    g[i] = even + odd;
+    g[i + 1] = even - odd;
+  }
+}
+
+// Whether a collision reads the per-direction equilibrium (cell_equilibrium): TRT, and REG in
+// its equilibrium form.
+template <int OP, bool REGROUPED>
+constexpr bool kNeedsEquilibrium = !(OP == kOpReg && REGROUPED);
+
+template <int OP, bool LES, int SC = 0, bool REGROUPED = false>
 __host__ __device__ __forceinline__ void collide(const StepParams& p, float* g, const CellState& c,
                                                  bool prescribed = false) {
   static_assert(OP == kOpReg || SC == 0, "the TRT relaxation works unscaled");
@@ -225,7 +276,9 @@ __host__ __device__ __forceinline__ void collide(const StepParams& p, float* g, 
   if (OP == kOpReg && prescribed) s = Stress{0, 0, 0, 0, 0, 0};
   const float tau = LES ? les_tau(p, s, c.inv_rho) : p.tau0;
   const float wp = LES ? fast_rcp(tau) : p.inv_tau0;
-  if (OP == kOpReg) {
+  if (OP == kOpReg && REGROUPED) {
+    collide_reg_regrouped<SC>(g, c, s, wp);
+  } else if (OP == kOpReg) {
     const float third = (s.xx + s.yy + s.zz) * (1.0f / 3.0f);
     const float k = (1.0f - wp) * 4.5f;
     g[0] = c.eq0 - k * (wf(0) * kOut<SC>) * third;
@@ -263,7 +316,7 @@ __host__ __device__ __forceinline__ float friction_velocity(float nu, float spee
 // flux along the slip is the wall-law shear; falls back to the bulk collision.
 // `learned`: the learned closure's wall shear stress vector for this cell (w > 0 when valid),
 // used in place of the wall law; w = 0 keeps the law.
-template <int OP, int SC = 0>
+template <int OP, int SC = 0, bool REGROUPED = false>
 __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float* g, const CellState& c,
                                                      uint32_t flag, const float4& learned) {
   float nx = 0, ny = 0, nz = 0, exposed = 0;
@@ -287,7 +340,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
   }
   const float speed = sqrtf(sx * sx + sy * sy + sz * sz);
   if (speed < p.slip_floor) {  // stream_collide.wgsl:40 (1e-7 with f32 storage)
-    collide<OP, true, SC>(p, g, c);
+    collide<OP, true, SC, REGROUPED>(p, g, c);
     return;
   }
   const float tx = sx / speed, ty = sy / speed, tz = sz / speed;
@@ -323,7 +376,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
   }
   const float norm = b.xx * b.xx + b.yy * b.yy + b.zz * b.zz + 2.0f * (b.xy * b.xy + b.xz * b.xz + b.yz * b.yz);
   if (norm < 0.05f) {
-    collide<OP, true, SC>(p, g, c);
+    collide<OP, true, SC, REGROUPED>(p, g, c);
     return;
   }
   const float scale = 4.5f * (shear - flux) / norm;
@@ -336,7 +389,7 @@ __host__ __device__ __forceinline__ void collide_wall(const StepParams& p, float
     admissible = admissible && (post[i] + wf(i) * kOut<SC> >= 0.0f);
   }
   if (!admissible) {
-    collide<OP, true, SC>(p, g, c);
+    collide<OP, true, SC, REGROUPED>(p, g, c);
     return;
   }
 #pragma unroll
@@ -625,18 +678,21 @@ __host__ __device__ __forceinline__ float4 relax(const StepParams& p, float* g, 
 // lane (EQ or wall) takes unit density and the velocity (ux, uy, uz) as its moments and zero
 // non-equilibrium stress, so it leaves with that equilibrium (a wall with the rest state, exactly
 // zero) through the same arithmetic as every fluid lane instead of a separate branch.
-template <int OP, bool LES, int SC = 0>
+template <int OP, bool LES, int SC = 0, bool REGROUPED = false>
 __host__ __device__ __forceinline__ float4 relax_prescribed(const StepParams& p, float* g, uint32_t flag,
                                                             const float4& learned, bool prescribed, float ux,
                                                             float uy, float uz) {
   CellState c;
   cell_moments(g, c);
   if (prescribed) prescribe_moments(c, ux, uy, uz);
-  cell_equilibrium<SC>(c);
-  if (LES && (flag >> 3) != 0)
-    collide_wall<OP, SC>(p, g, c, flag, learned);
-  else
-    collide<OP, LES, SC>(p, g, c, prescribed);
+  // The wall collision reads the per-direction equilibrium; the regrouped REG collision does not.
+  if (LES && (flag >> 3) != 0) {
+    cell_equilibrium<SC>(c);
+    collide_wall<OP, SC, REGROUPED>(p, g, c, flag, learned);
+  } else {
+    if (kNeedsEquilibrium<OP, REGROUPED>) cell_equilibrium<SC>(c);
+    collide<OP, LES, SC, REGROUPED>(p, g, c, prescribed);
+  }
   return make_float4(c.rho, c.ux, c.uy, c.uz);
 }
 
@@ -839,7 +895,8 @@ __host__ __device__ __forceinline__ void stream_collide_cell(const typename S::T
       }
       eq_velocity(p, x, up, ux, uy, uz);
     }
-    m = relax_prescribed<OP, LES, kOutLog2<S, OP>>(p, g, flag, LES ? buffered_stress(p, flag, n) : float4{},
+    m = relax_prescribed<OP, LES, kOutLog2<S, OP>, S::kRegroupedReg>(p, g, flag,
+                                                                    LES ? buffered_stress(p, flag, n) : float4{},
                                                    kind != kFluid, ux, uy, uz);
   } else if (kind == kEq) {
     if constexpr (EDGE) {

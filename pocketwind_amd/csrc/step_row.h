@@ -229,7 +229,7 @@ __device__ __forceinline__ void stream_collide_row_wave(const typename S::T* __r
     const bool prescribed = kind != kFluid;
     float ux = 0.0f, uy = 0.0f, uz = 0.0f;
     if (kind == kEq) eq_velocity(p, x, up, ux, uy, uz);
-    m = relax_prescribed<OP, LES, kOutLog2<S, OP>>(p, g, flag, learned, prescribed, ux, uy, uz);
+    m = relax_prescribed<OP, LES, kOutLog2<S, OP>, S::kRegroupedReg>(p, g, flag, learned, prescribed, ux, uy, uz);
   } else if (kind == kFluid) {
     m = relax<OP, LES>(p, g, flag, learned);
   } else if (kind == kEq) {
