@@ -210,7 +210,7 @@ struct StoreBF16 : HashedWords {
   static constexpr int kBytes = 2;
   static constexpr int kShift = 1;
   static constexpr int kScaleLog2 = 0;
-  static constexpr bool kRegroupedReg = false;
+  static constexpr bool kRegroupedReg = true;
   __host__ __device__ __forceinline__ static float decode(T v) { return as_float(static_cast<uint32_t>(v) << 16); }
   __host__ __device__ __forceinline__ static float load(const T* p) { return decode(*p); }
   __host__ __device__ __forceinline__ static T encode(float g, uint32_t r) { return round_bf16_sr(g, r); }

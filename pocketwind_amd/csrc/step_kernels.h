@@ -222,12 +222,13 @@ __host__ __device__ __forceinline__ float projected(int i, const Stress& s, floa
 // (a pair shares the even part and flips the odd one): collide-reg.wgsl:3-18 with
 // rho (4.5 (c.u)^2 - 1.5 u.u) = 4.5 (c c - I/3) : rho u u. It takes fewer vector instructions
 // than forming the nine pairs' equilibrium first: the f16 step on the 256x128x128 body ran
-// 0.7-0.9 us faster (same-box comparisons). Its f32 roundings differ from the equilibrium form's by about an ulp, far below the f16
-// store's 11 bits, so the f16 storage with the library's rounding takes it (S::kRegroupedReg);
-// f32 storage keeps the equilibrium form, whose density the FP64-oracle parity holds to three f32
-// ulps on a transient that amplifies either form's roundings (this form lands 3.8 there), and so
-// does the f16 storage with the reference's rounding, whose stored bits are pinned against the
-// reference's encoding of the f32 step's values.
+// 0.7-0.9 us faster (same-box comparisons). Its f32 roundings differ from the equilibrium
+// form's by about an ulp, far below the 16-bit stores' 11 (f16) or 8 (bf16) bits, so the 16-bit
+// storages with the library's rounding take it (S::kRegroupedReg). f32 storage keeps the
+// equilibrium form, whose density the FP64-oracle parity holds to three f32 ulps on a transient
+// that amplifies either form's roundings (this form lands 3.8 there), and so does the f16
+// storage with the reference's rounding, whose stored bits are pinned against the reference's
+// encoding of the f32 step's values.
 template <int SC>
 __host__ __device__ __forceinline__ void collide_reg_regrouped(float* g, const CellState& c, const Stress& s,
                                                                float wp) {
