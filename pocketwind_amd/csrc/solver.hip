@@ -372,8 +372,10 @@ hipError_t autotune(pw_solver* s) {
     return hipSuccess;
   };
   // A screen of every candidate (each timed twice over `reps` steps, the faster run kept), then
-  // the four fastest timed again over four times as many steps, three times: the variants lie
-  // within a few percent of each other, and one short sample can misorder them.
+  // the four fastest timed again over eight times as many steps in four interleaved rounds (each
+  // round times every finalist once, so a drift of the clock or of the memory affects them alike),
+  // the fastest run of each counting: the variants lie within a few percent of each other, and
+  // one short sample can misorder them.
   const int reps = s->cells >= (1u << 20) ? 12 : 2;
   std::vector<Candidate> screened;
   for (uint32_t parts : kXcdChunkings)
@@ -388,12 +390,19 @@ hipError_t autotune(pw_solver* s) {
   std::sort(screened.begin(), screened.end(), [](const Candidate& x, const Candidate& y) { return x.ms < y.ms; });
   Candidate best = screened.front();
   if (s->cells >= (1u << 20)) {
-    best.ms = 0.0f;
-    for (size_t i = 0; i < screened.size() && i < 4; ++i) {
-      Candidate c = screened[i];
-      if (hipError_t e = time_variant(c, 3, 4 * reps, &c.ms)) return e;
-      if (best.ms == 0.0f || c.ms < best.ms) best = c;
-    }
+    std::vector<Candidate> finalists(screened.begin(), screened.begin() + std::min<size_t>(4, screened.size()));
+    std::vector<float> fastest(finalists.size(), 0.0f);
+    for (int round = 0; round < 4; ++round)
+      for (size_t i = 0; i < finalists.size(); ++i) {
+        float ms = 0.0f;
+        if (hipError_t e = time_variant(finalists[i], 1, 8 * reps, &ms)) return e;
+        if (round == 0 || ms < fastest[i]) fastest[i] = ms;
+      }
+    for (size_t i = 0; i < finalists.size(); ++i)
+      if (i == 0 || fastest[i] < best.ms) {
+        best = finalists[i];
+        best.ms = fastest[i];
+      }
   }
   s->kernel = best.kernel;
   s->block = best.block;
